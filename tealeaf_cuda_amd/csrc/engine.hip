@@ -1,0 +1,1471 @@
+// engine.hip -- TeaLeaf kernel layer for MI355X (gfx950): the C-ABI of
+// include/tea_leaf_engine.h over hand-written HIP kernels.
+//
+// Replaces init_cuda.cu, tea_leaf_kernel_cuda.cu and the *_kernel_cuda.cu files
+// of the CUDA reference.  One chunk, one GPU, one HIP stream.  All arithmetic
+// is FP64 in the expression order of the reference kernels (kernel_files/
+// tea_leaf_*.cuknl); the build disables fused multiply-add contraction so the
+// results equal a plain-C evaluation of the same expressions bit for bit.
+//
+// Field layout in HBM: every field is one (ny+2h) x pitch array of doubles,
+// row-major with x fastest (the reference's THARR2D order).  The interior's
+// first column sits on a 128-byte boundary and pitch is a multiple of 16
+// doubles, so every wavefront row access is a set of whole cache lines.  All
+// fields live in one allocation; kernels index them with one flat index
+// i = c + r*pitch on pointers pre-shifted by (lead - h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tea_leaf_engine.h"
+#include "engine_device.hpp"
+
+using namespace tea;
+
+#define EXPORT extern "C" __attribute__((visibility("default")))
+
+// =============================================================================
+// Host state
+// =============================================================================
+namespace {
+
+enum FieldId {
+    F_DENSITY = 1, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD, F_R,
+    X_U0, X_W, X_Z, X_MI, X_KX, X_KY, X_RTEMP, X_UTEMP, X_RSTORE, X_CP, X_BFP, X_VOLUME, X_COUNT
+};
+enum { SOLVER_JACOBI = 0, SOLVER_CG = 1, SOLVER_CHEBY = 2, SOLVER_PPCG = 3 };
+constexpr int S_CG_FAMILY = (1 << SOLVER_CG) | (1 << SOLVER_CHEBY) | (1 << SOLVER_PPCG);
+
+struct Profiler {
+    bool on = false;
+    struct Pending {
+        int name;
+        hipEvent_t a, b;
+    };
+    std::vector<std::string> names;
+    std::map<std::string, int> ids;
+    std::vector<long> counts;
+    std::vector<double> ms;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+
+    int id(const char *n)
+    {
+        auto it = ids.find(n);
+        if (it != ids.end()) return it->second;
+        int k = (int)names.size();
+        names.push_back(n);
+        ids[n] = k;
+        counts.push_back(0);
+        ms.push_back(0.0);
+        return k;
+    }
+    hipEvent_t get()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    // Folds every completed pair into the totals (called after synchronisation).
+    void drain(bool wait)
+    {
+        size_t k = 0;
+        for (; k < pending.size(); k++) {
+            Pending &p = pending[k];
+            if (!wait && hipEventQuery(p.b) != hipSuccess) break;
+            if (wait) (void)hipEventSynchronize(p.b);
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, p.a, p.b);
+            counts[p.name]++;
+            ms[p.name] += t;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.erase(pending.begin(), pending.begin() + (long)k);
+    }
+};
+
+struct Chunk {
+    bool initialised = false;
+    int nx = 0, ny = 0, h = 0;
+    int lead = 0;       // physical column of interior column 0
+    long pitch = 0;     // row stride in doubles
+    long rows = 0;
+    int solver = SOLVER_JACOBI;
+    int prec = PREC_NONE;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    double *slab = nullptr;         // all 2D fields
+    double *f[X_COUNT] = {nullptr};  // base of each field (physical column 0)
+    double *fs[X_COUNT] = {nullptr}; // shifted: fs[id][c + r*pitch] is cell (c, r)
+    double *cellx = nullptr, *celly = nullptr, *vertexx = nullptr, *vertexy = nullptr;
+    std::vector<double> h_celldx, h_celldy;
+
+    DD *partials = nullptr;
+    unsigned *counter = nullptr;
+    double *h_out = nullptr;         // pinned, MAX_RED doubles
+    double *d_out = nullptr;         // device alias of h_out
+    int max_blocks = 0;
+
+    double *d_halo_buf = nullptr;    // staging for pack / unpack
+    size_t halo_buf_elems = 0;
+
+    std::vector<double> ch_a, ch_b;  // host copies of the uploaded coefficients
+
+    Profiler prof;
+};
+
+Chunk g;
+std::string g_deck_path = "tea.in";
+int g_error_mode = 0;
+char g_error[1024] = "";
+
+bool die(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "tea_engine: %s\n", g_error);
+    fflush(stderr);
+    if (g_error_mode == 0) abort();
+    return false;
+}
+
+#define HIP_OK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) { die("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); return; } \
+    } while (0)
+#define HIP_OK_R(call, ret)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) { die("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); return ret; } \
+    } while (0)
+#define CHECK_INIT()                                                                              \
+    do { if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return; } } while (0)
+// Kernels are registered per solver in the reference (init_cuda.cu:251-282) and
+// launching an unregistered one throws there; the same calls are errors here.
+#define REQUIRE_SOLVER(mask)                                                                      \
+    do { if (!((mask) & (1 << g.solver))) { die("%s: kernel not registered for the solver selected in the deck", __func__); return; } } while (0)
+
+// ---- deck reading: init_cuda.cu:63-163, cuda_strings.cu:7-83 ----------------
+std::string match_param(const char *name)
+{
+    std::ifstream in(g_deck_path);
+    std::string line;
+    while (std::getline(in, line)) {
+        while (!line.empty() && (line.back() == '\r' || line.back() == '\n')) line.pop_back();
+        if (line.find('!') != std::string::npos) continue;
+        if (line.find(name) != std::string::npos) {
+            auto eq = line.find('=');
+            return eq != std::string::npos ? line.substr(eq + 1) : std::string(name);
+        }
+    }
+    return std::string();
+}
+bool param_enabled(const char *name) { return !match_param(name).empty(); }
+int read_int(const char *name)
+{
+    std::string v = match_param(name);
+    int x;
+    if (v.empty() || sscanf(v.c_str(), " %d", &x) != 1) return -1;
+    return x;
+}
+std::string read_string(const char *name)
+{
+    std::string v = match_param(name);
+    for (auto &ch : v) ch = (char)tolower((unsigned char)ch);
+    return v;
+}
+
+// ---- launch helpers ------------------------------------------------------------
+Ext interior() { return Ext{g.h, g.h + g.nx - 1, g.h, g.h + g.ny - 1}; }
+
+int blocks_for(const Ext &e)
+{
+    if (e.x1 < e.x0 || e.y1 < e.y0) return 0;
+    const int nbx = (e.x1 - e.x0 + TILE_X) / TILE_X;
+    const int nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
+    return nbx * nby;
+}
+
+Red red_ctx() { return Red{g.partials, g.counter, g.d_out, g.max_blocks}; }
+
+struct Timed {
+    int name = -1;
+    hipEvent_t b;
+    explicit Timed(const char *n)
+    {
+        if (!g.prof.on) return;
+        name = g.prof.id(n);
+        hipEvent_t a = g.prof.get();
+        b = g.prof.get();
+        (void)hipEventRecord(a, g.stream);
+        g.prof.pending.push_back({name, a, b});
+    }
+    ~Timed()
+    {
+        if (name >= 0) (void)hipEventRecord(b, g.stream);
+    }
+};
+
+template <typename K, typename... A>
+bool launch(const char *name, K kernel, const Ext &e, A... args)
+{
+    const int nb = blocks_for(e);
+    if (nb == 0) return true;
+    {
+        Timed t(name);
+        hipLaunchKernelGGL(kernel, dim3(nb), dim3(BLOCK), 0, g.stream, e, args...);
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
+    return true;
+}
+
+// Waits for the queued work and returns out[0..n) of the last reduction.
+bool fetch(double *dst, int n)
+{
+    hipError_t err = hipStreamSynchronize(g.stream);
+    if (err != hipSuccess) return die("hipStreamSynchronize failed: %s", hipGetErrorString(err));
+    for (int j = 0; j < n; j++) dst[j] = g.h_out[j];
+    if (g.prof.on) g.prof.drain(false);
+    return true;
+}
+
+}  // namespace
+
+// =============================================================================
+// Kernels
+// =============================================================================
+namespace {
+
+#define STRIP_BEGIN                                       \
+    const Strip s = strip_of_lane(e);                     \
+    const long i0 = (long)s.c + (long)s.r0 * pitch;       \
+    (void)i0;
+
+// ---- set-up ---------------------------------------------------------------------
+
+// initialise_chunk_kernel.cuknl:54-70 (volume only; areas are not read on this path)
+__global__ __launch_bounds__(BLOCK) void k_init_volume(Ext e, long pitch, double *__restrict__ vol, double v)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) vol[i0 + k * pitch] = v;
+}
+
+// generate_chunk_kernel.cuknl:85-99
+__global__ __launch_bounds__(BLOCK) void k_generate_init(Ext e, long pitch, double *__restrict__ dens,
+                                                         double *__restrict__ e0, double sd, double se)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) {
+        e0[i0 + k * pitch] = se;
+        dens[i0 + k * pitch] = sd;
+    }
+}
+
+// generate_chunk_kernel.cuknl:5-67 (one state)
+__global__ __launch_bounds__(BLOCK) void k_generate_state(Ext e, long pitch, const double *__restrict__ vertexx,
+                                                          const double *__restrict__ vertexy,
+                                                          const double *__restrict__ cellx,
+                                                          const double *__restrict__ celly,
+                                                          double *__restrict__ dens, double *__restrict__ e0,
+                                                          double sdens, double senergy, double sxmin, double sxmax,
+                                                          double symin, double symax, double sradius, int geom,
+                                                          int g_rect, int g_circ, int g_point)
+{
+    STRIP_BEGIN;
+    const double x_cent = sxmin, y_cent = symin;
+    for (int k = 0; k < s.len; k++) {
+        const int c = s.c, r = s.r0 + k;
+        bool hit = false;
+        if (g_rect == geom) {
+            hit = vertexx[1 + c] >= sxmin && vertexx[c] < sxmax && vertexy[1 + r] >= symin && vertexy[r] < symax;
+        } else if (geom == g_circ) {
+            const double x_pos = cellx[c] - x_cent;
+            const double y_pos = celly[r] - y_cent;
+            const double radius = sqrt(x_pos * x_pos + y_pos * y_pos);
+            hit = radius <= sradius;
+        } else if (geom == g_point) {
+            hit = vertexx[c] == x_cent && vertexy[r] == y_cent;
+        }
+        if (hit) {
+            e0[i0 + k * pitch] = senergy;
+            dens[i0 + k * pitch] = sdens;
+        }
+    }
+}
+
+// generate_chunk_kernel.cuknl:69-83: u = u0 = energy*density
+__global__ __launch_bounds__(BLOCK) void k_init_u(Ext e, long pitch, const double *__restrict__ dens,
+                                                  const double *__restrict__ energy, double *__restrict__ u,
+                                                  double *__restrict__ u0)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        u[i] = energy[i] * dens[i];
+        u0[i] = energy[i] * dens[i];
+    }
+}
+
+// Generic a = b copy (set_field_kernel.cuknl:5-16, ppcg store_r / update_z,
+// jacobi copy_u).
+__global__ __launch_bounds__(BLOCK) void k_copy(Ext e, long pitch, double *__restrict__ dst,
+                                                const double *__restrict__ src)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) dst[i0 + k * pitch] = src[i0 + k * pitch];
+}
+
+// field_summary_kernel.cuknl:5-58
+__global__ __launch_bounds__(BLOCK) void k_field_summary(Ext e, long pitch, const double *__restrict__ vol,
+                                                         const double *__restrict__ dens,
+                                                         const double *__restrict__ energy,
+                                                         const double *__restrict__ u, Red red)
+{
+    STRIP_BEGIN;
+    DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const double cell_vol = vol[i];
+        const double cell_mass = cell_vol * dens[i];
+        dd_add_prod(acc[0], cell_vol, 1.0);
+        dd_add_prod(acc[1], cell_vol, dens[i]);
+        dd_add_prod(acc[2], cell_mass, energy[i]);
+        dd_add_prod(acc[3], cell_mass, u[i]);
+    }
+    grid_reduce<4>(acc, red);
+}
+
+// ---- halos: reflective update and pack / unpack ------------------------------------
+// One thread per (position along the face, depth level).  Reflection: the halo
+// cell at distance k from the face takes the interior cell at distance k-1.
+
+__global__ void k_reflect_bt(double *__restrict__ a, long pitch, int h, int nx, int ny, int d, int top)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;  // column offset in [0, nx+2d)
+    const int k = blockIdx.y + 1;                         // distance 1..d
+    if (j >= nx + 2 * d) return;
+    const int c = h - d + j;
+    if (!top) a[c + (long)(h - k) * pitch] = a[c + (long)(h + k - 1) * pitch];
+    else a[c + (long)(h + ny - 1 + k) * pitch] = a[c + (long)(h + ny - k) * pitch];
+}
+
+__global__ void k_reflect_lr(double *__restrict__ a, long pitch, int h, int nx, int ny, int d, int right)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;  // row offset in [0, ny+2d)
+    const int k = blockIdx.y + 1;
+    if (j >= ny + 2 * d) return;
+    const long r = h - d + j;
+    if (!right) a[(h - k) + r * pitch] = a[(h + k - 1) + r * pitch];
+    else a[(h + nx - 1 + k) + r * pitch] = a[(h + nx - k) + r * pitch];
+}
+
+// pack_kernel.cuknl:5-147 buffer layout: left/right strips are (ny+2d) x d with
+// element (j,k) at j + k*d; bottom/top strips are d x (nx+2d) with (j,k) at
+// j + k*(nx+2d).  Left and bottom strips hold the face cells mirrored.
+__global__ void k_pack(double *__restrict__ a, double *__restrict__ buf, long pitch, int h, int nx, int ny, int d,
+                       int face, int pack)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    long ai, bi;
+    if (face == TEA_CHUNK_LEFT || face == TEA_CHUNK_RIGHT) {
+        // j runs over rows (ny+2d), k over the d columns
+        if (j >= ny + 2 * d) return;
+        const long r = h - d + j;
+        int c;
+        if (face == TEA_CHUNK_LEFT) c = pack ? (h + d - 1 - k) : (h - d + k);
+        else c = pack ? (h + nx - d + k) : (h + nx + d - 1 - k);
+        ai = c + r * pitch;
+        bi = k + (long)j * d;
+    } else {
+        if (j >= nx + 2 * d) return;
+        const int c = h - d + j;
+        long r;
+        if (face == TEA_CHUNK_BOTTOM) r = pack ? (h + d - 1 - k) : (h - d + k);
+        else r = pack ? (h + ny - d + k) : (h + ny + d - 1 - k);
+        ai = c + r * pitch;
+        bi = j + (long)k * (nx + 2 * d);
+    }
+    if (pack) buf[bi] = a[ai];
+    else a[ai] = buf[bi];
+}
+
+// ---- solver: common --------------------------------------------------------------
+
+// tea_leaf_common.cuknl:60-97 over x,y in [0,+h]
+__global__ __launch_bounds__(BLOCK) void k_init_common(Ext e, long pitch, const double *__restrict__ dens,
+                                                       double *__restrict__ Kx, double *__restrict__ Ky,
+                                                       double rx, double ry, int coef, long i_origin)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        double dc = 0, dl = 0, dd = 0;
+        if (coef == TEA_CONDUCTIVITY) {
+            dc = dens[i];
+            if (i - 1 - i_origin > 0) dl = dens[i - 1];
+            if (i - pitch - i_origin > 0) dd = dens[i - pitch];
+        } else {
+            dc = 1.0 / dens[i];
+            dl = 1.0 / dens[i - 1];
+            dd = 1.0 / dens[i - pitch];
+        }
+        double kx = (dl + dc) / (2.0 * dl * dc);
+        kx *= rx;
+        double ky = (dd + dc) / (2.0 * dd * dc);
+        ky *= ry;
+        Kx[i] = kx;
+        Ky[i] = ky;
+    }
+}
+
+// tea_leaf_common.cuknl:99-129 over +-h
+__global__ __launch_bounds__(BLOCK) void k_zero_boundaries(Ext e, long pitch, double *__restrict__ Kx,
+                                                           double *__restrict__ Ky, int h, int nx, int ny, int zl,
+                                                           int zr, int zb, int zt)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const int c = s.c, r = s.r0 + k;
+        if (zl && c <= h) Kx[i] = 0;
+        if (zr && c > (nx - 1) + h) Kx[i] = 0;
+        if (zb && r <= h) Ky[i] = 0;
+        if (zt && r > (ny - 1) + h) Ky[i] = 0;
+    }
+}
+
+// tea_leaf_common.cuknl:19-39: r = u0 - A u
+__global__ __launch_bounds__(BLOCK) void k_calc_residual(Ext e, long pitch, const double *__restrict__ u,
+                                                         const double *__restrict__ u0, double *__restrict__ r,
+                                                         const double *__restrict__ Kx,
+                                                         const double *__restrict__ Ky)
+{
+    STRIP_BEGIN;
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const double smvp = apply_A(u, Kx, Ky, i, pitch);
+            r[i] = u0[i] - smvp;
+        }
+    }
+}
+
+// tea_leaf_common.cuknl:41-58: sum a*b
+__global__ __launch_bounds__(BLOCK) void k_dot(Ext e, long pitch, const double *__restrict__ a,
+                                               const double *__restrict__ b, Red red)
+{
+    STRIP_BEGIN;
+    DD acc[1] = {{0, 0}};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) dd_add_prod(acc[0], a[i0 + k * pitch], b[i0 + k * pitch]);
+    grid_reduce<1>(acc, red);
+}
+
+// tea_leaf_common.cuknl:5-17: energy1 = u/density
+__global__ __launch_bounds__(BLOCK) void k_finalise(Ext e, long pitch, const double *__restrict__ dens,
+                                                    const double *__restrict__ u, double *__restrict__ e1)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) e1[i0 + k * pitch] = u[i0 + k * pitch] / dens[i0 + k * pitch];
+}
+
+// tea_leaf_common.cuknl:131-147 (interior): Mi = 1/diag
+__global__ __launch_bounds__(BLOCK) void k_jac_diag(Ext e, long pitch, double *__restrict__ Mi,
+                                                    const double *__restrict__ Kx, const double *__restrict__ Ky)
+{
+    STRIP_BEGIN;
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const double diag = diag_A(Kx, Ky, i, pitch);
+        Mi[i] = 1.0 / diag;
+    }
+}
+
+// tea_block_jacobi.cuknl:83-111: Thomas factors cp, bfp per strip
+__global__ __launch_bounds__(BLOCK) void k_block_init(Ext e, long pitch, double *__restrict__ cp,
+                                                      double *__restrict__ bfp, const double *__restrict__ Kx,
+                                                      const double *__restrict__ Ky)
+{
+    STRIP_BEGIN;
+    if (s.len == 0) return;
+    cp[i0] = (-Ky[i0 + pitch]) / diag_A(Kx, Ky, i0, pitch);
+    for (int k = 1; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        bfp[i] = 1.0 / (diag_A(Kx, Ky, i, pitch) - (-Ky[i]) * cp[i - pitch]);
+        cp[i] = (-Ky[i + pitch]) * bfp[i];
+    }
+}
+
+// z = M^-1 r on a strip held in registers (block: Thomas; diag: r*Mi).
+template <int PREC>
+__device__ __forceinline__ void precond_strip(const double (&rv)[STRIP], double (&zv)[STRIP], int len, long i0,
+                                              long pitch, const double *__restrict__ Mi,
+                                              const double *__restrict__ cp, const double *__restrict__ bfp,
+                                              const double *__restrict__ Kx, const double *__restrict__ Ky)
+{
+    if (PREC == PREC_BLOCK) {
+        if (len > 0) block_solve_strip(rv, zv, len, cp, bfp, Kx, Ky, i0, pitch);
+    } else if (PREC == PREC_DIAG) {
+#pragma unroll
+        for (int k = 0; k < STRIP; k++)
+            if (k < len) zv[k] = rv[k] * Mi[i0 + k * pitch];
+    } else {
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) zv[k] = rv[k];
+    }
+}
+
+// ---- solver: CG -------------------------------------------------------------------
+
+// tea_leaf_cg.cuknl:5-39 (+ block_solve for jac_block): z = M^-1 r, p = z or r, sum r*p
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_cg_init_p(Ext e, long pitch, double *__restrict__ p,
+                                                     const double *__restrict__ r, double *__restrict__ z,
+                                                     const double *__restrict__ Mi, const double *__restrict__ cp,
+                                                     const double *__restrict__ bfp, const double *__restrict__ Kx,
+                                                     const double *__restrict__ Ky, Red red)
+{
+    STRIP_BEGIN;
+    DD acc[1] = {{0, 0}};
+    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) rv[k] = r[i0 + k * pitch];
+    if (PREC != PREC_NONE) precond_strip<PREC>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            double pv;
+            if (PREC != PREC_NONE) {
+                z[i] = zv[k];
+                pv = zv[k];
+            } else {
+                pv = rv[k];
+            }
+            p[i] = pv;
+            dd_add_prod(acc[0], rv[k], pv);
+        }
+    }
+    grid_reduce<1>(acc, red);
+}
+
+// tea_leaf_cg.cuknl:43-68: w = A p, sum p*w
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Ext e, long pitch, const double *__restrict__ p,
+                                                     double *__restrict__ w, const double *__restrict__ Kx,
+                                                     const double *__restrict__ Ky, Red red)
+{
+    STRIP_BEGIN;
+    DD acc[1] = {{0, 0}};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const double wv = apply_A(p, Kx, Ky, i, pitch);
+            w[i] = wv;
+            dd_add_prod(acc[0], p[i], wv);
+        }
+    }
+    grid_reduce<1>(acc, red);
+}
+
+// tea_leaf_cg.cuknl:72-137: u += a p; r -= a w; z = M^-1 r; sum r*z (r*r)
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Ext e, long pitch, double alpha, double *__restrict__ u,
+                                                      const double *__restrict__ p, double *__restrict__ r,
+                                                      const double *__restrict__ w, double *__restrict__ z,
+                                                      const double *__restrict__ Mi, const double *__restrict__ cp,
+                                                      const double *__restrict__ bfp, const double *__restrict__ Kx,
+                                                      const double *__restrict__ Ky, Red red)
+{
+    STRIP_BEGIN;
+    DD acc[1] = {{0, 0}};
+    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            u[i] += alpha * p[i];
+            rv[k] = r[i] - alpha * w[i];
+            r[i] = rv[k];
+        }
+    }
+    if (PREC != PREC_NONE) precond_strip<PREC>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            if (PREC != PREC_NONE) {
+                z[i0 + k * pitch] = zv[k];
+                dd_add_prod(acc[0], rv[k], zv[k]);
+            } else {
+                dd_add_prod(acc[0], rv[k], rv[k]);
+            }
+        }
+    }
+    grid_reduce<1>(acc, red);
+}
+
+// tea_leaf_cg.cuknl:141-161 and tea_leaf_ppcg.cuknl:301-313: p = src + beta p
+__global__ __launch_bounds__(BLOCK) void k_calc_p(Ext e, long pitch, double beta, double *__restrict__ p,
+                                                  const double *__restrict__ src)
+{
+    STRIP_BEGIN;
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            p[i] = src[i] + beta * p[i];
+        }
+}
+
+// ---- solver: Chebyshev --------------------------------------------------------------
+
+// tea_leaf_cheby.cuknl:5-68 (first = 1, p = M^-1 r / theta) and :84-152
+// (p = alpha p + beta M^-1 r); both start with w = A u, r = u0 - w.
+template <int PREC, int FIRST>
+__global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const double *__restrict__ u,
+                                                   const double *__restrict__ u0, double *__restrict__ p,
+                                                   double *__restrict__ r, double *__restrict__ w,
+                                                   const double *__restrict__ Mi, const double *__restrict__ cp,
+                                                   const double *__restrict__ bfp, const double *__restrict__ Kx,
+                                                   const double *__restrict__ Ky, double theta, double alpha,
+                                                   double beta)
+{
+    STRIP_BEGIN;
+    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const double wv = apply_A(u, Kx, Ky, i, pitch);
+            w[i] = wv;
+            rv[k] = u0[i] - wv;
+            r[i] = rv[k];
+        }
+    }
+    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            double pv;
+            if (FIRST) {
+                if (PREC == PREC_BLOCK) pv = zv[k] / theta;
+                else if (PREC == PREC_DIAG) pv = (Mi[i] * rv[k]) / theta;
+                else pv = rv[k] / theta;
+            } else {
+                if (PREC == PREC_BLOCK) pv = alpha * p[i] + beta * zv[k];
+                else if (PREC == PREC_DIAG) pv = alpha * p[i] + beta * Mi[i] * rv[k];
+                else pv = alpha * p[i] + beta * rv[k];
+            }
+            p[i] = pv;
+        }
+    }
+}
+
+// tea_leaf_cheby.cuknl:71-82: u += p
+__global__ __launch_bounds__(BLOCK) void k_add(Ext e, long pitch, double *__restrict__ u,
+                                               const double *__restrict__ p)
+{
+    STRIP_BEGIN;
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) u[i0 + k * pitch] += p[i0 + k * pitch];
+}
+
+// ---- solver: Jacobi -------------------------------------------------------------------
+
+// tea_leaf_jacobi.cuknl:18-46 (un is the copy of u held in the Mi buffer)
+__global__ __launch_bounds__(BLOCK) void k_jacobi(Ext e, long pitch, const double *__restrict__ Kx,
+                                                  const double *__restrict__ Ky, const double *__restrict__ u0,
+                                                  double *__restrict__ u, const double *__restrict__ un, Red red)
+{
+    STRIP_BEGIN;
+    DD acc[1] = {{0, 0}};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const double uv = (u0[i] + Kx[i + 1] * un[i + 1] + Kx[i] * un[i - 1] + Ky[i + pitch] * un[i + pitch]
+                               + Ky[i] * un[i - pitch])
+                              / (1.0 + (Kx[i] + Kx[i + 1]) + (Ky[i] + Ky[i + pitch]));
+            u[i] = uv;
+            dd_add_prod(acc[0], fabs(uv - un[i]), 1.0);
+        }
+    }
+    grid_reduce<1>(acc, red);
+}
+
+// ---- solver: PPCG ---------------------------------------------------------------------
+
+// tea_leaf_ppcg.cuknl:319-420
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_init(Ext e, long pitch, double *__restrict__ p,
+                                                     const double *__restrict__ r, double *__restrict__ z,
+                                                     const double *__restrict__ Mi, const double *__restrict__ cp,
+                                                     const double *__restrict__ bfp, const double *__restrict__ Kx,
+                                                     const double *__restrict__ Ky, int step, Red red)
+{
+    STRIP_BEGIN;
+    DD acc[1] = {{0, 0}};
+    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) rv[k] = r[i0 + k * pitch];
+    if (PREC != PREC_NONE && (step == 1 || step == 2))
+        precond_strip<PREC>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            if (step == 1) {
+                // p = z = 0, then z = M^-1 r and p = z (or p = r)
+                if (PREC != PREC_NONE) {
+                    z[i] = zv[k];
+                    p[i] = zv[k];
+                } else {
+                    z[i] = 0.0;
+                    p[i] = rv[k];
+                }
+                dd_add_prod(acc[0], rv[k], p[i]);
+            } else if (step == 2) {
+                if (PREC != PREC_NONE) z[i] = zv[k];
+            } else {
+                const double pv = z[i];
+                p[i] = pv;
+                dd_add_prod(acc[0], rv[k], pv);
+            }
+        }
+    }
+    grid_reduce<1>(acc, red);
+}
+
+// tea_leaf_ppcg.cuknl:5-55 (init_sd, not called by the reference driver)
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd(Ext e, long pitch, const double *__restrict__ r,
+                                                        double *__restrict__ sd, const double *__restrict__ Mi,
+                                                        const double *__restrict__ cp,
+                                                        const double *__restrict__ bfp,
+                                                        const double *__restrict__ Kx,
+                                                        const double *__restrict__ Ky, double theta)
+{
+    STRIP_BEGIN;
+    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) rv[k] = r[i0 + k * pitch];
+    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            if (PREC == PREC_BLOCK) sd[i] = zv[k] / theta;
+            else if (PREC == PREC_DIAG) sd[i] = rv[k] * Mi[i] / theta;
+            else sd[i] = rv[k] / theta;
+        }
+    }
+}
+
+// tea_leaf_ppcg.cuknl:58-89: sd = (z or r) * (1/theta); rtemp = r; utemp = sd
+__global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd_new(Ext e, long pitch, const double *__restrict__ r,
+                                                            double *__restrict__ sd,
+                                                            const double *__restrict__ src,
+                                                            double *__restrict__ rtemp,
+                                                            double *__restrict__ utemp, double theta_r)
+{
+    STRIP_BEGIN;
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const double sdv = src[i] * theta_r;
+            sd[i] = sdv;
+            rtemp[i] = r[i];
+            utemp[i] = sdv;
+        }
+    }
+}
+
+// tea_leaf_ppcg.cuknl:154-173: rtemp -= A sd
+__global__ __launch_bounds__(BLOCK) void k_ppcg_update_r(Ext e, long pitch, double *__restrict__ rtemp,
+                                                         const double *__restrict__ Kx,
+                                                         const double *__restrict__ Ky,
+                                                         const double *__restrict__ sd)
+{
+    STRIP_BEGIN;
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const double result = apply_A(sd, Kx, Ky, i, pitch);
+            rtemp[i] -= result;
+        }
+    }
+}
+
+// tea_leaf_ppcg.cuknl:178-244: sd = alpha sd + beta M^-1 rtemp; utemp += sd
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_calc_sd(Ext e, long pitch, double *__restrict__ sd,
+                                                        const double *__restrict__ rtemp,
+                                                        double *__restrict__ utemp, const double *__restrict__ Mi,
+                                                        const double *__restrict__ cp,
+                                                        const double *__restrict__ bfp,
+                                                        const double *__restrict__ Kx,
+                                                        const double *__restrict__ Ky, double alpha, double beta)
+{
+    STRIP_BEGIN;
+    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) rv[k] = rtemp[i0 + k * pitch];
+    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            double sdv;
+            if (PREC == PREC_BLOCK) sdv = alpha * sd[i] + beta * zv[k];
+            else if (PREC == PREC_DIAG) sdv = alpha * sd[i] + beta * rv[k] * Mi[i];
+            else sdv = alpha * sd[i] + beta * rv[k];
+            sd[i] = sdv;
+            utemp[i] += sdv;
+        }
+    }
+}
+
+// tea_leaf_ppcg.cuknl:279-297: sum (r - r_store)*z
+__global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const double *__restrict__ rstore,
+                                                    const double *__restrict__ r, const double *__restrict__ z,
+                                                    Red red)
+{
+    STRIP_BEGIN;
+    DD acc[1] = {{0, 0}};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            dd_add_prod(acc[0], r[i] - rstore[i], z[i]);
+        }
+    grid_reduce<1>(acc, red);
+}
+
+}  // namespace
+
+// =============================================================================
+// Entry points
+// =============================================================================
+namespace {
+
+double *F(int id) { return g.fs[id]; }
+
+void release_all()
+{
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    for (auto &p : g.prof.pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    g.prof.pending.clear();
+    for (auto e : g.prof.pool) (void)hipEventDestroy(e);
+    g.prof.pool.clear();
+    if (g.slab) (void)hipFree(g.slab);
+    if (g.cellx) (void)hipFree(g.cellx);
+    if (g.partials) (void)hipFree(g.partials);
+    if (g.counter) (void)hipFree(g.counter);
+    if (g.h_out) (void)hipHostFree(g.h_out);
+    if (g.d_halo_buf) (void)hipFree(g.d_halo_buf);
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    Profiler keep = g.prof;
+    g = Chunk();
+    g.prof.on = keep.on;
+}
+
+
+// Dispatch a preconditioner-templated kernel.
+#define PREC_SWITCH(PREC_VALUE, KERNEL, ...)                                    \
+    ((PREC_VALUE) == PREC_BLOCK ? launch(#KERNEL, KERNEL<PREC_BLOCK>, __VA_ARGS__) \
+     : (PREC_VALUE) == PREC_DIAG ? launch(#KERNEL, KERNEL<PREC_DIAG>, __VA_ARGS__) \
+                                 : launch(#KERNEL, KERNEL<PREC_NONE>, __VA_ARGS__))
+
+}  // namespace
+
+EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
+{
+    release_all();
+    {
+        std::ifstream probe(g_deck_path);
+        if (!probe) { die("Input file not found: %s", g_deck_path.c_str()); return; }
+    }
+    int dev = read_int("hip_device");
+    if (dev < 0) dev = read_int("cuda_device");
+    if (dev < 0) dev = 0;
+    int ndev = 0;
+    HIP_OK(hipGetDeviceCount(&ndev));
+    if (dev >= ndev) { die("Device id %d specified in the deck, but only %d devices available", dev, ndev); return; }
+    HIP_OK(hipSetDevice(dev));
+    g.device = dev;
+
+    g.h = read_int("halo_depth");
+    if (g.h < 1) { die("Halo exchange depth unspecified or was too small"); return; }
+    if (param_enabled("tl_use_ppcg")) g.solver = SOLVER_PPCG;
+    else if (param_enabled("tl_use_chebyshev")) g.solver = SOLVER_CHEBY;
+    else if (param_enabled("tl_use_cg")) g.solver = SOLVER_CG;
+    else g.solver = SOLVER_JACOBI;
+    const std::string prec = read_string("tl_preconditioner_type");
+    if (prec.find("jac_diag") != std::string::npos) g.prec = PREC_DIAG;
+    else if (prec.find("jac_block") != std::string::npos) g.prec = PREC_BLOCK;
+    else g.prec = PREC_NONE;
+    if (param_enabled("profiler_on")) g.prof.on = true;
+
+    g.nx = *x_max - *x_min + 1;
+    g.ny = *y_max - *y_min + 1;
+    if (g.nx < 1 || g.ny < 1) { die("invalid chunk size %d x %d", g.nx, g.ny); return; }
+    g.lead = (g.h + 15) / 16 * 16;
+    g.pitch = (long)((g.lead + g.nx + g.h + 15) / 16 * 16);
+    g.rows = (long)g.ny + 2L * g.h;
+
+    HIP_OK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    const size_t field_elems = (size_t)g.pitch * (size_t)g.rows + 16;  // + slack, keeps 128-byte alignment
+    const size_t field_stride = (field_elems + 31) / 32 * 32;
+    HIP_OK(hipMalloc(&g.slab, field_stride * X_COUNT * sizeof(double)));
+    HIP_OK(hipMemsetAsync(g.slab, 0, field_stride * X_COUNT * sizeof(double), g.stream));
+    for (int k = 1; k < X_COUNT; k++) {
+        g.f[k] = g.slab + field_stride * k;
+        g.fs[k] = g.f[k] + (g.lead - g.h);
+    }
+    // 1D coordinate arrays (cell and vertex x/y), indices 0 .. n+2h
+    const size_t nxv = (size_t)g.nx + 2 * g.h + 2, nyv = (size_t)g.ny + 2 * g.h + 2;
+    HIP_OK(hipMalloc(&g.cellx, (2 * nxv + 2 * nyv) * sizeof(double)));
+    g.vertexx = g.cellx + nxv;
+    g.celly = g.vertexx + nxv;
+    g.vertexy = g.celly + nyv;
+
+    g.max_blocks = blocks_for(Ext{0, g.nx + 2 * g.h - 1, 0, g.ny + 2 * g.h - 1}) + 1;
+    HIP_OK(hipMalloc(&g.partials, (size_t)MAX_RED * g.max_blocks * sizeof(DD)));
+    HIP_OK(hipMalloc(&g.counter, sizeof(unsigned)));
+    HIP_OK(hipMemsetAsync(g.counter, 0, sizeof(unsigned), g.stream));
+    HIP_OK(hipHostMalloc(&g.h_out, MAX_RED * sizeof(double), hipHostMallocDefault));
+    HIP_OK(hipHostGetDevicePointer((void **)&g.d_out, g.h_out, 0));
+
+    g.halo_buf_elems = (size_t)TEA_NUM_FIELDS * (size_t)g.h * (size_t)((g.nx > g.ny ? g.nx : g.ny) + 2 * g.h);
+    HIP_OK(hipMalloc(&g.d_halo_buf, g.halo_buf_elems * sizeof(double)));
+    HIP_OK(hipStreamSynchronize(g.stream));
+    g.initialised = true;
+}
+
+EXPORT void initialise_chunk_kernel_cuda_(double *xmin, double *ymin, double *dx, double *dy)
+{
+    CHECK_INIT();
+    const double d_xmin = *xmin, d_ymin = *ymin, d_dx = *dx, d_dy = *dy;
+    const int h = g.h;
+    const size_t nxv = (size_t)g.nx + 2 * h + 2, nyv = (size_t)g.ny + 2 * h + 2;
+    std::vector<double> host(2 * nxv + 2 * nyv, 0.0);
+    double *cx = host.data(), *vx = cx + nxv, *cy = vx + nxv, *vy = cy + nyv;
+    g.h_celldx.assign(nxv, 0.0);
+    g.h_celldy.assign(nyv, 0.0);
+    // initialise_chunk_kernel.cuknl:3-51
+    for (int c = 0; c <= g.nx + 2 * h; c++) {
+        vx[c] = d_xmin + d_dx * (double)((c - 1) - h + 1);
+        const double vplus = d_xmin + d_dx * (double)(c - h + 1);
+        cx[c] = 0.5 * (vx[c] + vplus);
+        g.h_celldx[c] = d_dx;
+    }
+    for (int r = 0; r <= g.ny + 2 * h; r++) {
+        vy[r] = d_ymin + d_dy * (double)((r - 1) - h + 1);
+        const double vplus = d_ymin + d_dy * (double)(r - h + 1);
+        cy[r] = 0.5 * (vy[r] + vplus);
+        g.h_celldy[r] = d_dy;
+    }
+    HIP_OK(hipMemcpyAsync(g.cellx, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    const Ext all{0, g.nx + 2 * h - 1, 0, g.ny + 2 * h - 1};
+    launch("initialise_chunk", k_init_volume, all, g.pitch, F(X_VOLUME), d_dx * d_dy);
+    HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+EXPORT void generate_chunk_kernel_cuda_(const int *number_of_states, const double *state_density,
+                                        const double *state_energy, const double *state_xmin,
+                                        const double *state_xmax, const double *state_ymin,
+                                        const double *state_ymax, const double *state_radius,
+                                        const int *state_geometry, const int *g_rect, const int *g_circ,
+                                        const int *g_point)
+{
+    CHECK_INIT();
+    const int ns = *number_of_states;
+    if (ns < 1) { die("generate_chunk: no states"); return; }
+    const Ext in = interior();
+    launch("generate_chunk_init", k_generate_init, in, g.pitch, F(F_DENSITY), F(F_ENERGY0), state_density[0],
+           state_energy[0]);
+    for (int s = 1; s < ns; s++) {
+        launch("generate_chunk_kernel", k_generate_state, in, g.pitch, (const double *)g.vertexx,
+               (const double *)g.vertexy, (const double *)g.cellx, (const double *)g.celly, F(F_DENSITY),
+               F(F_ENERGY0), state_density[s], state_energy[s], state_xmin[s], state_xmax[s], state_ymin[s],
+               state_ymax[s], state_radius[s], state_geometry[s], *g_rect, *g_circ, *g_point);
+    }
+    launch("generate_chunk_init_u", k_init_u, in, g.pitch, (const double *)F(F_DENSITY),
+           (const double *)F(F_ENERGY0), F(F_U), F(X_U0));
+    HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+EXPORT void set_field_kernel_cuda_(void)
+{
+    CHECK_INIT();
+    launch("set_field", k_copy, interior(), g.pitch, F(F_ENERGY1), (const double *)F(F_ENERGY0));
+}
+
+EXPORT void field_summary_kernel_cuda_(double *vol, double *mass, double *ie, double *temp)
+{
+    CHECK_INIT();
+    if (!launch("field_summary", k_field_summary, interior(), g.pitch, (const double *)F(X_VOLUME),
+                (const double *)F(F_DENSITY), (const double *)F(F_ENERGY1), (const double *)F(F_U), red_ctx()))
+        return;
+    double out[4];
+    if (!fetch(out, 4)) return;
+    *vol = out[0];
+    *mass = out[1];
+    *ie = out[2];
+    *temp = out[3];
+}
+
+EXPORT void update_halo_kernel_cuda_(const int *chunk_neighbours, const int *fields, const int *depth)
+{
+    CHECK_INIT();
+    const int d = *depth;
+    if (d < 1 || d > g.h) { die("update_halo: invalid depth %d (halo depth %d)", d, g.h); return; }
+    for (int f = 1; f <= TEA_NUM_FIELDS; f++) {
+        if (fields[f - 1] != 1) continue;
+        double *a = F(f);
+        Timed t("update_halo");
+        if (chunk_neighbours[TEA_CHUNK_BOTTOM - 1] == TEA_EXTERNAL_FACE)
+            hipLaunchKernelGGL(k_reflect_bt, dim3((g.nx + 2 * d + 255) / 256, d), dim3(256), 0, g.stream, a, g.pitch,
+                               g.h, g.nx, g.ny, d, 0);
+        if (chunk_neighbours[TEA_CHUNK_TOP - 1] == TEA_EXTERNAL_FACE)
+            hipLaunchKernelGGL(k_reflect_bt, dim3((g.nx + 2 * d + 255) / 256, d), dim3(256), 0, g.stream, a, g.pitch,
+                               g.h, g.nx, g.ny, d, 1);
+        if (chunk_neighbours[TEA_CHUNK_LEFT - 1] == TEA_EXTERNAL_FACE)
+            hipLaunchKernelGGL(k_reflect_lr, dim3((g.ny + 2 * d + 255) / 256, d), dim3(256), 0, g.stream, a, g.pitch,
+                               g.h, g.nx, g.ny, d, 0);
+        if (chunk_neighbours[TEA_CHUNK_RIGHT - 1] == TEA_EXTERNAL_FACE)
+            hipLaunchKernelGGL(k_reflect_lr, dim3((g.ny + 2 * d + 255) / 256, d), dim3(256), 0, g.stream, a, g.pitch,
+                               g.h, g.nx, g.ny, d, 1);
+    }
+    HIP_OK(hipGetLastError());
+}
+
+namespace {
+void pack_unpack(int *fields, int *offsets, int d, int face, int pack, double *host_buffer)
+{
+    CHECK_INIT();
+    if (d < 1 || d > g.h) { die("pack: invalid depth %d (halo depth %d)", d, g.h); return; }
+    if (face < TEA_CHUNK_LEFT || face > TEA_CHUNK_TOP) {
+        die("Invalid face identifier %d passed to mpi buffer packing", face);
+        return;
+    }
+    const bool lr = face <= TEA_CHUNK_RIGHT;
+    const long side = lr ? (long)d * (g.ny + 2 * d) : (long)d * (g.nx + 2 * d);
+    long end = 0;
+    int n = 0;
+    for (int f = 0; f < TEA_NUM_FIELDS; f++) {
+        if (!fields[f]) continue;
+        if (offsets[f] < 0 || offsets[f] > TEA_NUM_FIELDS * side) {
+            die("Tried to pack/unpack field %d but invalid offset %d given", f, offsets[f]);
+            return;
+        }
+        if (offsets[f] + side > end) end = offsets[f] + side;
+        n++;
+    }
+    if (n == 0) return;
+    if ((size_t)end > g.halo_buf_elems) { die("pack: buffer of %ld elements exceeds staging", end); return; }
+    if (!pack) HIP_OK(hipMemcpyAsync(g.d_halo_buf, host_buffer, end * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    const int len = lr ? g.ny + 2 * d : g.nx + 2 * d;
+    for (int f = 0; f < TEA_NUM_FIELDS; f++) {
+        if (!fields[f]) continue;
+        Timed t(pack ? "pack" : "unpack");
+        hipLaunchKernelGGL(k_pack, dim3((len + 255) / 256, d), dim3(256), 0, g.stream, F(f + 1),
+                           g.d_halo_buf + offsets[f], g.pitch, g.h, g.nx, g.ny, d, face, pack);
+    }
+    HIP_OK(hipGetLastError());
+    if (pack) HIP_OK(hipMemcpyAsync(host_buffer, g.d_halo_buf, end * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIP_OK(hipStreamSynchronize(g.stream));
+}
+}  // namespace
+
+EXPORT void cuda_pack_buffers_(int *fields, int *offsets, int *depth, int *face, double *host_buffer)
+{
+    pack_unpack(fields, offsets, *depth, *face, 1, host_buffer);
+}
+
+EXPORT void cuda_unpack_buffers_(int *fields, int *offsets, int *depth, int *face, double *host_buffer)
+{
+    pack_unpack(fields, offsets, *depth, *face, 0, host_buffer);
+}
+
+EXPORT void tea_leaf_common_init_kernel_cuda_(const int *coefficient, double *dt, double *rx, double *ry,
+                                              int *zero_boundary, int *reflective_boundary)
+{
+    CHECK_INIT();
+    const int coef = *coefficient;
+    if (coef != TEA_CONDUCTIVITY && coef != TEA_RECIP_CONDUCTIVITY) {
+        die("Unknown coefficient %d passed to tea leaf", coef);
+        return;
+    }
+    // calcrxry, tea_leaf_kernel_cuda.cu:16-32 (dx, dy kept on the host)
+    const double dxv = g.h_celldx[g.h], dyv = g.h_celldy[g.h];
+    *rx = *dt / (dxv * dxv);
+    *ry = *dt / (dyv * dyv);
+    const int h = g.h;
+    // index of cell (0,0) relative to the shifted base, for the reference's
+    // "THARR2D(-1,0,0) > 0" guards
+    const long i_origin = 0;
+    launch("init_common", k_init_common, Ext{h, h + g.nx - 1 + h, h, h + g.ny - 1 + h}, g.pitch,
+           (const double *)F(F_DENSITY), F(X_KX), F(X_KY), *rx, *ry, coef, i_origin);
+    if (!*reflective_boundary) {
+        launch("zero_boundaries", k_zero_boundaries, Ext{0, g.nx + 2 * h - 1, 0, g.ny + 2 * h - 1}, g.pitch,
+               F(X_KX), F(X_KY), h, g.nx, g.ny, zero_boundary[0], zero_boundary[1], zero_boundary[2],
+               zero_boundary[3]);
+    }
+    launch("generate_chunk_init_u", k_init_u, interior(), g.pitch, (const double *)F(F_DENSITY),
+           (const double *)F(F_ENERGY1), F(F_U), F(X_U0));
+}
+
+EXPORT void tea_leaf_calc_residual_cuda_(void)
+{
+    CHECK_INIT();
+    launch("calc_residual", k_calc_residual, interior(), g.pitch, (const double *)F(F_U),
+           (const double *)F(X_U0), F(F_R), (const double *)F(X_KX), (const double *)F(X_KY));
+}
+
+EXPORT void tea_leaf_common_finalise_kernel_cuda_(void)
+{
+    CHECK_INIT();
+    launch("finalise", k_finalise, interior(), g.pitch, (const double *)F(F_DENSITY), (const double *)F(F_U),
+           F(F_ENERGY1));
+}
+
+EXPORT void tea_leaf_calc_2norm_kernel_cuda_(int *norm_array, double *norm)
+{
+    CHECK_INIT();
+    const double *a, *b;
+    if (*norm_array == 0) { a = F(X_U0); b = F(X_U0); }
+    else if (*norm_array == 1) { a = F(F_R); b = F(F_R); }
+    else if (*norm_array == 2) { a = F(F_R); b = (g.prec != PREC_NONE) ? F(X_Z) : F(F_R); }
+    else { die("Invalid value '%d' for norm_array passed, should be [0, 1, 2]", *norm_array); return; }
+    if (!launch("calc_2norm", k_dot, interior(), g.pitch, a, b, red_ctx())) return;
+    fetch(norm, 1);
+}
+
+EXPORT void tea_leaf_cg_init_kernel_cuda_(double *rro)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(S_CG_FAMILY);
+    const Ext in = interior();
+    if (g.prec == PREC_BLOCK)
+        launch("block_init", k_block_init, in, g.pitch, F(X_CP), F(X_BFP), (const double *)F(X_KX),
+               (const double *)F(X_KY));
+    else if (g.prec == PREC_DIAG)
+        launch("init_jac_diag", k_jac_diag, in, g.pitch, F(X_MI), (const double *)F(X_KX), (const double *)F(X_KY));
+    if (!PREC_SWITCH(g.prec, k_cg_init_p, in, g.pitch, F(F_P), (const double *)F(F_R), F(X_Z),
+                     (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
+                     (const double *)F(X_KX), (const double *)F(X_KY), red_ctx()))
+        return;
+    fetch(rro, 1);
+}
+
+EXPORT void tea_leaf_cg_calc_w_kernel_cuda_(double *pw)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(S_CG_FAMILY);
+    if (!launch("cg_calc_w", k_cg_calc_w, interior(), g.pitch, (const double *)F(F_P), F(X_W),
+                (const double *)F(X_KX), (const double *)F(X_KY), red_ctx()))
+        return;
+    fetch(pw, 1);
+}
+
+EXPORT void tea_leaf_cg_calc_ur_kernel_cuda_(double *alpha, double *rrn)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(S_CG_FAMILY);
+    if (!PREC_SWITCH(g.prec, k_cg_calc_ur, interior(), g.pitch, *alpha, F(F_U), (const double *)F(F_P), F(F_R),
+                     (const double *)F(X_W), F(X_Z), (const double *)F(X_MI), (const double *)F(X_CP),
+                     (const double *)F(X_BFP), (const double *)F(X_KX), (const double *)F(X_KY), red_ctx()))
+        return;
+    fetch(rrn, 1);
+}
+
+EXPORT void tea_leaf_cg_calc_p_kernel_cuda_(double *beta)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(S_CG_FAMILY);
+    launch("cg_calc_p", k_calc_p, interior(), g.pitch, *beta, F(F_P),
+           (const double *)(g.prec != PREC_NONE ? F(X_Z) : F(F_R)));
+}
+
+namespace {
+void upload_coefs(const double *a, const double *b, int n)
+{
+    if (n < 0) n = 0;
+    g.ch_a.assign(a, a + n);
+    g.ch_b.assign(b, b + n);
+}
+
+template <int FIRST>
+bool cheby_p(double theta, double alpha, double beta)
+{
+    const char *name = FIRST ? "cheby_init_p" : "cheby_calc_p";
+    const Ext in = interior();
+    auto args = [&](auto kernel) {
+        return launch(name, kernel, in, g.pitch, (const double *)F(F_U), (const double *)F(X_U0), F(F_P), F(F_R),
+                      F(X_W), (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
+                      (const double *)F(X_KX), (const double *)F(X_KY), theta, alpha, beta);
+    };
+    if (g.prec == PREC_BLOCK) return args(k_cheby_p<PREC_BLOCK, FIRST>);
+    if (g.prec == PREC_DIAG) return args(k_cheby_p<PREC_DIAG, FIRST>);
+    return args(k_cheby_p<PREC_NONE, FIRST>);
+}
+}  // namespace
+
+EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const double *ch_betas, int *n_coefs,
+                                             const double *theta)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_CHEBY);
+    upload_coefs(ch_alphas, ch_betas, *n_coefs);
+    if (!cheby_p<1>(*theta, 0.0, 0.0)) return;
+    launch("cheby_calc_u", k_add, interior(), g.pitch, F(F_U), (const double *)F(F_P));
+}
+
+EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_CHEBY);
+    const int k = *cheby_calc_step - 1;
+    if (k < 0 || k >= (int)g.ch_a.size()) {
+        die("cheby_iterate: step %d outside the %d uploaded coefficients", k + 1, (int)g.ch_a.size());
+        return;
+    }
+    if (!cheby_p<0>(0.0, g.ch_a[k], g.ch_b[k])) return;
+    launch("cheby_calc_u", k_add, interior(), g.pitch, F(F_U), (const double *)F(F_P));
+}
+
+EXPORT void tea_leaf_jacobi_solve_kernel_cuda_(double *error)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_JACOBI);
+    const Ext in = interior();
+    launch("jacobi_copy_u", k_copy, in, g.pitch, F(X_MI), (const double *)F(F_U));
+    if (!launch("jacobi_solve", k_jacobi, in, g.pitch, (const double *)F(X_KX), (const double *)F(X_KY),
+                (const double *)F(X_U0), F(F_U), (const double *)F(X_MI), red_ctx()))
+        return;
+    fetch(error, 1);
+}
+
+EXPORT void tea_leaf_ppcg_init_constants_cuda_(const double *ch_alphas, const double *ch_betas, int *n_inner_steps)
+{
+    CHECK_INIT();
+    upload_coefs(ch_alphas, ch_betas, *n_inner_steps);
+}
+
+EXPORT void tea_leaf_ppcg_init_kernel_cuda_(const int *step, double *rro)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    const int st = *step;
+    if (st < 1 || st > 3) { die("ppcg_init: invalid step %d", st); return; }
+    if (!PREC_SWITCH(g.prec, k_ppcg_init, interior(), g.pitch, F(F_P), (const double *)F(F_R), F(X_Z),
+                     (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
+                     (const double *)F(X_KX), (const double *)F(X_KY), st, red_ctx()))
+        return;
+    double v = 0.0;
+    if (!fetch(&v, 1)) return;
+    *rro = (st == 2) ? 0.0 : v;
+}
+
+EXPORT void tea_leaf_ppcg_init_sd_kernel_cuda_(const double *theta)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    PREC_SWITCH(g.prec, k_ppcg_init_sd, interior(), g.pitch, (const double *)F(F_R), F(F_SD),
+                (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
+                (const double *)F(X_KX), (const double *)F(X_KY), *theta);
+}
+
+EXPORT void tea_leaf_ppcg_init_sd_new_kernel_cuda_(const double *theta)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    const double theta_r = 1.0 / *theta;
+    launch("ppcg_init_sd_new", k_ppcg_init_sd_new, interior(), g.pitch, (const double *)F(F_R), F(F_SD),
+           (const double *)(g.prec != PREC_NONE ? F(X_Z) : F(F_R)), F(X_RTEMP), F(X_UTEMP), theta_r);
+}
+
+EXPORT void tea_leaf_ppcg_inner_kernel_cuda_(int *ppcg_cur_step, int *bounds_extra, int *chunk_neighbours)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    const int k = *ppcg_cur_step - 1;
+    if (k < 0 || k >= (int)g.ch_a.size()) {
+        die("ppcg_inner: step %d outside the %d uploaded coefficients", k + 1, (int)g.ch_a.size());
+        return;
+    }
+    const int be = *bounds_extra;
+    if (be < 0 || be >= g.h) { die("ppcg_inner: invalid bounds_extra %d", be); return; }
+    // matrix-powers extent, tea_leaf_kernel_cuda.cu:418-452
+    const int h = g.h;
+    Ext e;
+    e.x0 = (chunk_neighbours[TEA_CHUNK_LEFT - 1] == TEA_EXTERNAL_FACE) ? h : h - be;
+    e.x1 = h + g.nx - 1 + ((chunk_neighbours[TEA_CHUNK_RIGHT - 1] == TEA_EXTERNAL_FACE) ? 0 : be);
+    e.y0 = (chunk_neighbours[TEA_CHUNK_BOTTOM - 1] == TEA_EXTERNAL_FACE) ? h : h - be;
+    e.y1 = h + g.ny - 1 + ((chunk_neighbours[TEA_CHUNK_TOP - 1] == TEA_EXTERNAL_FACE) ? 0 : be);
+    if (!launch("ppcg_update_r", k_ppcg_update_r, e, g.pitch, F(X_RTEMP), (const double *)F(X_KX),
+                (const double *)F(X_KY), (const double *)F(F_SD)))
+        return;
+    PREC_SWITCH(g.prec, k_ppcg_calc_sd, e, g.pitch, F(F_SD), (const double *)F(X_RTEMP), F(X_UTEMP),
+                (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
+                (const double *)F(X_KX), (const double *)F(X_KY), g.ch_a[k], g.ch_b[k]);
+}
+
+EXPORT void tea_leaf_ppcg_store_r_kernel_cuda_(void)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    launch("ppcg_store_r", k_copy, interior(), g.pitch, F(X_RSTORE), (const double *)F(F_R));
+}
+
+EXPORT void tea_leaf_ppcg_update_z_kernel_cuda_(void)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    launch("ppcg_update_z", k_copy, interior(), g.pitch, F(X_Z), (const double *)F(X_UTEMP));
+}
+
+EXPORT void tea_leaf_ppcg_calc_rrn_kernel_cuda_(double *norm)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    if (!launch("ppcg_calc_rrn", k_calc_rrn, interior(), g.pitch, (const double *)F(X_RSTORE),
+                (const double *)F(F_R), (const double *)F(X_Z), red_ctx()))
+        return;
+    fetch(norm, 1);
+}
+
+EXPORT void tea_leaf_ppcg_calc_2norm_kernel_cuda_(double *norm)
+{
+    CHECK_INIT();
+    if (!launch("ppcg_calc_2norm", k_dot, interior(), g.pitch, (const double *)F(F_R), (const double *)F(X_Z),
+                red_ctx()))
+        return;
+    fetch(norm, 1);
+}
+
+EXPORT void tea_leaf_ppcg_calc_p_kernel_cuda_(double *beta)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    launch("ppcg_calc_p", k_calc_p, interior(), g.pitch, *beta, F(F_P), (const double *)F(X_Z));
+}
+
+// =============================================================================
+// Extensions
+// =============================================================================
+
+EXPORT void tea_engine_set_deck_path(const char *path) { g_deck_path = path ? path : "tea.in"; }
+EXPORT void tea_engine_set_error_mode(int mode) { g_error_mode = mode; }
+EXPORT int tea_engine_last_error(char *buf, int buflen)
+{
+    if (buf && buflen > 0) snprintf(buf, (size_t)buflen, "%s", g_error);
+    return g_error[0] != 0;
+}
+EXPORT void tea_engine_clear_error(void) { g_error[0] = 0; }
+EXPORT void tea_engine_info(int *nx, int *ny, int *halo_depth, int *solver, int *preconditioner)
+{
+    *nx = g.nx;
+    *ny = g.ny;
+    *halo_depth = g.h;
+    *solver = g.solver;
+    *preconditioner = g.prec;
+}
+
+EXPORT int tea_engine_get_field(int id, double *host)
+{
+    if (!g.initialised || id < 1 || id >= X_COUNT) return -1;
+    const size_t w = (size_t)g.nx + 2 * g.h;
+    HIP_OK_R(hipMemcpy2DAsync(host, w * sizeof(double), g.f[id] + (g.lead - g.h), g.pitch * sizeof(double),
+                              w * sizeof(double), (size_t)g.rows, hipMemcpyDeviceToHost, g.stream), -1);
+    HIP_OK_R(hipStreamSynchronize(g.stream), -1);
+    return 0;
+}
+
+EXPORT int tea_engine_set_field(int id, const double *host)
+{
+    if (!g.initialised || id < 1 || id >= X_COUNT) return -1;
+    const size_t w = (size_t)g.nx + 2 * g.h;
+    HIP_OK_R(hipMemcpy2DAsync(g.f[id] + (g.lead - g.h), g.pitch * sizeof(double), host, w * sizeof(double),
+                              w * sizeof(double), (size_t)g.rows, hipMemcpyHostToDevice, g.stream), -1);
+    HIP_OK_R(hipStreamSynchronize(g.stream), -1);
+    return 0;
+}
+
+EXPORT void tea_engine_profile_enable(int on)
+{
+    if (!on && g.stream) {
+        (void)hipStreamSynchronize(g.stream);
+        g.prof.drain(true);
+    }
+    g.prof.on = on != 0;
+}
+EXPORT void tea_engine_profile_reset(void)
+{
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    g.prof.drain(true);
+    for (auto &c : g.prof.counts) c = 0;
+    for (auto &m : g.prof.ms) m = 0.0;
+}
+EXPORT int tea_engine_profile_get(const char *kernel, long *count, double *total_ms)
+{
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    g.prof.drain(true);
+    auto it = g.prof.ids.find(kernel);
+    if (it == g.prof.ids.end()) {
+        *count = 0;
+        *total_ms = 0.0;
+        return -1;
+    }
+    *count = g.prof.counts[it->second];
+    *total_ms = g.prof.ms[it->second];
+    return 0;
+}
+EXPORT int tea_engine_profile_names(char *buf, int buflen)
+{
+    std::string all;
+    for (auto &n : g.prof.names) all += n + "\n";
+    if (buf && buflen > 0) snprintf(buf, (size_t)buflen, "%s", all.c_str());
+    return (int)all.size();
+}
+EXPORT void *tea_engine_stream(void) { return (void *)g.stream; }
+EXPORT void tea_engine_synchronize(void)
+{
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    if (g.prof.on) g.prof.drain(false);
+}
+EXPORT void tea_engine_finalize(void) { release_all(); }

@@ -1,0 +1,226 @@
+// engine_device.hpp -- device-side building blocks of the TeaLeaf engine (gfx950).
+//
+// Work decomposition: a workgroup is 4 wavefronts of 64 lanes.  A wavefront owns
+// a 64-column x 4-row tile; each lane owns one column of it ("strip"), so the
+// 4-cell vertical tridiagonal of the block-Jacobi preconditioner
+// (tea_block_jacobi.cuknl:11-46) is solved in registers by one lane, and a
+// lane's vertical stencil neighbours are reused from registers.  The 4 waves
+// of a workgroup stack vertically: a workgroup covers 64 x 16 cells.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs; tile_of_block() maps
+// them so that each XCD walks one contiguous band of tiles, keeping the rows
+// shared by vertically adjacent tiles in the same L2.
+//
+// Reductions use compensated (double-double) accumulation: products are split
+// exactly with fma, sums carried with TwoSum.  The rounded result is the
+// correctly rounded dot product for all practical purposes, so it does not
+// depend on the order in which lanes, waves and workgroups are combined.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tea {
+
+constexpr int WAVE = 64;
+constexpr int TILE_X = 64;
+constexpr int STRIP = 4;          // block-Jacobi strip height (JACOBI_BLOCK_SIZE)
+constexpr int WAVES = 4;
+constexpr int TILE_Y = STRIP * WAVES;
+constexpr int BLOCK = WAVE * WAVES;
+constexpr int NUM_XCD = 8;
+constexpr int MAX_RED = 4;        // values reduced by one launch at most
+
+enum { PREC_NONE = 1, PREC_DIAG = 2, PREC_BLOCK = 3 };
+
+// Inclusive cell extent in reference coordinates (column, row; halo included).
+struct Ext {
+    int x0, x1, y0, y1;
+};
+
+struct DD {
+    double s, c;
+};
+
+__device__ __forceinline__ void dd_add_prod(DD &a, double x, double y)
+{
+    const double p = x * y;
+    const double pe = __builtin_fma(x, y, -p);
+    const double s = a.s + p;
+    const double bb = s - a.s;
+    const double se = (a.s - (s - bb)) + (p - bb);
+    a.s = s;
+    a.c += se + pe;
+}
+
+__device__ __forceinline__ void dd_merge(DD &a, const DD b)
+{
+    const double s = a.s + b.s;
+    const double bb = s - a.s;
+    const double se = (a.s - (s - bb)) + (b.s - bb);
+    a.s = s;
+    a.c += (se + b.c);
+}
+
+__device__ __forceinline__ DD wave_reduce(DD v)
+{
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        DD o;
+        o.s = __shfl_xor(v.s, off, WAVE);
+        o.c = __shfl_xor(v.c, off, WAVE);
+        dd_merge(v, o);
+    }
+    return v;
+}
+
+// Reduction context: per-workgroup partials, a completion counter, and the
+// destination of the final values (pinned host memory mapped into the GPU).
+struct Red {
+    DD *partials;        // MAX_RED * max_blocks
+    unsigned *counter;
+    double *out;         // MAX_RED doubles
+    int max_blocks;
+};
+
+// Combines NV per-lane accumulators over the whole grid.  Every workgroup
+// publishes its partial; the last one to finish combines all partials in
+// workgroup order and writes out[0..NV).
+template <int NV>
+__device__ __forceinline__ void grid_reduce(DD (&v)[NV], const Red red)
+{
+    __shared__ DD lds[NV][WAVES];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int j = 0; j < NV; j++) v[j] = wave_reduce(v[j]);
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) lds[j][wave] = v[j];
+    }
+    __syncthreads();
+    const int nblocks = (int)gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            DD t = lds[j][0];
+            for (int w = 1; w < WAVES; w++) dd_merge(t, lds[j][w]);
+            red.partials[j * red.max_blocks + blockIdx.x] = t;
+        }
+        __threadfence();
+        const unsigned prev = atomicAdd(red.counter, 1u);
+        is_last = (prev == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    DD acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        acc[j].s = 0.0;
+        acc[j].c = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
+            DD p;
+            p.s = __hip_atomic_load(&red.partials[j * red.max_blocks + b].s, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+            p.c = __hip_atomic_load(&red.partials[j * red.max_blocks + b].c, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+            dd_merge(acc[j], p);
+        }
+        acc[j] = wave_reduce(acc[j]);
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) lds[j][wave] = acc[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            DD t = lds[j][0];
+            for (int w = 1; w < WAVES; w++) dd_merge(t, lds[j][w]);
+            red.out[j] = t.s + t.c;
+        }
+        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Tile (tx, ty) of this workgroup, XCD-aware: workgroup b runs on XCD b % 8;
+// XCD j walks tiles [start_j, start_j + count_j) in row-major tile order.
+__device__ __forceinline__ void tile_of_block(int nbx, int nblocks, int &tx, int &ty)
+{
+    const int b = (int)blockIdx.x;
+    const int xcd = b % NUM_XCD;
+    const int k = b / NUM_XCD;
+    const int q = nblocks / NUM_XCD;
+    const int rem = nblocks % NUM_XCD;
+    const int start = xcd * q + (xcd < rem ? xcd : rem);
+    const int t = start + k;
+    ty = t / nbx;
+    tx = t - ty * nbx;
+}
+
+// Column and first row of this lane's strip in extent e; len = rows in the strip (0 if none).
+struct Strip {
+    int c, r0, len;
+};
+
+__device__ __forceinline__ Strip strip_of_lane(const Ext e)
+{
+    const int nbx = (e.x1 - e.x0 + TILE_X) / TILE_X;
+    const int nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
+    int tx, ty;
+    tile_of_block(nbx, nbx * nby, tx, ty);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    Strip s;
+    s.c = e.x0 + tx * TILE_X + lane;
+    s.r0 = e.y0 + ty * TILE_Y + wave * STRIP;
+    int len = e.y1 - s.r0 + 1;
+    len = len < STRIP ? len : STRIP;
+    if (s.c > e.x1 || len < 0) len = 0;
+    s.len = len;
+    return s;
+}
+
+// 5-point operator (I - dt div K grad) x at flat index i, in the expression
+// order of tea_leaf_cg.cuknl:53-57.
+__device__ __forceinline__ double apply_A(const double *__restrict__ x, const double *__restrict__ Kx,
+                                          const double *__restrict__ Ky, long i, long pitch)
+{
+    return (1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * x[i]
+           - (Ky[i + pitch] * x[i + pitch] + Ky[i] * x[i - pitch])
+           - (Kx[i + 1] * x[i + 1] + Kx[i] * x[i - 1]);
+}
+
+// Diagonal of the operator at i (COEF_B of tea_block_jacobi.cuknl:6).
+__device__ __forceinline__ double diag_A(const double *__restrict__ Kx, const double *__restrict__ Ky, long i,
+                                         long pitch)
+{
+    return 1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i]);
+}
+
+// z = M_block^-1 rhs on one strip (Thomas algorithm, tea_block_jacobi.cuknl:11-46).
+// i0 = flat index of the strip's first cell; len in 1..4.
+__device__ __forceinline__ void block_solve_strip(const double (&rhs)[STRIP], double (&z)[STRIP], int len,
+                                                  const double *__restrict__ cp, const double *__restrict__ bfp,
+                                                  const double *__restrict__ Kx, const double *__restrict__ Ky,
+                                                  long i0, long pitch)
+{
+    double dp[STRIP];
+    dp[0] = rhs[0] / diag_A(Kx, Ky, i0, pitch);
+#pragma unroll
+    for (int k = 1; k < STRIP; k++) {
+        if (k < len) {
+            const long i = i0 + k * pitch;
+            dp[k] = (rhs[k] - (-Ky[i]) * dp[k - 1]) * bfp[i];
+        }
+    }
+#pragma unroll
+    for (int k = STRIP - 1; k >= 0; k--) {
+        if (k == len - 1) z[k] = dp[k];
+        else if (k < len - 1) z[k] = dp[k] - cp[i0 + k * pitch] * z[k + 1];
+    }
+}
+
+}  // namespace tea

@@ -1,0 +1,192 @@
+"""tea.in decks and tea.out logs.
+
+* ``Deck`` writes the ``*tea ... *endtea`` keyword deck read by read_input.f90
+  (and, for its solver keys, by the kernel layer);
+* ``standard_states`` is the 4-rectangle layout of the reference decks
+  (tea.in:2-6, Benchmarks/tea_bm_*.in);
+* ``random_layout`` is a seeded layout of 3-8 rectangles with log-uniform
+  density in [0.1, 100] and energy in [1e-4, 25] on a density-100 /
+  energy-1e-4 background;
+* ``parse_tea_out`` reads the field-summary lines and per-step iteration counts
+  of a tea.out log (field_summary.f90:83, tea_solve.f90:444-455).
+"""
+from __future__ import annotations
+
+import math
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def _f(x: float) -> str:
+    """A real literal that both deck readers parse exactly (always has a '.')."""
+    s = repr(float(x))
+    mant, _, exp = s.partition("e")
+    if "." not in mant:
+        mant += ".0"
+    return mant + ("e" + exp if exp else "")
+
+
+@dataclass
+class StateSpec:
+    density: float
+    energy: float
+    geometry: str | None = None      # "rectangle" | "circle" | "point"
+    xmin: float = 0.0
+    xmax: float = 0.0
+    ymin: float = 0.0
+    ymax: float = 0.0
+    radius: float = 0.0
+
+    def line(self, n: int) -> str:
+        s = f"state {n} density={_f(self.density)} energy={_f(self.energy)}"
+        if self.geometry:
+            s += f" geometry={self.geometry}"
+            if self.geometry == "rectangle":
+                s += f" xmin={_f(self.xmin)} xmax={_f(self.xmax)} ymin={_f(self.ymin)} ymax={_f(self.ymax)}"
+            elif self.geometry == "circle":
+                s += f" xmin={_f(self.xmin)} ymin={_f(self.ymin)} radius={_f(self.radius)}"
+            else:
+                s += f" xmin={_f(self.xmin)} ymin={_f(self.ymin)}"
+        return s
+
+
+def standard_states() -> list[StateSpec]:
+    """The reference's 4-rectangle problem on [0,10]^2 (tea.in:2-6)."""
+    return [
+        StateSpec(100.0, 0.0001),
+        StateSpec(0.1, 25.0, "rectangle", 0.0, 1.0, 1.0, 2.0),
+        StateSpec(0.1, 0.1, "rectangle", 1.0, 6.0, 1.0, 2.0),
+        StateSpec(0.1, 0.1, "rectangle", 5.0, 6.0, 1.0, 8.0),
+        StateSpec(0.1, 0.1, "rectangle", 5.0, 10.0, 7.0, 8.0),
+    ]
+
+
+def random_layout(seed: int, n_min: int = 3, n_max: int = 8, cells: int = 4096, extent: float = 10.0) -> list[StateSpec]:
+    """Seeded layout: background density 100, energy 1e-4, then 3-8 axis-aligned
+    rectangles (later ones overwrite earlier ones) with density 10^U(-1,2) and
+    energy 10^U(-4, log10 25); each side at least two cells wide."""
+    rng = np.random.default_rng(seed)
+    states = [StateSpec(100.0, 0.0001)]
+    n = int(rng.integers(n_min, n_max + 1))
+    min_w = 2.0 * extent / cells
+    for _ in range(n):
+        dens = float(10.0 ** rng.uniform(-1.0, 2.0))
+        ener = float(10.0 ** rng.uniform(-4.0, math.log10(25.0)))
+        box = []
+        for _axis in range(2):
+            while True:
+                a, b = sorted(rng.uniform(0.0, extent, size=2).tolist())
+                if b - a >= min_w:
+                    break
+            box += [a, b]
+        states.append(StateSpec(dens, ener, "rectangle", box[0], box[1], box[2], box[3]))
+    return states
+
+
+@dataclass
+class Deck:
+    x_cells: int = 256
+    y_cells: int = 256
+    xmin: float = 0.0
+    ymin: float = 0.0
+    xmax: float = 10.0
+    ymax: float = 10.0
+    initial_timestep: float = 0.004
+    end_step: int = 10
+    solver: str = "cg"                 # cg | chebyshev | ppcg | jacobi
+    preconditioner: str = "none"       # none | jac_diag | jac_block
+    eps: float = 1.0e-15
+    max_iters: int = 10000
+    halo_depth: int = 1
+    ppcg_inner_steps: int | None = None
+    ch_cg_presteps: int | None = None
+    ch_cg_epslim: float | None = None
+    check_result: bool = False
+    summary_frequency: int | None = None
+    reflective_boundary: bool = False
+    profiler_on: bool = False
+    inverse_density: bool = False
+    test_problem: int | None = None
+    states: list = field(default_factory=standard_states)
+    extra: list = field(default_factory=list)
+
+    def text(self) -> str:
+        lines = ["*tea"]
+        lines += [s.line(i + 1) for i, s in enumerate(self.states)]
+        lines += [
+            f"x_cells={self.x_cells}",
+            f"y_cells={self.y_cells}",
+            f"xmin={_f(self.xmin)}",
+            f"ymin={_f(self.ymin)}",
+            f"xmax={_f(self.xmax)}",
+            f"ymax={_f(self.ymax)}",
+            f"initial_timestep={_f(self.initial_timestep)}",
+            f"end_step={self.end_step}",
+            f"tl_max_iters={self.max_iters}",
+            f"tl_use_{self.solver}",
+            f"tl_preconditioner_type={self.preconditioner}",
+            f"tl_eps={_f(self.eps)}",
+            f"halo_depth={self.halo_depth}",
+        ]
+        if self.ppcg_inner_steps is not None:
+            lines.append(f"tl_ppcg_inner_steps={self.ppcg_inner_steps}")
+        if self.ch_cg_presteps is not None:
+            lines.append(f"tl_ch_cg_presteps={self.ch_cg_presteps}")
+        if self.ch_cg_epslim is not None:
+            lines.append(f"tl_ch_cg_epslim={_f(self.ch_cg_epslim)}")
+        if self.check_result:
+            lines.append("tl_check_result")
+        if self.summary_frequency is not None:
+            lines.append(f"summary_frequency={self.summary_frequency}")
+        if self.reflective_boundary:
+            lines.append("reflective_boundary")
+        if self.profiler_on:
+            lines.append("profiler_on")
+        if self.inverse_density:
+            lines.append("tl_coefficient_inverse_density")
+        if self.test_problem is not None:
+            lines.append(f"test_problem {self.test_problem}")
+        lines += list(self.extra)
+        lines.append("*endtea")
+        return "\n".join(lines) + "\n"
+
+    def write(self, path) -> str:
+        with open(path, "w") as f:
+            f.write(self.text())
+        return str(path)
+
+
+# ------------------------------------------------------------------ tea.out
+
+_SUMMARY = re.compile(r"^\s*step:\s*(\d+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s*$")
+_ITER = re.compile(r"^\s*Iteration count\s+(\d+)")
+_ERR = re.compile(r"^\s*Conduction error\s+(\S+)")
+
+
+def _fortran_float(s: str) -> float:
+    s = s.replace("D", "E").replace("d", "e")
+    if re.match(r"^-?\d*\.\d+[+-]\d+$", s):  # 0.1-100 style
+        s = re.sub(r"([+-]\d+)$", r"E\1", s)
+    return float(s)
+
+
+def parse_tea_out(path) -> dict:
+    """Field summaries [(step, (vol, mass, density, energy, u))], iteration
+    counts and conduction errors from a tea.out log."""
+    summaries, iterations, errors = [], [], []
+    with open(path) as f:
+        for line in f:
+            m = _SUMMARY.match(line)
+            if m:
+                summaries.append((int(m.group(1)), tuple(_fortran_float(m.group(k)) for k in range(2, 7))))
+                continue
+            m = _ITER.match(line)
+            if m:
+                iterations.append(int(m.group(1)))
+                continue
+            m = _ERR.match(line)
+            if m:
+                errors.append(_fortran_float(m.group(1)))
+    return {"summaries": summaries, "iterations": iterations, "conduction_error": errors}

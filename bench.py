@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark: ms per timestep to an FP64 relative residual of 1e-10 at 4096^2.
+
+Workload: the reference's 4-rectangle problem (tea.in:2-6 geometry on [0,10]^2)
+on a 4096 x 4096 mesh, plain FP64 PPCG with 10 inner steps and the block-Jacobi
+preconditioner, tl_eps 1e-10, dt 0.004, one chunk on one GPU.  A step is one
+timestep: solver set-up, the CG pre-steps / PPCG iterations to tl_eps, and the
+finalisation, all through the host driver over libtea_engine.so.
+
+W warm-up steps run first (the first holds the CG eigenvalue estimate), then
+exactly K timed steps.  The timed region is bracketed by device
+synchronisation; HIP events recorded on the engine's own stream give the GPU
+time of the region, and the engine's per-launch event profiler gives each
+kernel's average duration for the roofline figure.
+
+The CPU baseline runs the same deck through the same driver over the CPU
+oracle (OpenMP, plain reductions) on a bounded sample of PPCG iterations and
+extrapolates to the GPU run's iteration counts.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "ms/timestep to FP64 rel-res 1e-10 at 4096²: learned+FP32 vs plain FP64 PPCG"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per interior cell of each kernel launch: every distinct
+# FP64 array the reference kernel reads or writes, once, 8 bytes each.
+ALGO_BYTES = {
+    # one whole inner step (the reference's update_r + calc_sd_new launches, jac_block)
+    "k_ppcg_inner_fused": 112,
+    "ppcg_update_r": 40,              # rtemp, sd, Kx, Ky read; rtemp written
+    "k_ppcg_calc_sd": 72,             # jac_block: rtemp, sd, utemp, cp, bfp, Kx, Ky read; sd, utemp written
+    "cg_calc_w": 32,                  # p, Kx, Ky read; w written
+    "k_cg_calc_ur": 88,               # jac_block: u, p, r, w, cp, bfp, Kx, Ky read; u, r, z written
+    "ppcg_calc_rrn": 24,
+    "ppcg_calc_2norm": 16,
+    "ppcg_store_r": 16,
+    "ppcg_update_z": 16,
+    "ppcg_calc_p": 24,
+    "ppcg_init_sd_new": 40,           # r, z read; sd, rtemp, utemp written
+}
+
+
+def make_deck(path, n, steps, solver="ppcg", prec="jac_block", eps=1e-10, max_iters=10000, inner=10,
+              epslim=None):
+    from tealeaf_cuda_amd import Deck
+
+    d = Deck(x_cells=n, y_cells=n, end_step=steps, solver=solver, preconditioner=prec, eps=eps,
+             max_iters=max_iters, halo_depth=1, ppcg_inner_steps=inner, summary_frequency=0,
+             check_result=True, ch_cg_epslim=epslim)
+    return d.write(path)
+
+
+def run_gpu(n, warmup, steps, tmp):
+    import torch
+    import tealeaf_cuda_amd as tl
+
+    deck = make_deck(Path(tmp) / "tea.in", n, warmup + steps)
+    eng = tl.Engine()  # same library instance the driver loads (RTLD_LOCAL handles are shared per path)
+    ev = {}
+
+    def on_step(step):
+        stream = torch.cuda.ExternalStream(eng.lib.tea_engine_stream())
+        if step == warmup:
+            eng.synchronize()
+            eng.profile_reset()
+            eng.profile_enable(True)
+            ev["start"] = torch.cuda.Event(enable_timing=True)
+            ev["start"].record(stream)
+            ev["t0"] = time.perf_counter()
+        elif step == warmup + steps:
+            ev["stop"] = torch.cuda.Event(enable_timing=True)
+            ev["stop"].record(stream)
+            ev["stop"].synchronize()
+            ev["t1"] = time.perf_counter()
+            eng.profile_enable(False)
+            ev["profile"] = eng.profile()
+
+    res = tl.run_deck(deck, on_step=on_step)
+    gpu_ms = ev["start"].elapsed_time(ev["stop"])
+    wall_ms = (ev["t1"] - ev["t0"]) * 1e3
+    timed = slice(warmup, warmup + steps)
+    return res, gpu_ms, wall_ms, ev["profile"], timed
+
+
+def roofline(profile, n):
+    """Dominant kernel (largest total time) with its algorithmic bandwidth."""
+    best = None
+    for name, (count, ms) in profile.items():
+        if count and (best is None or ms > best[2]):
+            best = (name, count, ms)
+    if best is None:
+        return None, None
+    name, count, ms = best
+    per_cell = ALGO_BYTES.get(name)
+    avg_us = ms * 1e3 / count
+    out = {"kernel": name, "launches": count, "avg_us": round(avg_us, 3)}
+    if per_cell is None:
+        return out, None
+    bytes_per_launch = per_cell * n * n
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    roof = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": name,
+        "bytes_per_launch": bytes_per_launch,
+        "avg_launch_us": round(avg_us, 3),
+    }
+    return out, roof
+
+
+def cpu_baseline(n, gpu_res, timed, tmp):
+    """Oracle on the host cores: time two bounded first-step runs that differ
+    only in the number of PPCG iterations, take the per-iteration cost, and
+    scale it by the GPU run's outer iterations per timed step."""
+    import tealeaf_cuda_amd as tl
+
+    oracle = REPO / "oracle" / "liboracle.so"
+    if not oracle.exists():
+        return None
+    cores = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    os.environ["TEA_ORACLE_DOT"] = "plain"
+    # the sample switches to PPCG right after the 25 CG pre-steps (epslim
+    # lifted) so that the two runs differ only in PPCG iterations
+    presteps = 25
+    lo, hi = presteps + 2, presteps + 32
+    times = {}
+    for iters in (lo, hi):
+        deck = make_deck(Path(tmp) / f"cpu_{iters}.in", n, 1, max_iters=iters, epslim=1e30)
+        t0 = time.perf_counter()
+        tl.run_deck(deck, backend=oracle)
+        times[iters] = time.perf_counter() - t0
+    per_it = (times[hi] - times[lo]) / (hi - lo)
+    its = gpu_res.iterations[timed]
+    mean_its = sum(its) / len(its)
+    ms_step = per_it * mean_its * 1e3
+    return {
+        "value": round(ms_step, 1),
+        "unit": "ms/timestep",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle (C restatement of the kernels, OpenMP {cores} threads, plain reductions) through the "
+                   f"same driver: first timestep at {n}^2 with tl_max_iters {lo} and {hi}; "
+                   f"{per_it * 1e3:.1f} ms per PPCG iteration x {mean_its:.1f} iterations per timed GPU step"),
+        "seconds": round(times[lo] + times[hi], 1),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs one chunk on one GPU; multi-GPU runs are not part of this build "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    n = args.size
+    with tempfile.TemporaryDirectory() as tmp:
+        res, gpu_ms, wall_ms, profile, timed = run_gpu(n, args.warmup, args.steps, tmp)
+        ms_per_step = gpu_ms / args.steps
+        dominant, roof = roofline(profile, n)
+        cpu = None if args.no_cpu_baseline else cpu_baseline(n, res, timed, tmp)
+    line = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 3),
+        "unit": "ms/timestep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FP64 PPCG, jac_block, 10 inner steps, {n}x{n} cells, 4-rectangle layout, "
+                        "tl_eps 1e-10, dt 0.004, halo_depth 1",
+            "cells": n * n,
+            "solver": "ppcg",
+            "preconditioner": "jac_block",
+            "tl_eps": 1e-10,
+        },
+        "wall_ms_per_step": round(wall_ms / args.steps, 3),
+        "iterations": res.iterations[timed],
+        "ppcg_inner": res.ppcg_inner[timed],
+        "exact_rel_residual": res.exact_error[timed],
+        "roofline": roof,
+        "dominant_kernel": dominant,
+        "cpu_baseline": cpu,
+        "kernels": {k: {"launches": c, "total_ms": round(ms, 3)} for k, (c, ms) in sorted(profile.items())},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,7 @@ static struct {
     int solver, prec;
     int dot_mode;
     double *f[X_COUNT];        /* 2D fields, index = c + r*pitch */
+    double *zl;                /* block-Jacobi result kept out of z (the reference's shared-memory z_l) */
     double *xarea, *yarea;
     double *cellx, *celly, *celldx, *celldy, *vertexx, *vertexy, *vertexdx, *vertexdy;
     double *ch_a, *ch_b;
@@ -316,6 +317,7 @@ static void block_solve_extent(const double *rhs, double *z, int x0, int x1, int
 static void free_all(void)
 {
     for (int k = 0; k < X_COUNT; k++) { free(g.f[k]); g.f[k] = NULL; }
+    free(g.zl); g.zl = NULL;
     free(g.xarea); free(g.yarea);
     free(g.cellx); free(g.celly); free(g.celldx); free(g.celldy);
     free(g.vertexx); free(g.vertexy); free(g.vertexdx); free(g.vertexdy);
@@ -357,6 +359,7 @@ EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
     g.rows = g.ny + 2 * g.h;
     const size_t n2 = (size_t)g.pitch * (size_t)g.rows;
     for (int k = 1; k < X_COUNT; k++) g.f[k] = (double *)calloc(n2, sizeof(double));
+    g.zl = (double *)calloc(n2, sizeof(double));
     g.xarea = (double *)calloc((size_t)(g.pitch + 1) * (size_t)g.rows, sizeof(double));
     g.yarea = (double *)calloc((size_t)g.pitch * (size_t)(g.rows + 1), sizeof(double));
     /* one spare element: the vertex kernel writes index nx+2h (initialise_chunk_kernel.cuknl:21,43) */
@@ -731,8 +734,8 @@ EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const doub
     const double th = *theta;
     cheby_residual();
     if (g.prec == PREC_BLOCK) {
-        block_solve_extent(R, Z, IX0, IX1, IY0, IY1);  /* z_l held in shared memory in the reference */
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = Z[i] / th; });
+        block_solve_extent(R, g.zl, IX0, IX1, IY0, IY1);
+        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = g.zl[i] / th; });
     } else if (g.prec == PREC_DIAG) {
         FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = (MI[i] * R[i]) / th; });
     } else {
@@ -751,8 +754,8 @@ EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
     const double al = g.ch_a[k], be = g.ch_b[k];
     cheby_residual();
     if (g.prec == PREC_BLOCK) {
-        block_solve_extent(R, Z, IX0, IX1, IY0, IY1);
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = al * P[i] + be * Z[i]; });
+        block_solve_extent(R, g.zl, IX0, IX1, IY0, IY1);
+        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = al * P[i] + be * g.zl[i]; });
     } else if (g.prec == PREC_DIAG) {
         FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = al * P[i] + be * MI[i] * R[i]; });
     } else {
@@ -824,8 +827,8 @@ EXPORT void tea_leaf_ppcg_init_sd_kernel_cuda_(const double *theta)
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     const double th = *theta;
     if (g.prec == PREC_BLOCK) {
-        block_solve_extent(R, Z, IX0, IX1, IY0, IY1);
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = Z[i] / th; });
+        block_solve_extent(R, g.zl, IX0, IX1, IY0, IY1);
+        FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = g.zl[i] / th; });
     } else if (g.prec == PREC_DIAG) {
         FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = R[i] * MI[i] / th; });
     } else {
@@ -871,8 +874,8 @@ EXPORT void tea_leaf_ppcg_inner_kernel_cuda_(int *ppcg_cur_step, int *bounds_ext
         RTEMP[i] -= result;
     });
     if (g.prec == PREC_BLOCK) {
-        block_solve_extent(RTEMP, Z, x0, x1, y0, y1);  /* z_l of the reference (shared memory) */
-        FOR_EXTENT(x0, x1, y0, y1, { SD[i] = al * SD[i] + be * Z[i]; UTEMP[i] += SD[i]; });
+        block_solve_extent(RTEMP, g.zl, x0, x1, y0, y1);
+        FOR_EXTENT(x0, x1, y0, y1, { SD[i] = al * SD[i] + be * g.zl[i]; UTEMP[i] += SD[i]; });
     } else if (g.prec == PREC_DIAG) {
         FOR_EXTENT(x0, x1, y0, y1, { SD[i] = al * SD[i] + be * RTEMP[i] * MI[i]; UTEMP[i] += SD[i]; });
     } else {

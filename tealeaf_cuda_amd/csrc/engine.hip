@@ -23,6 +23,7 @@
 #include <fstream>
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/tea_leaf_engine.h"
@@ -39,7 +40,9 @@ namespace {
 
 enum FieldId {
     F_DENSITY = 1, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD, F_R,
-    X_U0, X_W, X_Z, X_MI, X_KX, X_KY, X_RTEMP, X_UTEMP, X_RSTORE, X_CP, X_BFP, X_VOLUME, X_COUNT
+    X_U0, X_W, X_Z, X_MI, X_KX, X_KY, X_RTEMP, X_UTEMP, X_RSTORE, X_CP, X_BFP, X_VOLUME, X_COUNT,
+    X_SD2 = X_COUNT,    // second sd buffer of the fused inner step (internal)
+    N_SLOTS
 };
 enum { SOLVER_JACOBI = 0, SOLVER_CG = 1, SOLVER_CHEBY = 2, SOLVER_PPCG = 3 };
 constexpr int S_CG_FAMILY = (1 << SOLVER_CG) | (1 << SOLVER_CHEBY) | (1 << SOLVER_PPCG);
@@ -110,21 +113,26 @@ struct Chunk {
     hipStream_t stream = nullptr;
 
     double *slab = nullptr;         // all 2D fields
-    double *f[X_COUNT] = {nullptr};  // base of each field (physical column 0)
-    double *fs[X_COUNT] = {nullptr}; // shifted: fs[id][c + r*pitch] is cell (c, r)
+    double *f[N_SLOTS] = {nullptr};  // base of each field (physical column 0)
+    double *fs[N_SLOTS] = {nullptr}; // shifted: fs[id][c + r*pitch] is cell (c, r)
     double *cellx = nullptr, *celly = nullptr, *vertexx = nullptr, *vertexy = nullptr;
     std::vector<double> h_celldx, h_celldy;
 
-    DD *partials = nullptr;
-    unsigned *counter = nullptr;
-    double *h_out = nullptr;         // pinned, MAX_RED doubles
-    double *d_out = nullptr;         // device alias of h_out
-    int max_blocks = 0;
+    DD *h_partials = nullptr;        // pinned: MAX_RED x max_blocks workgroup partials
+    DD *d_partials = nullptr;        // device alias of h_partials
+    int max_blocks = 0;              // largest grid of any launch
+    int last_grid = 0;               // grid of the latest launch
 
     double *d_halo_buf = nullptr;    // staging for pack / unpack
     size_t halo_buf_elems = 0;
 
     std::vector<double> ch_a, ch_b;  // host copies of the uploaded coefficients
+
+    // The fused inner step writes the new sd into the spare buffer, interior
+    // only; the two buffers' halo rings agree until something writes the sd
+    // halo (update_halo, unpack, set_field), after which the ring is copied
+    // across before the next fused step.
+    bool sd_halo_dirty = false;
 
     Profiler prof;
 };
@@ -204,7 +212,35 @@ int blocks_for(const Ext &e)
     return nbx * nby;
 }
 
-Red red_ctx() { return Red{g.partials, g.counter, g.d_out, g.max_blocks}; }
+Red red_ctx() { return Red{g.d_partials, g.max_blocks}; }
+
+// The PPCG inner step runs as one fused launch unless TEA_FUSED_INNER=0.
+bool fused_inner()
+{
+    static bool v = [] {
+        const char *s = getenv("TEA_FUSED_INNER");
+        return !(s && s[0] == '0');
+    }();
+    return v;
+}
+
+// Largest grid: 8 workgroups of 4 wavefronts per CU on 256 CUs fills the GPU;
+// larger extents loop over their tiles.
+int grid_max()
+{
+    static int v = [] {
+        const char *s = getenv("TEA_GRID_MAX");
+        int x = s ? atoi(s) : 0;
+        return x > 0 ? x : 2048;
+    }();
+    return v;
+}
+
+int grid_for(const Ext &e)
+{
+    const int nb = blocks_for(e);
+    return nb < grid_max() ? nb : grid_max();
+}
 
 struct Timed {
     int name = -1;
@@ -227,7 +263,8 @@ struct Timed {
 template <typename K, typename... A>
 bool launch(const char *name, K kernel, const Ext &e, A... args)
 {
-    const int nb = blocks_for(e);
+    const int nb = grid_for(e);
+    g.last_grid = nb;
     if (nb == 0) return true;
     {
         Timed t(name);
@@ -238,12 +275,28 @@ bool launch(const char *name, K kernel, const Ext &e, A... args)
     return true;
 }
 
-// Waits for the queued work and returns out[0..n) of the last reduction.
+// Host copy of the device-side compensated merge (engine_device.hpp).
+void host_merge(DD &a, const DD b)
+{
+    const double s = a.s + b.s;
+    const double bb = s - a.s;
+    const double se = (a.s - (s - bb)) + (b.s - bb);
+    a.s = s;
+    a.c += (se + b.c);
+}
+
+// Waits for the queued work and combines, in workgroup order, the partials of
+// the last reduction launch into dst[0..n).
 bool fetch(double *dst, int n)
 {
     hipError_t err = hipStreamSynchronize(g.stream);
     if (err != hipSuccess) return die("hipStreamSynchronize failed: %s", hipGetErrorString(err));
-    for (int j = 0; j < n; j++) dst[j] = g.h_out[j];
+    for (int j = 0; j < n; j++) {
+        DD t{0.0, 0.0};
+        const DD *p = g.h_partials + (size_t)j * g.max_blocks;
+        for (int b = 0; b < g.last_grid; b++) host_merge(t, p[b]);
+        dst[j] = t.s + t.c;
+    }
     if (g.prof.on) g.prof.drain(false);
     return true;
 }
@@ -255,29 +308,35 @@ bool fetch(double *dst, int n)
 // =============================================================================
 namespace {
 
-#define STRIP_BEGIN                                       \
-    const Strip s = strip_of_lane(e);                     \
-    const long i0 = (long)s.c + (long)s.r0 * pitch;       \
-    (void)i0;
+// Loop over this workgroup's tiles; the body sees the lane's strip s and i0,
+// the flat index of the strip's first cell.
+#define STRIP_BEGIN                                                    \
+    for (TileLoop tl_(e); tl_.more(); tl_.advance()) {                 \
+        const Strip s = tl_.strip();                                   \
+        const long i0 = (long)s.c + (long)s.r0 * pitch;                \
+        (void)i0;
+#define STRIP_END }
 
 // ---- set-up ---------------------------------------------------------------------
 
 // initialise_chunk_kernel.cuknl:54-70 (volume only; areas are not read on this path)
 __global__ __launch_bounds__(BLOCK) void k_init_volume(Ext e, long pitch, double *__restrict__ vol, double v)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) vol[i0 + k * pitch] = v;
+    STRIP_END
 }
 
 // generate_chunk_kernel.cuknl:85-99
 __global__ __launch_bounds__(BLOCK) void k_generate_init(Ext e, long pitch, double *__restrict__ dens,
                                                          double *__restrict__ e0, double sd, double se)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         e0[i0 + k * pitch] = se;
         dens[i0 + k * pitch] = sd;
     }
+    STRIP_END
 }
 
 // generate_chunk_kernel.cuknl:5-67 (one state)
@@ -290,7 +349,7 @@ __global__ __launch_bounds__(BLOCK) void k_generate_state(Ext e, long pitch, con
                                                           double symin, double symax, double sradius, int geom,
                                                           int g_rect, int g_circ, int g_point)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     const double x_cent = sxmin, y_cent = symin;
     for (int k = 0; k < s.len; k++) {
         const int c = s.c, r = s.r0 + k;
@@ -310,6 +369,7 @@ __global__ __launch_bounds__(BLOCK) void k_generate_state(Ext e, long pitch, con
             dens[i0 + k * pitch] = sdens;
         }
     }
+    STRIP_END
 }
 
 // generate_chunk_kernel.cuknl:69-83: u = u0 = energy*density
@@ -317,12 +377,13 @@ __global__ __launch_bounds__(BLOCK) void k_init_u(Ext e, long pitch, const doubl
                                                   const double *__restrict__ energy, double *__restrict__ u,
                                                   double *__restrict__ u0)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         u[i] = energy[i] * dens[i];
         u0[i] = energy[i] * dens[i];
     }
+    STRIP_END
 }
 
 // Generic a = b copy (set_field_kernel.cuknl:5-16, ppcg store_r / update_z,
@@ -330,8 +391,9 @@ __global__ __launch_bounds__(BLOCK) void k_init_u(Ext e, long pitch, const doubl
 __global__ __launch_bounds__(BLOCK) void k_copy(Ext e, long pitch, double *__restrict__ dst,
                                                 const double *__restrict__ src)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) dst[i0 + k * pitch] = src[i0 + k * pitch];
+    STRIP_END
 }
 
 // field_summary_kernel.cuknl:5-58
@@ -340,8 +402,8 @@ __global__ __launch_bounds__(BLOCK) void k_field_summary(Ext e, long pitch, cons
                                                          const double *__restrict__ energy,
                                                          const double *__restrict__ u, Red red)
 {
-    STRIP_BEGIN;
     DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         const double cell_vol = vol[i];
@@ -351,7 +413,8 @@ __global__ __launch_bounds__(BLOCK) void k_field_summary(Ext e, long pitch, cons
         dd_add_prod(acc[2], cell_mass, energy[i]);
         dd_add_prod(acc[3], cell_mass, u[i]);
     }
-    grid_reduce<4>(acc, red);
+    STRIP_END
+    block_reduce<4>(acc, red);
 }
 
 // ---- halos: reflective update and pack / unpack ------------------------------------
@@ -376,6 +439,24 @@ __global__ void k_reflect_lr(double *__restrict__ a, long pitch, int h, int nx, 
     const long r = h - d + j;
     if (!right) a[(h - k) + r * pitch] = a[(h + k - 1) + r * pitch];
     else a[(h + nx - 1 + k) + r * pitch] = a[(h + nx - k) + r * pitch];
+}
+
+// Copies the halo ring (every cell outside the interior) of src into dst:
+// bottom rows, top rows, then the left and right columns of the interior rows.
+__global__ void k_copy_ring(double *__restrict__ dst, const double *__restrict__ src, long pitch, int h, int nx,
+                            int ny)
+{
+    const long w = nx + 2L * h;
+    const long n_bt = w * h;           // cells in the bottom (or top) band
+    const long n_lr = (long)ny * h;    // cells in the left (or right) band
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long c, r;
+    if (j < n_bt) { c = j % w; r = j / w; }
+    else if ((j -= n_bt) < n_bt) { c = j % w; r = h + ny + j / w; }
+    else if ((j -= n_bt) < n_lr) { c = j % h; r = h + j / h; }
+    else if ((j -= n_lr) < n_lr) { c = h + nx + j % h; r = h + j / h; }
+    else return;
+    dst[c + r * pitch] = src[c + r * pitch];
 }
 
 // pack_kernel.cuknl:5-147 buffer layout: left/right strips are (ny+2d) x d with
@@ -416,7 +497,7 @@ __global__ __launch_bounds__(BLOCK) void k_init_common(Ext e, long pitch, const 
                                                        double *__restrict__ Kx, double *__restrict__ Ky,
                                                        double rx, double ry, int coef, long i_origin)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         double dc = 0, dl = 0, dd = 0;
@@ -436,6 +517,7 @@ __global__ __launch_bounds__(BLOCK) void k_init_common(Ext e, long pitch, const 
         Kx[i] = kx;
         Ky[i] = ky;
     }
+    STRIP_END
 }
 
 // tea_leaf_common.cuknl:99-129 over +-h
@@ -443,7 +525,7 @@ __global__ __launch_bounds__(BLOCK) void k_zero_boundaries(Ext e, long pitch, do
                                                            double *__restrict__ Ky, int h, int nx, int ny, int zl,
                                                            int zr, int zb, int zt)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         const int c = s.c, r = s.r0 + k;
@@ -452,6 +534,7 @@ __global__ __launch_bounds__(BLOCK) void k_zero_boundaries(Ext e, long pitch, do
         if (zb && r <= h) Ky[i] = 0;
         if (zt && r > (ny - 1) + h) Ky[i] = 0;
     }
+    STRIP_END
 }
 
 // tea_leaf_common.cuknl:19-39: r = u0 - A u
@@ -460,7 +543,7 @@ __global__ __launch_bounds__(BLOCK) void k_calc_residual(Ext e, long pitch, cons
                                                          const double *__restrict__ Kx,
                                                          const double *__restrict__ Ky)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
@@ -469,38 +552,42 @@ __global__ __launch_bounds__(BLOCK) void k_calc_residual(Ext e, long pitch, cons
             r[i] = u0[i] - smvp;
         }
     }
+    STRIP_END
 }
 
 // tea_leaf_common.cuknl:41-58: sum a*b
 __global__ __launch_bounds__(BLOCK) void k_dot(Ext e, long pitch, const double *__restrict__ a,
                                                const double *__restrict__ b, Red red)
 {
-    STRIP_BEGIN;
     DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
         if (k < s.len) dd_add_prod(acc[0], a[i0 + k * pitch], b[i0 + k * pitch]);
-    grid_reduce<1>(acc, red);
+    STRIP_END
+    block_reduce<1>(acc, red);
 }
 
 // tea_leaf_common.cuknl:5-17: energy1 = u/density
 __global__ __launch_bounds__(BLOCK) void k_finalise(Ext e, long pitch, const double *__restrict__ dens,
                                                     const double *__restrict__ u, double *__restrict__ e1)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) e1[i0 + k * pitch] = u[i0 + k * pitch] / dens[i0 + k * pitch];
+    STRIP_END
 }
 
 // tea_leaf_common.cuknl:131-147 (interior): Mi = 1/diag
 __global__ __launch_bounds__(BLOCK) void k_jac_diag(Ext e, long pitch, double *__restrict__ Mi,
                                                     const double *__restrict__ Kx, const double *__restrict__ Ky)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         const double diag = diag_A(Kx, Ky, i, pitch);
         Mi[i] = 1.0 / diag;
     }
+    STRIP_END
 }
 
 // tea_block_jacobi.cuknl:83-111: Thomas factors cp, bfp per strip
@@ -508,14 +595,15 @@ __global__ __launch_bounds__(BLOCK) void k_block_init(Ext e, long pitch, double 
                                                       double *__restrict__ bfp, const double *__restrict__ Kx,
                                                       const double *__restrict__ Ky)
 {
-    STRIP_BEGIN;
-    if (s.len == 0) return;
+    STRIP_BEGIN
+    if (s.len == 0) continue;
     cp[i0] = (-Ky[i0 + pitch]) / diag_A(Kx, Ky, i0, pitch);
     for (int k = 1; k < s.len; k++) {
         const long i = i0 + k * pitch;
         bfp[i] = 1.0 / (diag_A(Kx, Ky, i, pitch) - (-Ky[i]) * cp[i - pitch]);
         cp[i] = (-Ky[i + pitch]) * bfp[i];
     }
+    STRIP_END
 }
 
 // z = M^-1 r on a strip held in registers (block: Thomas; diag: r*Mi).
@@ -547,8 +635,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_init_p(Ext e, long pitch, double *
                                                      const double *__restrict__ bfp, const double *__restrict__ Kx,
                                                      const double *__restrict__ Ky, Red red)
 {
-    STRIP_BEGIN;
     DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
     double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
@@ -569,7 +657,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_init_p(Ext e, long pitch, double *
             dd_add_prod(acc[0], rv[k], pv);
         }
     }
-    grid_reduce<1>(acc, red);
+    STRIP_END
+    block_reduce<1>(acc, red);
 }
 
 // tea_leaf_cg.cuknl:43-68: w = A p, sum p*w
@@ -577,8 +666,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Ext e, long pitch, const do
                                                      double *__restrict__ w, const double *__restrict__ Kx,
                                                      const double *__restrict__ Ky, Red red)
 {
-    STRIP_BEGIN;
     DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
@@ -588,7 +677,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Ext e, long pitch, const do
             dd_add_prod(acc[0], p[i], wv);
         }
     }
-    grid_reduce<1>(acc, red);
+    STRIP_END
+    block_reduce<1>(acc, red);
 }
 
 // tea_leaf_cg.cuknl:72-137: u += a p; r -= a w; z = M^-1 r; sum r*z (r*r)
@@ -600,8 +690,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Ext e, long pitch, double 
                                                       const double *__restrict__ bfp, const double *__restrict__ Kx,
                                                       const double *__restrict__ Ky, Red red)
 {
-    STRIP_BEGIN;
     DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
     double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
@@ -624,20 +714,22 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Ext e, long pitch, double 
             }
         }
     }
-    grid_reduce<1>(acc, red);
+    STRIP_END
+    block_reduce<1>(acc, red);
 }
 
 // tea_leaf_cg.cuknl:141-161 and tea_leaf_ppcg.cuknl:301-313: p = src + beta p
 __global__ __launch_bounds__(BLOCK) void k_calc_p(Ext e, long pitch, double beta, double *__restrict__ p,
                                                   const double *__restrict__ src)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
         if (k < s.len) {
             const long i = i0 + k * pitch;
             p[i] = src[i] + beta * p[i];
         }
+    STRIP_END
 }
 
 // ---- solver: Chebyshev --------------------------------------------------------------
@@ -653,7 +745,7 @@ __global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const doub
                                                    const double *__restrict__ Ky, double theta, double alpha,
                                                    double beta)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
@@ -683,16 +775,18 @@ __global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const doub
             p[i] = pv;
         }
     }
+    STRIP_END
 }
 
 // tea_leaf_cheby.cuknl:71-82: u += p
 __global__ __launch_bounds__(BLOCK) void k_add(Ext e, long pitch, double *__restrict__ u,
                                                const double *__restrict__ p)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
         if (k < s.len) u[i0 + k * pitch] += p[i0 + k * pitch];
+    STRIP_END
 }
 
 // ---- solver: Jacobi -------------------------------------------------------------------
@@ -702,8 +796,8 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Ext e, long pitch, const doubl
                                                   const double *__restrict__ Ky, const double *__restrict__ u0,
                                                   double *__restrict__ u, const double *__restrict__ un, Red red)
 {
-    STRIP_BEGIN;
     DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
@@ -715,7 +809,8 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Ext e, long pitch, const doubl
             dd_add_prod(acc[0], fabs(uv - un[i]), 1.0);
         }
     }
-    grid_reduce<1>(acc, red);
+    STRIP_END
+    block_reduce<1>(acc, red);
 }
 
 // ---- solver: PPCG ---------------------------------------------------------------------
@@ -728,8 +823,8 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_init(Ext e, long pitch, double *
                                                      const double *__restrict__ bfp, const double *__restrict__ Kx,
                                                      const double *__restrict__ Ky, int step, Red red)
 {
-    STRIP_BEGIN;
     DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
     double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
@@ -759,7 +854,8 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_init(Ext e, long pitch, double *
             }
         }
     }
-    grid_reduce<1>(acc, red);
+    STRIP_END
+    block_reduce<1>(acc, red);
 }
 
 // tea_leaf_ppcg.cuknl:5-55 (init_sd, not called by the reference driver)
@@ -771,7 +867,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd(Ext e, long pitch, const
                                                         const double *__restrict__ Kx,
                                                         const double *__restrict__ Ky, double theta)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
@@ -786,6 +882,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd(Ext e, long pitch, const
             else sd[i] = rv[k] / theta;
         }
     }
+    STRIP_END
 }
 
 // tea_leaf_ppcg.cuknl:58-89: sd = (z or r) * (1/theta); rtemp = r; utemp = sd
@@ -795,7 +892,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd_new(Ext e, long pitch, c
                                                             double *__restrict__ rtemp,
                                                             double *__restrict__ utemp, double theta_r)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
@@ -806,6 +903,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd_new(Ext e, long pitch, c
             utemp[i] = sdv;
         }
     }
+    STRIP_END
 }
 
 // tea_leaf_ppcg.cuknl:154-173: rtemp -= A sd
@@ -814,7 +912,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_update_r(Ext e, long pitch, doub
                                                          const double *__restrict__ Ky,
                                                          const double *__restrict__ sd)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
@@ -823,6 +921,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_update_r(Ext e, long pitch, doub
             rtemp[i] -= result;
         }
     }
+    STRIP_END
 }
 
 // tea_leaf_ppcg.cuknl:178-244: sd = alpha sd + beta M^-1 rtemp; utemp += sd
@@ -835,7 +934,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_calc_sd(Ext e, long pitch, doubl
                                                         const double *__restrict__ Kx,
                                                         const double *__restrict__ Ky, double alpha, double beta)
 {
-    STRIP_BEGIN;
+    STRIP_BEGIN
     double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
@@ -853,6 +952,48 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_calc_sd(Ext e, long pitch, doubl
             utemp[i] += sdv;
         }
     }
+    STRIP_END
+}
+
+// One PPCG inner step in one pass (tea_leaf_ppcg.cuknl:154-173 then :178-244):
+// rtemp -= A sd; sd' = alpha sd + beta M^-1 rtemp; utemp += sd'.  sd is read
+// from sd_in and the new sd written to sd_out, so every cell sees the previous
+// step's sd exactly as the two separate launches of the reference do.
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_inner_fused(Ext e, long pitch, const double *__restrict__ sd_in,
+                                                            double *__restrict__ sd_out, double *__restrict__ rtemp,
+                                                            double *__restrict__ utemp,
+                                                            const double *__restrict__ Mi,
+                                                            const double *__restrict__ cp,
+                                                            const double *__restrict__ bfp,
+                                                            const double *__restrict__ Kx,
+                                                            const double *__restrict__ Ky, double alpha, double beta)
+{
+    STRIP_BEGIN
+    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const double result = apply_A(sd_in, Kx, Ky, i, pitch);
+            rv[k] = rtemp[i] - result;
+            rtemp[i] = rv[k];
+        }
+    }
+    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            double sdv;
+            if (PREC == PREC_BLOCK) sdv = alpha * sd_in[i] + beta * zv[k];
+            else if (PREC == PREC_DIAG) sdv = alpha * sd_in[i] + beta * rv[k] * Mi[i];
+            else sdv = alpha * sd_in[i] + beta * rv[k];
+            sd_out[i] = sdv;
+            utemp[i] += sdv;
+        }
+    }
+    STRIP_END
 }
 
 // tea_leaf_ppcg.cuknl:279-297: sum (r - r_store)*z
@@ -860,15 +1001,16 @@ __global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const dou
                                                     const double *__restrict__ r, const double *__restrict__ z,
                                                     Red red)
 {
-    STRIP_BEGIN;
     DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
 #pragma unroll
     for (int k = 0; k < STRIP; k++)
         if (k < s.len) {
             const long i = i0 + k * pitch;
             dd_add_prod(acc[0], r[i] - rstore[i], z[i]);
         }
-    grid_reduce<1>(acc, red);
+    STRIP_END
+    block_reduce<1>(acc, red);
 }
 
 }  // namespace
@@ -892,9 +1034,7 @@ void release_all()
     g.prof.pool.clear();
     if (g.slab) (void)hipFree(g.slab);
     if (g.cellx) (void)hipFree(g.cellx);
-    if (g.partials) (void)hipFree(g.partials);
-    if (g.counter) (void)hipFree(g.counter);
-    if (g.h_out) (void)hipHostFree(g.h_out);
+    if (g.h_partials) (void)hipHostFree(g.h_partials);
     if (g.d_halo_buf) (void)hipFree(g.d_halo_buf);
     if (g.stream) (void)hipStreamDestroy(g.stream);
     Profiler keep = g.prof;
@@ -949,9 +1089,9 @@ EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
     HIP_OK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     const size_t field_elems = (size_t)g.pitch * (size_t)g.rows + 16;  // + slack, keeps 128-byte alignment
     const size_t field_stride = (field_elems + 31) / 32 * 32;
-    HIP_OK(hipMalloc(&g.slab, field_stride * X_COUNT * sizeof(double)));
-    HIP_OK(hipMemsetAsync(g.slab, 0, field_stride * X_COUNT * sizeof(double), g.stream));
-    for (int k = 1; k < X_COUNT; k++) {
+    HIP_OK(hipMalloc(&g.slab, field_stride * N_SLOTS * sizeof(double)));
+    HIP_OK(hipMemsetAsync(g.slab, 0, field_stride * N_SLOTS * sizeof(double), g.stream));
+    for (int k = 1; k < N_SLOTS; k++) {
         g.f[k] = g.slab + field_stride * k;
         g.fs[k] = g.f[k] + (g.lead - g.h);
     }
@@ -962,12 +1102,9 @@ EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
     g.celly = g.vertexx + nxv;
     g.vertexy = g.celly + nyv;
 
-    g.max_blocks = blocks_for(Ext{0, g.nx + 2 * g.h - 1, 0, g.ny + 2 * g.h - 1}) + 1;
-    HIP_OK(hipMalloc(&g.partials, (size_t)MAX_RED * g.max_blocks * sizeof(DD)));
-    HIP_OK(hipMalloc(&g.counter, sizeof(unsigned)));
-    HIP_OK(hipMemsetAsync(g.counter, 0, sizeof(unsigned), g.stream));
-    HIP_OK(hipHostMalloc(&g.h_out, MAX_RED * sizeof(double), hipHostMallocDefault));
-    HIP_OK(hipHostGetDevicePointer((void **)&g.d_out, g.h_out, 0));
+    g.max_blocks = grid_max();
+    HIP_OK(hipHostMalloc(&g.h_partials, (size_t)MAX_RED * g.max_blocks * sizeof(DD), hipHostMallocDefault));
+    HIP_OK(hipHostGetDevicePointer((void **)&g.d_partials, g.h_partials, 0));
 
     g.halo_buf_elems = (size_t)TEA_NUM_FIELDS * (size_t)g.h * (size_t)((g.nx > g.ny ? g.nx : g.ny) + 2 * g.h);
     HIP_OK(hipMalloc(&g.d_halo_buf, g.halo_buf_elems * sizeof(double)));
@@ -1053,6 +1190,7 @@ EXPORT void update_halo_kernel_cuda_(const int *chunk_neighbours, const int *fie
     CHECK_INIT();
     const int d = *depth;
     if (d < 1 || d > g.h) { die("update_halo: invalid depth %d (halo depth %d)", d, g.h); return; }
+    if (fields[TEA_FIELD_SD - 1] == 1) g.sd_halo_dirty = true;
     for (int f = 1; f <= TEA_NUM_FIELDS; f++) {
         if (fields[f - 1] != 1) continue;
         double *a = F(f);
@@ -1097,6 +1235,7 @@ void pack_unpack(int *fields, int *offsets, int d, int face, int pack, double *h
     }
     if (n == 0) return;
     if ((size_t)end > g.halo_buf_elems) { die("pack: buffer of %ld elements exceeds staging", end); return; }
+    if (!pack && fields[TEA_FIELD_SD - 1]) g.sd_halo_dirty = true;
     if (!pack) HIP_OK(hipMemcpyAsync(g.d_halo_buf, host_buffer, end * sizeof(double), hipMemcpyHostToDevice, g.stream));
     const int len = lr ? g.ny + 2 * d : g.nx + 2 * d;
     for (int f = 0; f < TEA_NUM_FIELDS; f++) {
@@ -1337,6 +1476,23 @@ EXPORT void tea_leaf_ppcg_inner_kernel_cuda_(int *ppcg_cur_step, int *bounds_ext
     e.x1 = h + g.nx - 1 + ((chunk_neighbours[TEA_CHUNK_RIGHT - 1] == TEA_EXTERNAL_FACE) ? 0 : be);
     e.y0 = (chunk_neighbours[TEA_CHUNK_BOTTOM - 1] == TEA_EXTERNAL_FACE) ? h : h - be;
     e.y1 = h + g.ny - 1 + ((chunk_neighbours[TEA_CHUNK_TOP - 1] == TEA_EXTERNAL_FACE) ? 0 : be);
+    if (fused_inner()) {
+        if (g.sd_halo_dirty) {
+            const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+            Timed t("sd_halo_ring");
+            hipLaunchKernelGGL(k_copy_ring, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream, F(X_SD2),
+                               (const double *)F(F_SD), g.pitch, g.h, g.nx, g.ny);
+            g.sd_halo_dirty = false;
+        }
+        // the new sd goes to the spare buffer, which then becomes the sd field
+        if (!PREC_SWITCH(g.prec, k_ppcg_inner_fused, e, g.pitch, (const double *)F(F_SD), F(X_SD2), F(X_RTEMP),
+                         F(X_UTEMP), (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
+                         (const double *)F(X_KX), (const double *)F(X_KY), g.ch_a[k], g.ch_b[k]))
+            return;
+        std::swap(g.f[F_SD], g.f[X_SD2]);
+        std::swap(g.fs[F_SD], g.fs[X_SD2]);
+        return;
+    }
     if (!launch("ppcg_update_r", k_ppcg_update_r, e, g.pitch, F(X_RTEMP), (const double *)F(X_KX),
                 (const double *)F(X_KY), (const double *)F(F_SD)))
         return;
@@ -1419,6 +1575,7 @@ EXPORT int tea_engine_get_field(int id, double *host)
 EXPORT int tea_engine_set_field(int id, const double *host)
 {
     if (!g.initialised || id < 1 || id >= X_COUNT) return -1;
+    if (id == F_SD) g.sd_halo_dirty = true;
     const size_t w = (size_t)g.nx + 2 * g.h;
     HIP_OK_R(hipMemcpy2DAsync(g.f[id] + (g.lead - g.h), g.pitch * sizeof(double), host, w * sizeof(double),
                               w * sizeof(double), (size_t)g.rows, hipMemcpyHostToDevice, g.stream), -1);

@@ -72,23 +72,20 @@ __device__ __forceinline__ DD wave_reduce(DD v)
     return v;
 }
 
-// Reduction context: per-workgroup partials, a completion counter, and the
-// destination of the final values (pinned host memory mapped into the GPU).
+// Reduction output: one compensated partial per workgroup and value, written
+// straight into pinned host memory; the host combines them in workgroup order
+// once the launch has completed.  No atomics: every workgroup owns its slot.
 struct Red {
-    DD *partials;        // MAX_RED * max_blocks
-    unsigned *counter;
-    double *out;         // MAX_RED doubles
+    DD *partials;        // MAX_RED * max_blocks, host memory mapped into the GPU
     int max_blocks;
 };
 
-// Combines NV per-lane accumulators over the whole grid.  Every workgroup
-// publishes its partial; the last one to finish combines all partials in
-// workgroup order and writes out[0..NV).
+// Combines NV per-lane accumulators over the workgroup and publishes the
+// workgroup's partials.
 template <int NV>
-__device__ __forceinline__ void grid_reduce(DD (&v)[NV], const Red red)
+__device__ __forceinline__ void block_reduce(DD (&v)[NV], const Red red)
 {
     __shared__ DD lds[NV][WAVES];
-    __shared__ int is_last;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
 #pragma unroll
@@ -98,7 +95,6 @@ __device__ __forceinline__ void grid_reduce(DD (&v)[NV], const Red red)
         for (int j = 0; j < NV; j++) lds[j][wave] = v[j];
     }
     __syncthreads();
-    const int nblocks = (int)gridDim.x;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < NV; j++) {
@@ -106,82 +102,55 @@ __device__ __forceinline__ void grid_reduce(DD (&v)[NV], const Red red)
             for (int w = 1; w < WAVES; w++) dd_merge(t, lds[j][w]);
             red.partials[j * red.max_blocks + blockIdx.x] = t;
         }
-        __threadfence();
-        const unsigned prev = atomicAdd(red.counter, 1u);
-        is_last = (prev == (unsigned)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    DD acc[NV];
-#pragma unroll
-    for (int j = 0; j < NV; j++) {
-        acc[j].s = 0.0;
-        acc[j].c = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
-            DD p;
-            p.s = __hip_atomic_load(&red.partials[j * red.max_blocks + b].s, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-            p.c = __hip_atomic_load(&red.partials[j * red.max_blocks + b].c, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-            dd_merge(acc[j], p);
-        }
-        acc[j] = wave_reduce(acc[j]);
-    }
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; j++) lds[j][wave] = acc[j];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; j++) {
-            DD t = lds[j][0];
-            for (int w = 1; w < WAVES; w++) dd_merge(t, lds[j][w]);
-            red.out[j] = t.s + t.c;
-        }
-        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// Tile (tx, ty) of this workgroup, XCD-aware: workgroup b runs on XCD b % 8;
-// XCD j walks tiles [start_j, start_j + count_j) in row-major tile order.
-__device__ __forceinline__ void tile_of_block(int nbx, int nblocks, int &tx, int &ty)
-{
-    const int b = (int)blockIdx.x;
-    const int xcd = b % NUM_XCD;
-    const int k = b / NUM_XCD;
-    const int q = nblocks / NUM_XCD;
-    const int rem = nblocks % NUM_XCD;
-    const int start = xcd * q + (xcd < rem ? xcd : rem);
-    const int t = start + k;
-    ty = t / nbx;
-    tx = t - ty * nbx;
-}
-
-// Column and first row of this lane's strip in extent e; len = rows in the strip (0 if none).
+// Column and first row of a lane's strip; len = rows in the strip (0 if none).
 struct Strip {
     int c, r0, len;
 };
 
-__device__ __forceinline__ Strip strip_of_lane(const Ext e)
-{
-    const int nbx = (e.x1 - e.x0 + TILE_X) / TILE_X;
-    const int nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
-    int tx, ty;
-    tile_of_block(nbx, nbx * nby, tx, ty);
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = threadIdx.x / WAVE;
-    Strip s;
-    s.c = e.x0 + tx * TILE_X + lane;
-    s.r0 = e.y0 + ty * TILE_Y + wave * STRIP;
-    int len = e.y1 - s.r0 + 1;
-    len = len < STRIP ? len : STRIP;
-    if (s.c > e.x1 || len < 0) len = 0;
-    s.len = len;
-    return s;
-}
+// Walks the tiles of extent e assigned to this workgroup.  Workgroup b runs on
+// XCD b % 8; the tiles (row-major, 64 x 16 cells) are split into 8 contiguous
+// bands, one per XCD, and the workgroups of an XCD take the tiles of its band
+// round-robin, so vertically adjacent tiles share an L2.
+struct TileLoop {
+    int nbx, count, start, k, nk;
+    int t;
+    Ext e;
+    __device__ __forceinline__ explicit TileLoop(const Ext ext) : e(ext)
+    {
+        nbx = (e.x1 - e.x0 + TILE_X) / TILE_X;
+        const int nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
+        const int ntiles = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : nbx * nby;
+        const int b = (int)blockIdx.x, g = (int)gridDim.x;
+        const int xcd = b % NUM_XCD;
+        const int q = ntiles / NUM_XCD, rem = ntiles % NUM_XCD;
+        count = q + (xcd < rem ? 1 : 0);
+        start = xcd * q + (xcd < rem ? xcd : rem);
+        k = b / NUM_XCD;
+        nk = g / NUM_XCD + (xcd < g % NUM_XCD ? 1 : 0);
+        t = k;
+    }
+    __device__ __forceinline__ bool more() const { return t < count; }
+    __device__ __forceinline__ void advance() { t += nk; }
+    __device__ __forceinline__ Strip strip() const
+    {
+        const int tile = start + t;
+        const int ty = tile / nbx;
+        const int tx = tile - ty * nbx;
+        const int lane = threadIdx.x & (WAVE - 1);
+        const int wave = threadIdx.x / WAVE;
+        Strip s;
+        s.c = e.x0 + tx * TILE_X + lane;
+        s.r0 = e.y0 + ty * TILE_Y + wave * STRIP;
+        int len = e.y1 - s.r0 + 1;
+        len = len < STRIP ? len : STRIP;
+        if (s.c > e.x1 || len < 0) len = 0;
+        s.len = len;
+        return s;
+    }
+};
 
 // 5-point operator (I - dt div K grad) x at flat index i, in the expression
 // order of tea_leaf_cg.cuknl:53-57.

@@ -40,7 +40,10 @@ constexpr int FIELD_DENSITY = 1, FIELD_ENERGY0 = 2, FIELD_ENERGY1 = 3, FIELD_U =
 constexpr int G_RECT = 1, G_CIRC = 2, G_POINT = 3;
 constexpr int CONDUCTIVITY = 1, RECIP_CONDUCTIVITY = 2;
 constexpr int TL_PREC_NONE = 1, TL_PREC_JAC_DIAG = 2, TL_PREC_JAC_BLOCK = 3;
-constexpr double G_SMALL = 1.0e-16;
+// Default-kind (single precision) real literals of the Fortran sources keep
+// their float value when they meet REAL(KIND=8) variables.
+constexpr double F32(float x) { return (double)x; }
+constexpr double G_SMALL = F32(1.0e-16f);  // data.f90:26
 constexpr int G_IBIG = 640000;
 
 struct Fail : std::runtime_error {
@@ -56,6 +59,9 @@ struct Fail : std::runtime_error {
     va_end(ap);
     throw Fail(buf);
 }
+
+tea_step_callback g_step_cb = nullptr;
+void *g_step_user = nullptr;
 
 double wall_time()
 {
@@ -119,9 +125,9 @@ struct Deck {
     double end_time = 10.0;
     int end_step = G_IBIG;
     int summary_frequency = 10;
-    double dtinit = 0.1;
+    double dtinit = F32(0.1f);       // read_input.f90:49
     int max_iters = 1000;
-    double eps = 1.0e-10;
+    double eps = F32(1.0e-10f);      // read_input.f90:52
     int coefficient = CONDUCTIVITY;
     bool profiler_on = false;
     int tl_ch_cg_presteps = 25;
@@ -146,7 +152,7 @@ std::vector<std::vector<std::string>> deck_lines(const std::string &path)
     if (!in) fail("cannot open input deck %s", path.c_str());
     std::vector<std::vector<std::string>> out;
     std::string l;
-    bool inside = false;
+    bool inside = false, joining = false;
     while (std::getline(in, l)) {
         if (l.size() > 100) l.resize(100);  // READ (a100)
         for (auto &ch : l)
@@ -168,7 +174,13 @@ std::vector<std::vector<std::string>> deck_lines(const std::string &path)
             else if (words[0] == "*endtea") inside = false;
             continue;
         }
-        if (inside) out.push_back(words);
+        if (!inside) continue;
+        // a lone backslash continues the logical line on the next line (parse.f90:260)
+        const bool cont = words.back() == "\\";
+        if (cont) words.pop_back();
+        if (joining) out.back().insert(out.back().end(), words.begin(), words.end());
+        else out.push_back(words);
+        joining = cont;
     }
     return out;
 }
@@ -499,7 +511,7 @@ void Run::field_summary()
                                      97.277332050749976, 95.462351583362249, 95.174738768320850};
         if (d.test_problem <= 6) {
             double qa_diff = std::fabs((100.0 * (temp / qa[d.test_problem])) - 100.0);
-            log("Test problem %4d is within %16.7E%% of the expected solution\n", d.test_problem, qa_diff);
+            log("Test problem %4d is within%s%% of the expected solution\n", d.test_problem, fortran_e(qa_diff, 16, 7).c_str());
             log(qa_diff < 0.001 ? " This test is considered PASSED\n" : " This is test is considered NOT PASSED\n");
         }
     }
@@ -658,8 +670,8 @@ void Run::tea_leaf()
                     int info = calc_eigenvalues(cg_alphas, cg_betas, eigmin, eigmax, n - 1);
                     first = false;
                     if (info != 0) fail("tea_leaf: Error in calculating eigenvalues");
-                    eigmin = eigmin * 0.95;
-                    eigmax = eigmax * 1.05;
+                    eigmin = eigmin * F32(0.95f);  // tea_solve.f90:186-187
+                    eigmax = eigmax * F32(1.05f);
                 }
                 if (d.tl_use_chebyshev) {
                     calc_ch_coefs(ch_alphas, ch_betas, eigmin, eigmax, theta, max_cheby_iters);
@@ -672,9 +684,9 @@ void Run::tea_leaf()
                     b.check();
                 }
                 cn = eigmax / eigmin;
-                log("Switching after %3d CG its, error %15.7E\n", n, rro);
-                log("Eigen min%14.6E Eigen max%14.6E Condition number%14.6f Error%14.6E\n", eigmin, eigmax, cn,
-                    old_error);
+                log("Switching after %3d CG its, error %s\n", n, fortran_e(rro, 15, 7).c_str());
+                log("Eigen min%s Eigen max%s Condition number%14.6f Error%s\n", fortran_e(eigmin, 14, 6).c_str(),
+                    fortran_e(eigmax, 14, 6).c_str(), cn, fortran_e(old_error, 14, 6).c_str());
 
                 if (d.tl_use_ppcg) {
                     int st = 2;
@@ -785,8 +797,8 @@ void Run::tea_leaf()
         exact_error = std::sqrt(exact_error);
     }
 
-    log("Conduction error %14.7E\n", error / initial_residual);
-    if (d.tl_check_result) log("EXACT error calculated as%14.7E\n", exact_error / initial_residual);
+    log("Conduction error %s\n", fortran_e(error / initial_residual, 14, 7).c_str());
+    if (d.tl_check_result) log("EXACT error calculated as%s\n", fortran_e(exact_error / initial_residual, 14, 7).c_str());
     log("Iteration count %8d\n", n - 1);
     if (d.tl_use_ppcg) log("PPCG Iteration count%8d (Total %8d)\n", ppcg_inner_iters, ppcg_inner_iters + n - 1);
 
@@ -825,6 +837,7 @@ void Run::diffuse()
         if (R->n_steps < TEA_MAX_STEPS) R->step_seconds[R->n_steps] = now - step_time;
         R->n_steps++;
         log(" Wall clock %.15g\n", now - timerstart);
+        if (g_step_cb) g_step_cb(step, g_step_user);
         if (time + G_SMALL > d.end_time || step >= d.end_step) {
             complete = true;
             field_summary();
@@ -836,6 +849,12 @@ void Run::diffuse()
 }
 
 }  // namespace
+
+extern "C" void tea_driver_set_step_callback(tea_step_callback cb, void *user)
+{
+    g_step_cb = cb;
+    g_step_user = user;
+}
 
 extern "C" int tea_driver_run(const char *deck_path, const char *backend_lib, const char *out_path,
                               tea_run_result *result)

@@ -35,6 +35,11 @@ typedef struct {
     char error_msg[512];
 } tea_run_result;
 
+/* Called after every timestep (after the step's work has completed on the
+ * device) with the 1-based step number; NULL to remove. */
+typedef void (*tea_step_callback)(int step, void *user);
+void tea_driver_set_step_callback(tea_step_callback cb, void *user);
+
 /* Runs the deck at deck_path with the kernel backend library backend_lib
  * (a path to a shared library exporting the tea_leaf_engine.h symbols).
  * If out_path is non-NULL a tea.out-style log is written there.

@@ -141,7 +141,13 @@ class Engine:
             fn.argtypes = args
             fn.restype = ret
         self.lib.tea_engine_set_error_mode(1)
-        self.nx = self.ny = self.h = 0
+        self.refresh()
+
+    def refresh(self):
+        """Reads the chunk geometry of the library's current chunk (if any)."""
+        nx_, ny_, h_, s_, p_ = (ctypes.c_int() for _ in range(5))
+        self.lib.tea_engine_info(*(ctypes.byref(v) for v in (nx_, ny_, h_, s_, p_)))
+        self.nx, self.ny, self.h, self.solver, self.prec = nx_.value, ny_.value, h_.value, s_.value, p_.value
 
     # -- error handling
     def _check(self):
@@ -158,9 +164,7 @@ class Engine:
     def initialise(self, deck_path, nx: int, ny: int):
         self.lib.tea_engine_set_deck_path(str(deck_path).encode())
         self._call("initialise_cuda_", _ip(1), _ip(nx), _ip(1), _ip(ny))
-        nx_, ny_, h_, s_, p_ = (ctypes.c_int() for _ in range(5))
-        self.lib.tea_engine_info(*(ctypes.byref(v) for v in (nx_, ny_, h_, s_, p_)))
-        self.nx, self.ny, self.h, self.solver, self.prec = nx_.value, ny_.value, h_.value, s_.value, p_.value
+        self.refresh()
 
     def initialise_chunk(self, xmin, ymin, dx, dy):
         d = [ctypes.c_double(v) for v in (xmin, ymin, dx, dy)]
@@ -370,6 +374,7 @@ class RunResult:
         return self.summaries[-1]
 
 
+_STEP_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p)
 _driver = None
 
 
@@ -380,16 +385,25 @@ def _driver_lib():
         _driver.tea_driver_run.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                            ctypes.POINTER(_RunResult)]
         _driver.tea_driver_run.restype = ctypes.c_int
+        _driver.tea_driver_set_step_callback.argtypes = [_STEP_CB, ctypes.c_void_p]
+        _driver.tea_driver_set_step_callback.restype = None
     return _driver
 
 
-def run_deck(deck_path, backend: str | os.PathLike | None = None, out_path=None) -> RunResult:
+def run_deck(deck_path, backend: str | os.PathLike | None = None, out_path=None, on_step=None) -> RunResult:
     """Runs a tea.in deck through the host driver over the kernel library
-    ``backend`` (default: the HIP engine) and returns per-step results."""
+    ``backend`` (default: the HIP engine) and returns per-step results.
+    ``on_step(step)`` is called after each timestep has completed on the device."""
     lib = Path(backend) if backend is not None else engine_library()
     res = _RunResult()
-    rc = _driver_lib().tea_driver_run(str(deck_path).encode(), str(lib).encode(),
-                                      str(out_path).encode() if out_path else None, ctypes.byref(res))
+    drv = _driver_lib()
+    cb = _STEP_CB(lambda step, _user: on_step(step)) if on_step is not None else _STEP_CB()
+    drv.tea_driver_set_step_callback(cb, None)
+    try:
+        rc = drv.tea_driver_run(str(deck_path).encode(), str(lib).encode(),
+                                str(out_path).encode() if out_path else None, ctypes.byref(res))
+    finally:
+        drv.tea_driver_set_step_callback(_STEP_CB(), None)
     if rc != 0:
         raise TeaError(res.error_msg.decode())
     n = res.n_steps

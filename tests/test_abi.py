@@ -1,0 +1,49 @@
+"""The C-ABI libraries load (no GPU needed) and export every entry point that
+include/tea_leaf_engine.h declares; the driver binds exactly the 29 kernel
+entry points of the reference's Fortran interface."""
+from __future__ import annotations
+
+import ctypes
+import re
+
+import pytest
+
+from conftest import REPO
+from tealeaf_cuda_amd import ABI_SYMBOLS, EXTENSION_SYMBOLS
+from tealeaf_cuda_amd.build import DRIVER_SO, ENGINE_SO
+
+HEADER = REPO / "include" / "tea_leaf_engine.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:void|int|void\s*\*)\s*\*?\s*(\w+)\s*\(", text, flags=re.M)
+
+
+def test_header_declares_the_reference_abi():
+    names = declared()
+    fortran = [n for n in names if n.endswith("_")]
+    assert len(fortran) == 29
+    assert set(fortran) == set(ABI_SYMBOLS)
+    assert set(names) - set(fortran) == set(EXTENSION_SYMBOLS)
+
+
+def test_driver_binds_the_same_29_symbols():
+    inc = (REPO / "tealeaf_cuda_amd" / "csrc" / "tea_abi_list.inc").read_text()
+    assert set(re.findall(r"^TEA_FN\((\w+),", inc, flags=re.M)) == set(ABI_SYMBOLS)
+
+
+@pytest.mark.parametrize("lib", ["engine", "oracle"])
+def test_library_exports_every_declared_symbol(lib, oracle_so):
+    path = ENGINE_SO if lib == "engine" else oracle_so
+    if lib == "engine" and not path.exists():
+        pytest.fail("libtea_engine.so has not been built (run __graft_entry__.build())")
+    so = ctypes.CDLL(str(path))
+    missing = [n for n in declared() if not hasattr(so, n)]
+    assert not missing
+
+
+def test_driver_library_loads(driver_built):
+    so = ctypes.CDLL(str(DRIVER_SO))
+    assert hasattr(so, "tea_driver_run") and hasattr(so, "tea_driver_set_step_callback")

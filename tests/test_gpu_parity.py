@@ -1,0 +1,184 @@
+"""GPU parity: the HIP engine against the oracle on the same inputs.
+
+Whole decks go through the same host driver over both kernel libraries.  The
+engine's reductions are compensated (order-independent), as are the oracle's
+in its default mode, and both evaluate every pointwise expression in the
+reference's order without fused multiply-adds, so the bar is bit-for-bit:
+identical iteration counts, field summaries, conduction errors and exact
+residuals.  (The floating-point tolerance the project states for parity with
+the reference, 1e-10 relative, is checked against the golden files in
+test_gpu_golden.py.)
+
+Every ABI entry point is also called one at a time on random fields in both
+libraries, comparing every field afterwards.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tealeaf_cuda_amd import Deck, Engine, StateSpec, random_layout, run_deck
+from tealeaf_cuda_amd import runtime as rt
+
+pytestmark = pytest.mark.gpu
+
+LAYOUT = [StateSpec(100.0, 0.0001), StateSpec(0.1, 25.0, "rectangle", 0.0, 1.0, 1.0, 2.0),
+          StateSpec(0.1, 0.1, "rectangle", 1.0, 6.0, 1.0, 2.0), StateSpec(0.1, 0.1, "rectangle", 5.0, 6.0, 1.0, 8.0),
+          StateSpec(0.1, 0.1, "rectangle", 5.0, 10.0, 7.0, 8.0),
+          StateSpec(5.0, 2.0, "circle", 3.0, 0.0, 6.0, 0.0, 1.5), StateSpec(1.0, 9.0, "point", 2.5, 0.0, 2.5, 0.0)]
+
+DECKS = {
+    "cg_none_eps15": dict(solver="cg", preconditioner="none", eps=1e-15),
+    "cg_diag": dict(solver="cg", preconditioner="jac_diag"),
+    "cg_block": dict(solver="cg", preconditioner="jac_block"),
+    "cg_block_ragged": dict(solver="cg", preconditioner="jac_block", x_cells=131, y_cells=77),
+    "cheby_none": dict(solver="chebyshev", preconditioner="none", ch_cg_presteps=20),
+    "cheby_diag": dict(solver="chebyshev", preconditioner="jac_diag", ch_cg_presteps=20),
+    "cheby_block": dict(solver="chebyshev", preconditioner="jac_block", ch_cg_presteps=20),
+    "ppcg_none": dict(solver="ppcg", preconditioner="none", ppcg_inner_steps=10),
+    "ppcg_diag_h2": dict(solver="ppcg", preconditioner="jac_diag", ppcg_inner_steps=9, halo_depth=2),
+    "ppcg_block": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10),
+    "ppcg_block_ragged": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+                              x_cells=97, y_cells=66),
+    "ppcg_none_h4": dict(solver="ppcg", preconditioner="none", ppcg_inner_steps=10, halo_depth=4),
+    "jacobi": dict(solver="jacobi", preconditioner="none", eps=1e-7, max_iters=3000),
+    "reflective_ppcg": dict(solver="ppcg", preconditioner="jac_diag", reflective_boundary=True,
+                            ppcg_inner_steps=8),
+    "reflective_cg_h2": dict(solver="cg", preconditioner="jac_block", reflective_boundary=True, halo_depth=2),
+    "inverse_density": dict(solver="cg", preconditioner="jac_diag", inverse_density=True),
+    "random_layout_ppcg": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+                               states=random_layout(11, cells=128), x_cells=128, y_cells=128),
+    "tiny_3x5": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=4, x_cells=3, y_cells=5),
+}
+
+
+def _deck(**kw):
+    base = dict(x_cells=96, y_cells=80, end_step=3, eps=1e-10, check_result=True, summary_frequency=1,
+                states=LAYOUT)
+    base.update(kw)
+    return Deck(**base)
+
+
+@pytest.mark.parametrize("name", sorted(DECKS))
+def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
+    oracle_env("dot2")
+    print(f"[gpu parity] deck {name}", flush=True)
+    path = _deck(**DECKS[name]).write(tmp_path / "tea.in")
+    gpu = run_deck(path)
+    cpu = run_deck(path, backend=oracle_so)
+    assert gpu.iterations == cpu.iterations
+    assert gpu.ppcg_inner == cpu.ppcg_inner
+    assert gpu.conduction_error == cpu.conduction_error
+    assert gpu.exact_error == cpu.exact_error
+    assert len(gpu.summaries) == len(cpu.summaries)
+    for a, b in zip(gpu.summaries, cpu.summaries):
+        assert a.values() == b.values(), (a, b)
+
+
+# ------------------------------------------------------------------ entry points
+
+def _prepare(eng, tmp_path, solver, prec, nx=70, ny=45, halo=2):
+    deck = Deck(x_cells=nx, y_cells=ny, solver=solver, preconditioner=prec, halo_depth=halo, states=LAYOUT)
+    eng.initialise(deck.write(tmp_path / f"ep_{solver}_{prec}.in"), nx, ny)
+    eng.initialise_chunk(0.0, 0.0, 10.0 / nx, 10.0 / ny)
+    st = LAYOUT
+    eng.generate_chunk([s.density for s in st], [s.energy for s in st], [s.xmin for s in st],
+                       [s.xmax for s in st], [s.ymin for s in st], [s.ymax for s in st], [s.radius for s in st],
+                       [{"rectangle": 1, "circle": 2, "point": 3}.get(s.geometry or "", 0) for s in st])
+    eng.set_field_kernel()
+    eng.common_init(rt.CONDUCTIVITY, 0.004)
+
+
+def _randomise(engines, seed):
+    rng = np.random.default_rng(seed)
+    shape = engines[0].shape()
+    for fid in (rt.FIELD_U, rt.FIELD_P, rt.FIELD_SD, rt.FIELD_R, rt.XFIELD_U0, rt.XFIELD_W, rt.XFIELD_Z,
+                rt.XFIELD_RTEMP, rt.XFIELD_UTEMP, rt.XFIELD_RSTORE):
+        a = rng.standard_normal(shape)
+        for e in engines:
+            e.set_field(fid, a)
+
+
+def _compare(g, c, label):
+    for fid in range(1, 20):
+        a, b = g.get_field(fid), c.get_field(fid)
+        assert np.array_equal(a, b, equal_nan=True), f"{label}: field {fid} differs"
+
+
+CALLS = {
+    "calc_residual": lambda e: e.calc_residual(),
+    "calc_2norm_0": lambda e: e.calc_2norm(0),
+    "calc_2norm_1": lambda e: e.calc_2norm(1),
+    "calc_2norm_2": lambda e: e.calc_2norm(2),
+    "finalise": lambda e: e.finalise(),
+    "cg_init": lambda e: e.cg_init(),
+    "cg_calc_w": lambda e: e.cg_calc_w(),
+    "cg_calc_ur": lambda e: e.cg_calc_ur(0.37),
+    "cg_calc_p": lambda e: e.cg_calc_p(0.61),
+    "field_summary": lambda e: e.field_summary(),
+}
+PPCG_CALLS = {
+    "ppcg_init_1": lambda e: e.ppcg_init(1),
+    "ppcg_init_2": lambda e: e.ppcg_init(2),
+    "ppcg_init_3": lambda e: e.ppcg_init(3),
+    "ppcg_init_sd": lambda e: e.ppcg_init_sd(1.7),
+    "ppcg_init_sd_new": lambda e: e.ppcg_init_sd_new(1.7),
+    "ppcg_inner": lambda e: (e.ppcg_init_constants([0.3, 0.5, 0.7], [1.1, 0.9, 0.8]), e.ppcg_inner(2)),
+    "ppcg_store_r": lambda e: e.ppcg_store_r(),
+    "ppcg_update_z": lambda e: e.ppcg_update_z(),
+    "ppcg_calc_rrn": lambda e: e.ppcg_calc_rrn(),
+    "ppcg_calc_2norm": lambda e: e.ppcg_calc_2norm(),
+    "ppcg_calc_p": lambda e: e.ppcg_calc_p(0.45),
+}
+CHEBY_CALLS = {
+    "cheby_init": lambda e: e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3),
+    "cheby_iterate": lambda e: (e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3), e.cheby_iterate(3)),
+}
+
+
+@pytest.mark.parametrize("prec", ["none", "jac_diag", "jac_block"])
+@pytest.mark.parametrize("solver,calls", [("cg", CALLS), ("ppcg", PPCG_CALLS), ("chebyshev", CHEBY_CALLS)])
+def test_entry_points_bitwise(solver, calls, prec, oracle_so, oracle_env, tmp_path):
+    oracle_env("dot2")
+    g, c = Engine(), Engine(oracle_so)
+    try:
+        for k, (name, fn) in enumerate(sorted(calls.items())):
+            print(f"[gpu parity] entry point {solver}/{prec}/{name}", flush=True)
+            for e in (g, c):
+                _prepare(e, tmp_path, solver, prec)
+            _randomise((g, c), seed=k)
+            for e in (g, c):
+                e.cg_init()                 # preconditioner set-up (Mi or the block factors)
+            _randomise((g, c), seed=100 + k)
+            out_g, out_c = fn(g), fn(c)
+            assert out_g == out_c, name
+            _compare(g, c, name)
+    finally:
+        g.finalize()
+        c.finalize()
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+def test_halo_entry_points_bitwise(depth, oracle_so, oracle_env, tmp_path):
+    oracle_env("dot2")
+    g, c = Engine(), Engine(oracle_so)
+    try:
+        for e in (g, c):
+            _prepare(e, tmp_path, "cg", "none", nx=37, ny=29, halo=2)
+        _randomise((g, c), seed=99)
+        fields = [1, 1, 1, 1, 1, 1, 1]
+        for e in (g, c):
+            e.update_halo(fields, depth)
+        _compare(g, c, "update_halo")
+        for face in (rt.CHUNK_LEFT, rt.CHUNK_RIGHT, rt.CHUNK_BOTTOM, rt.CHUNK_TOP):
+            side = depth * ((29 if face <= rt.CHUNK_RIGHT else 37) + 2 * depth)
+            off = [k * side for k in range(7)]
+            bufs = [e.pack(fields, off, depth, face, 7 * side) for e in (g, c)]
+            assert np.array_equal(bufs[0], bufs[1])
+            rnd = np.random.default_rng(face).standard_normal(bufs[0].shape)
+            for e in (g, c):
+                e.unpack(fields, off, depth, face, rnd)
+            _compare(g, c, f"unpack face {face}")
+    finally:
+        g.finalize()
+        c.finalize()

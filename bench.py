@@ -93,6 +93,17 @@ def run_gpu(n, warmup, steps, tmp):
     return res, gpu_ms, wall_ms, ev["profile"], timed
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summaries
+    (lines 'TRAFFIC <kernel> <bytes>' in profiles/*.md), or None."""
+    for p in sorted((REPO / "profiles").glob("*.md")):
+        for line in p.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
 def roofline(profile, n):
     """Dominant kernel (largest total time) with its algorithmic bandwidth."""
     best = None
@@ -115,7 +126,7 @@ def roofline(profile, n):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(name) if n == 4096 else None,
         "kernel": name,
         "bytes_per_launch": bytes_per_launch,
         "avg_launch_us": round(avg_us, 3),

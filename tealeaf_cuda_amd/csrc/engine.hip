@@ -118,8 +118,9 @@ struct Chunk {
     double *cellx = nullptr, *celly = nullptr, *vertexx = nullptr, *vertexy = nullptr;
     std::vector<double> h_celldx, h_celldy;
 
-    DD *h_partials = nullptr;        // pinned: MAX_RED x max_blocks workgroup partials
-    DD *d_partials = nullptr;        // device alias of h_partials
+    DD *d_partials = nullptr;        // MAX_RED x max_blocks workgroup partials
+    double *h_out = nullptr;         // pinned: the finished values
+    double *d_out = nullptr;         // device alias of h_out
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -224,14 +225,14 @@ bool fused_inner()
     return v;
 }
 
-// Largest grid: 8 workgroups of 4 wavefronts per CU on 256 CUs fills the GPU;
-// larger extents loop over their tiles.
+// One workgroup per tile measured fastest on MI355X for every kernel here;
+// TEA_GRID_MAX caps the grid (workgroups then loop over several tiles).
 int grid_max()
 {
     static int v = [] {
         const char *s = getenv("TEA_GRID_MAX");
         int x = s ? atoi(s) : 0;
-        return x > 0 ? x : 2048;
+        return x > 0 ? x : (1 << 30);
     }();
     return v;
 }
@@ -275,28 +276,20 @@ bool launch(const char *name, K kernel, const Ext &e, A... args)
     return true;
 }
 
-// Host copy of the device-side compensated merge (engine_device.hpp).
-void host_merge(DD &a, const DD b)
-{
-    const double s = a.s + b.s;
-    const double bb = s - a.s;
-    const double se = (a.s - (s - bb)) + (b.s - bb);
-    a.s = s;
-    a.c += (se + b.c);
-}
-
-// Waits for the queued work and combines, in workgroup order, the partials of
-// the last reduction launch into dst[0..n).
+// Combines the partials of the last reduction launch on the device, waits,
+// and returns the n finished values.
 bool fetch(double *dst, int n)
 {
-    hipError_t err = hipStreamSynchronize(g.stream);
-    if (err != hipSuccess) return die("hipStreamSynchronize failed: %s", hipGetErrorString(err));
-    for (int j = 0; j < n; j++) {
-        DD t{0.0, 0.0};
-        const DD *p = g.h_partials + (size_t)j * g.max_blocks;
-        for (int b = 0; b < g.last_grid; b++) host_merge(t, p[b]);
-        dst[j] = t.s + t.c;
+    {
+        Timed t("finish");
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, g.stream, (const DD *)g.d_partials,
+                           g.max_blocks, g.last_grid, n, g.d_out);
     }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return die("launch of k_finish failed: %s", hipGetErrorString(err));
+    err = hipStreamSynchronize(g.stream);
+    if (err != hipSuccess) return die("hipStreamSynchronize failed: %s", hipGetErrorString(err));
+    for (int j = 0; j < n; j++) dst[j] = g.h_out[j];
     if (g.prof.on) g.prof.drain(false);
     return true;
 }
@@ -980,7 +973,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_inner_fused(Ext e, long pitch, c
             rtemp[i] = rv[k];
         }
     }
-    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+    if (PREC == PREC_BLOCK && s.len > 0) block_solve_strip_regs(rv, zv, s.len, Kx, Ky, i0, pitch);
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
@@ -1034,7 +1027,8 @@ void release_all()
     g.prof.pool.clear();
     if (g.slab) (void)hipFree(g.slab);
     if (g.cellx) (void)hipFree(g.cellx);
-    if (g.h_partials) (void)hipHostFree(g.h_partials);
+    if (g.d_partials) (void)hipFree(g.d_partials);
+    if (g.h_out) (void)hipHostFree(g.h_out);
     if (g.d_halo_buf) (void)hipFree(g.d_halo_buf);
     if (g.stream) (void)hipStreamDestroy(g.stream);
     Profiler keep = g.prof;
@@ -1102,9 +1096,13 @@ EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
     g.celly = g.vertexx + nxv;
     g.vertexy = g.celly + nyv;
 
-    g.max_blocks = grid_max();
-    HIP_OK(hipHostMalloc(&g.h_partials, (size_t)MAX_RED * g.max_blocks * sizeof(DD), hipHostMallocDefault));
-    HIP_OK(hipHostGetDevicePointer((void **)&g.d_partials, g.h_partials, 0));
+    {
+        const int all = blocks_for(Ext{0, g.nx + 2 * g.h - 1, 0, g.ny + 2 * g.h - 1});
+        g.max_blocks = (all < grid_max() ? all : grid_max()) + 1;
+    }
+    HIP_OK(hipMalloc(&g.d_partials, (size_t)MAX_RED * g.max_blocks * sizeof(DD)));
+    HIP_OK(hipHostMalloc(&g.h_out, MAX_RED * sizeof(double), hipHostMallocDefault));
+    HIP_OK(hipHostGetDevicePointer((void **)&g.d_out, g.h_out, 0));
 
     g.halo_buf_elems = (size_t)TEA_NUM_FIELDS * (size_t)g.h * (size_t)((g.nx > g.ny ? g.nx : g.ny) + 2 * g.h);
     HIP_OK(hipMalloc(&g.d_halo_buf, g.halo_buf_elems * sizeof(double)));

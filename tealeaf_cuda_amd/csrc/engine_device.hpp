@@ -72,11 +72,12 @@ __device__ __forceinline__ DD wave_reduce(DD v)
     return v;
 }
 
-// Reduction output: one compensated partial per workgroup and value, written
-// straight into pinned host memory; the host combines them in workgroup order
-// once the launch has completed.  No atomics: every workgroup owns its slot.
+// Reduction output: one compensated partial per workgroup and value in device
+// memory; a one-workgroup finishing launch (k_finish) combines them in a fixed
+// order and writes the rounded results to pinned host memory.  No atomics:
+// every workgroup owns its slot.
 struct Red {
-    DD *partials;        // MAX_RED * max_blocks, host memory mapped into the GPU
+    DD *partials;        // MAX_RED * max_blocks
     int max_blocks;
 };
 
@@ -102,6 +103,33 @@ __device__ __forceinline__ void block_reduce(DD (&v)[NV], const Red red)
             for (int w = 1; w < WAVES; w++) dd_merge(t, lds[j][w]);
             red.partials[j * red.max_blocks + blockIdx.x] = t;
         }
+    }
+}
+
+// Combines the nblocks partials of each of nv values (lane t takes partials
+// t, t+1024, ...; then the wave and workgroup trees) and writes out[j] =
+// rounded sum.  Launched with one workgroup of FINISH_THREADS threads.
+constexpr int FINISH_THREADS = 1024;
+
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish(const DD *__restrict__ partials, int max_blocks,
+                                                          int nblocks, int nv, double *__restrict__ out)
+{
+    __shared__ DD lds[FINISH_THREADS / WAVE];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    for (int j = 0; j < nv; j++) {
+        DD acc{0.0, 0.0};
+        const DD *p = partials + (size_t)j * max_blocks;
+        for (int b = threadIdx.x; b < nblocks; b += FINISH_THREADS) dd_merge(acc, p[b]);
+        acc = wave_reduce(acc);
+        if (lane == 0) lds[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            DD t = lds[0];
+            for (int w = 1; w < FINISH_THREADS / WAVE; w++) dd_merge(t, lds[w]);
+            out[j] = t.s + t.c;
+        }
+        __syncthreads();
     }
 }
 
@@ -189,6 +217,34 @@ __device__ __forceinline__ void block_solve_strip(const double (&rhs)[STRIP], do
     for (int k = STRIP - 1; k >= 0; k--) {
         if (k == len - 1) z[k] = dp[k];
         else if (k < len - 1) z[k] = dp[k] - cp[i0 + k * pitch] * z[k + 1];
+    }
+}
+
+// The same solve with the Thomas factors recomputed from Kx, Ky in registers
+// instead of read from cp and bfp: the factor expressions are those of
+// tea_block_jacobi.cuknl:83-111, so every value is bit-identical to the stored
+// one, and the pass moves 16 bytes per cell less.
+__device__ __forceinline__ void block_solve_strip_regs(const double (&rhs)[STRIP], double (&z)[STRIP], int len,
+                                                       const double *__restrict__ Kx,
+                                                       const double *__restrict__ Ky, long i0, long pitch)
+{
+    double cp[STRIP], bfp[STRIP], dp[STRIP];
+    const double b0 = diag_A(Kx, Ky, i0, pitch);
+    cp[0] = (-Ky[i0 + pitch]) / b0;
+    dp[0] = rhs[0] / b0;
+#pragma unroll
+    for (int k = 1; k < STRIP; k++) {
+        if (k < len) {
+            const long i = i0 + k * pitch;
+            bfp[k] = 1.0 / (diag_A(Kx, Ky, i, pitch) - (-Ky[i]) * cp[k - 1]);
+            cp[k] = (-Ky[i + pitch]) * bfp[k];
+            dp[k] = (rhs[k] - (-Ky[i]) * dp[k - 1]) * bfp[k];
+        }
+    }
+#pragma unroll
+    for (int k = STRIP - 1; k >= 0; k--) {
+        if (k == len - 1) z[k] = dp[k];
+        else if (k < len - 1) z[k] = dp[k] - cp[k] * z[k + 1];
     }
 }
 

@@ -212,6 +212,26 @@ void tea_engine_synchronize(void);
 /* Releases every device allocation; the next call must be initialise_cuda_. */
 void tea_engine_finalize(void);
 
+/* ---- mixed precision: FP32 inner solver under FP64 refinement ------------ *
+ * tea_engine_set_precision(32) switches the CG and PPCG entry points
+ * (cg_init, cg_calc_w/ur/p, ppcg_init, init_sd(_new), inner, store_r,
+ * update_z, calc_rrn, ppcg_calc_2norm, ppcg_calc_p) to FP32 shadow fields
+ * (allocated on first use); their scalars stay FP64 and their dot products
+ * accumulate in compensated FP64.  calc_residual, calc_2norm, field_summary,
+ * the set-up and finalise calls always use the FP64 fields.  Chebyshev and
+ * Jacobi calls are rejected in FP32 mode.  (64) switches back. */
+int tea_engine_set_precision(int bits);
+/* FP32 copies of the FP64 face coefficients (after tea_leaf_common_init). */
+void tea_engine_mp_setup(void);
+/* r32 = (float) r (the FP64 residual field), u32 = 0 (the FP32 correction);
+ * returns rr = r32.r32. */
+void tea_engine_mp_begin(double *rr);
+/* u += (double) u32. */
+void tea_engine_mp_end(void);
+/* Copy an FP32 shadow field to/from a host array of (nx+2h)*(ny+2h) floats. */
+int tea_engine_get_field32(int field_id, float *host);
+int tea_engine_set_field32(int field_id, const float *host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,9 @@ static struct {
     int dot_mode;
     double *f[X_COUNT];        /* 2D fields, index = c + r*pitch */
     double *zl;                /* block-Jacobi result kept out of z (the reference's shared-memory z_l) */
+    float *f32[X_COUNT];       /* FP32 shadow fields of the mixed-precision inner solver */
+    float *zl32;
+    int fp32;                  /* CG / PPCG entry points work on the FP32 fields */
     double *xarea, *yarea;
     double *cellx, *celly, *celldx, *celldy, *vertexx, *vertexy, *vertexdx, *vertexdy;
     double *ch_a, *ch_b;
@@ -239,78 +242,28 @@ static inline void dd_merge(dd_t *acc, dd_t x)
 #define BFP   g.f[X_BFP]
 #define VOL   g.f[X_VOLUME]
 
-/* The 5-point operator (I - dt div K grad) applied to x at cell i, in the
- * expression order of tea_leaf_cg.cuknl:53-57 (and the identical expressions in
- * tea_leaf_common.cuknl:29-33, tea_leaf_cheby.cuknl:26-30, tea_leaf_ppcg.cuknl:163-167). */
-static inline double apply_A(const double *x, size_t i)
-{
-    const size_t pp = (size_t)g.pitch;
-    return (1.0
-            + (KY[i + pp] + KY[i])
-            + (KX[i + 1] + KX[i])) * x[i]
-           - (KY[i + pp] * x[i + pp] + KY[i] * x[i - pp])
-           - (KX[i + 1] * x[i + 1] + KX[i] * x[i - 1]);
-}
+static double *zl_f64(void) { return g.zl; }
+static float *zl_f32(void) { return g.zl32; }
 
-/* ---------------- block Jacobi: tea_block_jacobi.cuknl:5-111 ------------------ */
+#define REAL double
+#define S(n) n##_f64
+#define FLD(id) g.f[id]
+#include "tea_oracle_solver.inc"
+#undef REAL
+#undef S
+#undef FLD
+#define REAL float
+#define S(n) n##_f32
+#define FLD(id) g.f32[id]
+#include "tea_oracle_solver.inc"
+#undef REAL
+#undef S
+#undef FLD
 
-/* Diagonal, sub- and super-diagonal of the strip system at cell i
- * (COEF_B, COEF_A, COEF_C of tea_block_jacobi.cuknl:5-7). */
-static inline double coef_b(size_t i)
-{
-    const size_t pp = (size_t)g.pitch;
-    return 1.0 + (KY[i + pp] + KY[i]) + (KX[i + 1] + KX[i]);
-}
-static inline double coef_a(size_t i) { return -KY[i]; }
-static inline double coef_c(size_t i) { return -KY[i + (size_t)g.pitch]; }
-
-/* Strips start at row y_lo and hold up to 4 rows (cuda_kernel_header.hpp:11-16). */
-static void block_init_extent(int x0, int x1, int y0, int y1)
-{
-    const int nstrips = (y1 - y0 + 1 + JAC_BLOCK - 1) / JAC_BLOCK;
-    #pragma omp parallel for schedule(static)
-    for (int s = 0; s < nstrips; s++) {
-        const int r0 = y0 + s * JAC_BLOCK;
-        const int len = (y1 - r0 + 1) < JAC_BLOCK ? (y1 - r0 + 1) : JAC_BLOCK;
-        for (int c = x0; c <= x1; c++) {
-            size_t i = IDX(c, r0);
-            CP[i] = coef_c(i) / coef_b(i);
-            for (int k = 1; k < len; k++) {
-                i = IDX(c, r0 + k);
-                BFP[i] = 1.0 / (coef_b(i) - coef_a(i) * CP[IDX(c, r0 + k - 1)]);
-                CP[i] = coef_c(i) * BFP[i];
-            }
-        }
-    }
-}
-
-/* z = M_block^-1 rhs for the strip at (c, r0) of length len (block_solve_func). */
-static inline void block_solve_column(const double *rhs, double *z, int c, int r0, int len)
-{
-    double dp[JAC_BLOCK];
-    size_t i = IDX(c, r0);
-    dp[0] = rhs[i] / coef_b(i);
-    for (int k = 1; k < len; k++) {
-        i = IDX(c, r0 + k);
-        dp[k] = (rhs[i] - coef_a(i) * dp[k - 1]) * BFP[i];
-    }
-    z[IDX(c, r0 + len - 1)] = dp[len - 1];
-    for (int k = len - 2; k >= 0; k--) {
-        i = IDX(c, r0 + k);
-        z[i] = dp[k] - CP[i] * z[IDX(c, r0 + k + 1)];
-    }
-}
-
-static void block_solve_extent(const double *rhs, double *z, int x0, int x1, int y0, int y1)
-{
-    const int nstrips = (y1 - y0 + 1 + JAC_BLOCK - 1) / JAC_BLOCK;
-    #pragma omp parallel for schedule(static)
-    for (int s = 0; s < nstrips; s++) {
-        const int r0 = y0 + s * JAC_BLOCK;
-        const int len = (y1 - r0 + 1) < JAC_BLOCK ? (y1 - r0 + 1) : JAC_BLOCK;
-        for (int c = x0; c <= x1; c++) block_solve_column(rhs, z, c, r0, len);
-    }
-}
+/* FP64 names used by the Chebyshev, Jacobi and residual kernels */
+#define apply_A apply_A_f64
+#define coef_b coef_b_f64
+#define block_solve_extent block_solve_extent_f64
 
 /* ---------------- set-up ------------------------------------------------------ */
 
@@ -318,6 +271,9 @@ static void free_all(void)
 {
     for (int k = 0; k < X_COUNT; k++) { free(g.f[k]); g.f[k] = NULL; }
     free(g.zl); g.zl = NULL;
+    for (int k = 0; k < X_COUNT; k++) { free(g.f32[k]); g.f32[k] = NULL; }
+    free(g.zl32); g.zl32 = NULL;
+    g.fp32 = 0;
     free(g.xarea); free(g.yarea);
     free(g.cellx); free(g.celly); free(g.celldx); free(g.celldy);
     free(g.vertexx); free(g.vertexy); free(g.vertexdx); free(g.vertexdy);
@@ -635,24 +591,7 @@ EXPORT void tea_leaf_cg_init_kernel_cuda_(double *rro)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    if (g.prec == PREC_BLOCK) {
-        block_init_extent(IX0, IX1, IY0, IY1);
-        block_solve_extent(R, Z, IX0, IX1, IY0, IY1);
-    } else if (g.prec == PREC_DIAG) {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, {
-            const double diag = coef_b(i);
-            MI[i] = 1.0 / diag;
-        });
-    }
-    const int prec = g.prec;
-    double s = 0;
-    REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, {
-        if (prec == PREC_BLOCK) P[i] = Z[i];
-        else if (prec == PREC_DIAG) { Z[i] = R[i] * MI[i]; P[i] = Z[i]; }
-        else P[i] = R[i];
-        ta = R[i]; tb = P[i];
-    });
-    *rro = s;
+    if (g.fp32) cg_init_f32(rro); else cg_init_f64(rro);
 }
 
 /* tea_leaf_cg.cuknl:43-68 */
@@ -660,12 +599,7 @@ EXPORT void tea_leaf_cg_calc_w_kernel_cuda_(double *pw)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    double s = 0;
-    REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, {
-        W[i] = apply_A(P, i);
-        ta = P[i]; tb = W[i];
-    });
-    *pw = s;
+    if (g.fp32) cg_calc_w_f32(pw); else cg_calc_w_f64(pw);
 }
 
 /* tea_leaf_cg.cuknl:72-137 */
@@ -673,22 +607,7 @@ EXPORT void tea_leaf_cg_calc_ur_kernel_cuda_(double *alpha, double *rrn)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    const double a = *alpha;
-    const int prec = g.prec;
-    double s = 0;
-    if (prec == PREC_BLOCK) {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { U[i] += a * P[i]; R[i] -= a * W[i]; });
-        block_solve_extent(R, Z, IX0, IX1, IY0, IY1);
-        REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, { ta = R[i]; tb = Z[i]; });
-    } else {
-        REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, {
-            U[i] += a * P[i];
-            R[i] -= a * W[i];
-            if (prec == PREC_DIAG) { Z[i] = R[i] * MI[i]; ta = R[i]; tb = Z[i]; }
-            else { ta = R[i]; tb = R[i]; }
-        });
-    }
-    *rrn = s;
+    if (g.fp32) cg_calc_ur_f32(*alpha, rrn); else cg_calc_ur_f64(*alpha, rrn);
 }
 
 /* tea_leaf_cg.cuknl:141-161 */
@@ -696,9 +615,8 @@ EXPORT void tea_leaf_cg_calc_p_kernel_cuda_(double *beta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    const double b = *beta;
-    if (g.prec != PREC_NONE) FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = Z[i] + b * P[i]; });
-    else FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = R[i] + b * P[i]; });
+    if (g.fp32) calc_p_f32(*beta, g.prec != PREC_NONE ? g.f32[X_Z] : g.f32[F_R]);
+    else calc_p_f64(*beta, g.prec != PREC_NONE ? g.f[X_Z] : g.f[F_R]);
 }
 
 /* ---------------- solver: Chebyshev ------------------------------------------------ */
@@ -729,6 +647,7 @@ EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const doub
                                              int *n_coefs, const double *theta)
 {
     CHECK_INIT();
+    if (g.fp32) { die("%s: not available in the FP32 solver mode", __func__); return; }
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
     upload_coefs(ch_alphas, ch_betas, *n_coefs);
     const double th = *theta;
@@ -748,6 +667,7 @@ EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const doub
 EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
 {
     CHECK_INIT();
+    if (g.fp32) { die("%s: not available in the FP32 solver mode", __func__); return; }
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
     const int k = *cheby_calc_step - 1;
     if (k < 0 || k >= g.n_coefs) { die("cheby_iterate: step %d outside the %d uploaded coefficients", k + 1, g.n_coefs); return; }
@@ -770,6 +690,7 @@ EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
 EXPORT void tea_leaf_jacobi_solve_kernel_cuda_(double *error)
 {
     CHECK_INIT();
+    if (g.fp32) { die("%s: not available in the FP32 solver mode", __func__); return; }
     REQUIRE_SOLVER(1 << SOLVER_JACOBI);
     double *un = MI;
     const size_t pp = (size_t)g.pitch;
@@ -804,20 +725,7 @@ EXPORT void tea_leaf_ppcg_init_kernel_cuda_(const int *step, double *rro)
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     const int st = *step;
     if (st < 1 || st > 3) { die("ppcg_init: invalid step %d", st); return; }
-    if (st == 1) FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = 0.0; Z[i] = 0.0; });
-    else if (st == 3) FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = 0.0; });
-    if (g.prec != PREC_NONE || st == 3) {
-        if (st == 1 || st == 2) {
-            if (g.prec == PREC_BLOCK) block_solve_extent(R, Z, IX0, IX1, IY0, IY1);
-            else if (g.prec == PREC_DIAG) FOR_EXTENT(IX0, IX1, IY0, IY1, { Z[i] = MI[i] * R[i]; });
-        }
-        if (st == 1 || st == 3) FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = Z[i]; });
-    } else if (st == 1) {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = R[i]; });
-    }
-    double s = 0;
-    if (st == 1 || st == 3) REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, { ta = R[i]; tb = P[i]; });
-    *rro = s;
+    if (g.fp32) ppcg_init_f32(st, rro); else ppcg_init_f64(st, rro);
 }
 
 /* tea_leaf_ppcg.cuknl:5-55 (never called by the reference driver) */
@@ -825,15 +733,7 @@ EXPORT void tea_leaf_ppcg_init_sd_kernel_cuda_(const double *theta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    const double th = *theta;
-    if (g.prec == PREC_BLOCK) {
-        block_solve_extent(R, g.zl, IX0, IX1, IY0, IY1);
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = g.zl[i] / th; });
-    } else if (g.prec == PREC_DIAG) {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = R[i] * MI[i] / th; });
-    } else {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = R[i] / th; });
-    }
+    if (g.fp32) ppcg_init_sd_f32(*theta); else ppcg_init_sd_f64(*theta);
 }
 
 /* tea_leaf_ppcg.cuknl:58-89 */
@@ -841,11 +741,7 @@ EXPORT void tea_leaf_ppcg_init_sd_new_kernel_cuda_(const double *theta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    const double theta_r = 1.0 / *theta;
-    if (g.prec == PREC_DIAG || g.prec == PREC_BLOCK)
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = Z[i] * theta_r; RTEMP[i] = R[i]; UTEMP[i] = SD[i]; });
-    else
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { SD[i] = R[i] * theta_r; RTEMP[i] = R[i]; UTEMP[i] = SD[i]; });
+    if (g.fp32) ppcg_init_sd_new_f32(*theta); else ppcg_init_sd_new_f64(*theta);
 }
 
 /* Matrix-powers extent of one inner step (tea_leaf_kernel_cuda.cu:418-452). */
@@ -868,19 +764,8 @@ EXPORT void tea_leaf_ppcg_inner_kernel_cuda_(int *ppcg_cur_step, int *bounds_ext
     if (*bounds_extra < 0 || *bounds_extra >= g.h) { die("ppcg_inner: invalid bounds_extra %d", *bounds_extra); return; }
     int x0, x1, y0, y1;
     ppcg_extent(*bounds_extra, chunk_neighbours, &x0, &x1, &y0, &y1);
-    const double al = g.ch_a[k], be = g.ch_b[k];
-    FOR_EXTENT(x0, x1, y0, y1, {
-        const double result = apply_A(SD, i);
-        RTEMP[i] -= result;
-    });
-    if (g.prec == PREC_BLOCK) {
-        block_solve_extent(RTEMP, g.zl, x0, x1, y0, y1);
-        FOR_EXTENT(x0, x1, y0, y1, { SD[i] = al * SD[i] + be * g.zl[i]; UTEMP[i] += SD[i]; });
-    } else if (g.prec == PREC_DIAG) {
-        FOR_EXTENT(x0, x1, y0, y1, { SD[i] = al * SD[i] + be * RTEMP[i] * MI[i]; UTEMP[i] += SD[i]; });
-    } else {
-        FOR_EXTENT(x0, x1, y0, y1, { SD[i] = al * SD[i] + be * RTEMP[i]; UTEMP[i] += SD[i]; });
-    }
+    if (g.fp32) ppcg_inner_f32(g.ch_a[k], g.ch_b[k], x0, x1, y0, y1);
+    else ppcg_inner_f64(g.ch_a[k], g.ch_b[k], x0, x1, y0, y1);
 }
 
 /* tea_leaf_ppcg.cuknl:249-260 */
@@ -888,7 +773,7 @@ EXPORT void tea_leaf_ppcg_store_r_kernel_cuda_(void)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    FOR_EXTENT(IX0, IX1, IY0, IY1, { RSTORE[i] = R[i]; });
+    if (g.fp32) copy_f32(g.f32[X_RSTORE], g.f32[F_R]); else copy_f64(g.f[X_RSTORE], g.f[F_R]);
 }
 
 /* tea_leaf_ppcg.cuknl:263-274 */
@@ -896,7 +781,7 @@ EXPORT void tea_leaf_ppcg_update_z_kernel_cuda_(void)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    FOR_EXTENT(IX0, IX1, IY0, IY1, { Z[i] = UTEMP[i]; });
+    if (g.fp32) copy_f32(g.f32[X_Z], g.f32[X_UTEMP]); else copy_f64(g.f[X_Z], g.f[X_UTEMP]);
 }
 
 /* tea_leaf_ppcg.cuknl:279-297 */
@@ -904,18 +789,14 @@ EXPORT void tea_leaf_ppcg_calc_rrn_kernel_cuda_(double *norm)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    double s = 0;
-    REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, { ta = R[i] - RSTORE[i]; tb = Z[i]; });
-    *norm = s;
+    if (g.fp32) ppcg_calc_rrn_f32(norm); else ppcg_calc_rrn_f64(norm);
 }
 
 /* tea_leaf_kernel_cuda.cu:494-500 */
 EXPORT void tea_leaf_ppcg_calc_2norm_kernel_cuda_(double *norm)
 {
     CHECK_INIT();
-    double s = 0;
-    REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, { ta = R[i]; tb = Z[i]; });
-    *norm = s;
+    if (g.fp32) ppcg_calc_2norm_f32(norm); else ppcg_calc_2norm_f64(norm);
 }
 
 /* tea_leaf_ppcg.cuknl:301-313 */
@@ -923,8 +804,7 @@ EXPORT void tea_leaf_ppcg_calc_p_kernel_cuda_(double *beta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    const double b = *beta;
-    FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = Z[i] + b * P[i]; });
+    if (g.fp32) calc_p_f32(*beta, g.f32[X_Z]); else calc_p_f64(*beta, g.f[X_Z]);
 }
 
 /* ---------------- extensions ---------------------------------------------------------- */
@@ -960,3 +840,65 @@ EXPORT int tea_engine_profile_names(char *buf, int n) { if (buf && n > 0) buf[0]
 EXPORT void *tea_engine_stream(void) { return NULL; }
 EXPORT void tea_engine_synchronize(void) {}
 EXPORT void tea_engine_finalize(void) { free_all(); }
+
+/* ---------------- mixed precision (no reference counterpart) ---------------------------- */
+
+EXPORT int tea_engine_set_precision(int bits)
+{
+    if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return -1; }
+    if (bits != 32 && bits != 64) { die("tea_engine_set_precision: %d bits is not 32 or 64", bits); return -1; }
+    if (bits == 32 && !g.f32[1]) {
+        const size_t n2 = (size_t)g.pitch * (size_t)g.rows;
+        for (int k = 1; k < X_COUNT; k++) g.f32[k] = (float *)calloc(n2, sizeof(float));
+        g.zl32 = (float *)calloc(n2, sizeof(float));
+    }
+    g.fp32 = bits == 32;
+    return 0;
+}
+
+/* FP32 copies of the face coefficients, whole arrays */
+EXPORT void tea_engine_mp_setup(void)
+{
+    CHECK_INIT();
+    if (!g.f32[1]) { die("tea_engine_mp_setup: call tea_engine_set_precision(32) first"); return; }
+    const size_t n2 = (size_t)g.pitch * (size_t)g.rows;
+    for (size_t i = 0; i < n2; i++) { g.f32[X_KX][i] = (float)KX[i]; g.f32[X_KY][i] = (float)KY[i]; }
+}
+
+/* r32 = (float) r, u32 = 0 (the correction), rr = r32.r32 */
+EXPORT void tea_engine_mp_begin(double *rr)
+{
+    CHECK_INIT();
+    if (!g.f32[1]) { die("tea_engine_mp_begin: call tea_engine_set_precision(32) first"); return; }
+    float *R32 = g.f32[F_R], *U32 = g.f32[F_U];
+    double s = 0;
+    REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, {
+        R32[i] = (float)R[i];
+        U32[i] = 0.0f;
+        ta = R32[i]; tb = R32[i];
+    });
+    *rr = s;
+}
+
+/* u += (double) u32 */
+EXPORT void tea_engine_mp_end(void)
+{
+    CHECK_INIT();
+    if (!g.f32[1]) { die("tea_engine_mp_end: call tea_engine_set_precision(32) first"); return; }
+    const float *U32 = g.f32[F_U];
+    FOR_EXTENT(IX0, IX1, IY0, IY1, { U[i] += (double)U32[i]; });
+}
+
+EXPORT int tea_engine_get_field32(int id, float *host)
+{
+    if (!g.initialised || !g.f32[1] || id < 1 || id >= X_COUNT) return -1;
+    memcpy(host, g.f32[id], (size_t)g.pitch * (size_t)g.rows * sizeof(float));
+    return 0;
+}
+
+EXPORT int tea_engine_set_field32(int id, const float *host)
+{
+    if (!g.initialised || !g.f32[1] || id < 1 || id >= X_COUNT) return -1;
+    memcpy(g.f32[id], host, (size_t)g.pitch * (size_t)g.rows * sizeof(float));
+    return 0;
+}

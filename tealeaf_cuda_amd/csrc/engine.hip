@@ -23,11 +23,13 @@
 #include <fstream>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
 #include "../../include/tea_leaf_engine.h"
 #include "engine_device.hpp"
+#include "engine_solver.hpp"
 
 using namespace tea;
 
@@ -129,11 +131,20 @@ struct Chunk {
 
     std::vector<double> ch_a, ch_b;  // host copies of the uploaded coefficients
 
+    // FP32 shadow fields of the mixed-precision inner solver (allocated on the
+    // first tea_engine_set_precision(32)); same cell coordinates, own padding.
+    bool fp32 = false;
+    float *slab32 = nullptr;
+    float *fs32[N_SLOTS] = {nullptr};
+    int lead32 = 0;
+    long pitch32 = 0;
+
     // The fused inner step writes the new sd into the spare buffer, interior
     // only; the two buffers' halo rings agree until something writes the sd
     // halo (update_halo, unpack, set_field), after which the ring is copied
     // across before the next fused step.
     bool sd_halo_dirty = false;
+    bool sd32_halo_dirty = false;  // the same for the FP32 shadow (set_field32)
 
     Profiler prof;
 };
@@ -301,14 +312,6 @@ bool fetch(double *dst, int n)
 // =============================================================================
 namespace {
 
-// Loop over this workgroup's tiles; the body sees the lane's strip s and i0,
-// the flat index of the strip's first cell.
-#define STRIP_BEGIN                                                    \
-    for (TileLoop tl_(e); tl_.more(); tl_.advance()) {                 \
-        const Strip s = tl_.strip();                                   \
-        const long i0 = (long)s.c + (long)s.r0 * pitch;                \
-        (void)i0;
-#define STRIP_END }
 
 // ---- set-up ---------------------------------------------------------------------
 
@@ -379,16 +382,6 @@ __global__ __launch_bounds__(BLOCK) void k_init_u(Ext e, long pitch, const doubl
     STRIP_END
 }
 
-// Generic a = b copy (set_field_kernel.cuknl:5-16, ppcg store_r / update_z,
-// jacobi copy_u).
-__global__ __launch_bounds__(BLOCK) void k_copy(Ext e, long pitch, double *__restrict__ dst,
-                                                const double *__restrict__ src)
-{
-    STRIP_BEGIN
-    for (int k = 0; k < s.len; k++) dst[i0 + k * pitch] = src[i0 + k * pitch];
-    STRIP_END
-}
-
 // field_summary_kernel.cuknl:5-58
 __global__ __launch_bounds__(BLOCK) void k_field_summary(Ext e, long pitch, const double *__restrict__ vol,
                                                          const double *__restrict__ dens,
@@ -436,8 +429,8 @@ __global__ void k_reflect_lr(double *__restrict__ a, long pitch, int h, int nx, 
 
 // Copies the halo ring (every cell outside the interior) of src into dst:
 // bottom rows, top rows, then the left and right columns of the interior rows.
-__global__ void k_copy_ring(double *__restrict__ dst, const double *__restrict__ src, long pitch, int h, int nx,
-                            int ny)
+template <typename T>
+__global__ void k_copy_ring(T *__restrict__ dst, const T *__restrict__ src, long pitch, int h, int nx, int ny)
 {
     const long w = nx + 2L * h;
     const long n_bt = w * h;           // cells in the bottom (or top) band
@@ -548,19 +541,6 @@ __global__ __launch_bounds__(BLOCK) void k_calc_residual(Ext e, long pitch, cons
     STRIP_END
 }
 
-// tea_leaf_common.cuknl:41-58: sum a*b
-__global__ __launch_bounds__(BLOCK) void k_dot(Ext e, long pitch, const double *__restrict__ a,
-                                               const double *__restrict__ b, Red red)
-{
-    DD acc[1] = {{0, 0}};
-    STRIP_BEGIN
-#pragma unroll
-    for (int k = 0; k < STRIP; k++)
-        if (k < s.len) dd_add_prod(acc[0], a[i0 + k * pitch], b[i0 + k * pitch]);
-    STRIP_END
-    block_reduce<1>(acc, red);
-}
-
 // tea_leaf_common.cuknl:5-17: energy1 = u/density
 __global__ __launch_bounds__(BLOCK) void k_finalise(Ext e, long pitch, const double *__restrict__ dens,
                                                     const double *__restrict__ u, double *__restrict__ e1)
@@ -570,160 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalise(Ext e, long pitch, const dou
     STRIP_END
 }
 
-// tea_leaf_common.cuknl:131-147 (interior): Mi = 1/diag
-__global__ __launch_bounds__(BLOCK) void k_jac_diag(Ext e, long pitch, double *__restrict__ Mi,
-                                                    const double *__restrict__ Kx, const double *__restrict__ Ky)
-{
-    STRIP_BEGIN
-    for (int k = 0; k < s.len; k++) {
-        const long i = i0 + k * pitch;
-        const double diag = diag_A(Kx, Ky, i, pitch);
-        Mi[i] = 1.0 / diag;
-    }
-    STRIP_END
-}
-
-// tea_block_jacobi.cuknl:83-111: Thomas factors cp, bfp per strip
-__global__ __launch_bounds__(BLOCK) void k_block_init(Ext e, long pitch, double *__restrict__ cp,
-                                                      double *__restrict__ bfp, const double *__restrict__ Kx,
-                                                      const double *__restrict__ Ky)
-{
-    STRIP_BEGIN
-    if (s.len == 0) continue;
-    cp[i0] = (-Ky[i0 + pitch]) / diag_A(Kx, Ky, i0, pitch);
-    for (int k = 1; k < s.len; k++) {
-        const long i = i0 + k * pitch;
-        bfp[i] = 1.0 / (diag_A(Kx, Ky, i, pitch) - (-Ky[i]) * cp[i - pitch]);
-        cp[i] = (-Ky[i + pitch]) * bfp[i];
-    }
-    STRIP_END
-}
-
-// z = M^-1 r on a strip held in registers (block: Thomas; diag: r*Mi).
-template <int PREC>
-__device__ __forceinline__ void precond_strip(const double (&rv)[STRIP], double (&zv)[STRIP], int len, long i0,
-                                              long pitch, const double *__restrict__ Mi,
-                                              const double *__restrict__ cp, const double *__restrict__ bfp,
-                                              const double *__restrict__ Kx, const double *__restrict__ Ky)
-{
-    if (PREC == PREC_BLOCK) {
-        if (len > 0) block_solve_strip(rv, zv, len, cp, bfp, Kx, Ky, i0, pitch);
-    } else if (PREC == PREC_DIAG) {
-#pragma unroll
-        for (int k = 0; k < STRIP; k++)
-            if (k < len) zv[k] = rv[k] * Mi[i0 + k * pitch];
-    } else {
-#pragma unroll
-        for (int k = 0; k < STRIP; k++) zv[k] = rv[k];
-    }
-}
-
 // ---- solver: CG -------------------------------------------------------------------
-
-// tea_leaf_cg.cuknl:5-39 (+ block_solve for jac_block): z = M^-1 r, p = z or r, sum r*p
-template <int PREC>
-__global__ __launch_bounds__(BLOCK) void k_cg_init_p(Ext e, long pitch, double *__restrict__ p,
-                                                     const double *__restrict__ r, double *__restrict__ z,
-                                                     const double *__restrict__ Mi, const double *__restrict__ cp,
-                                                     const double *__restrict__ bfp, const double *__restrict__ Kx,
-                                                     const double *__restrict__ Ky, Red red)
-{
-    DD acc[1] = {{0, 0}};
-    STRIP_BEGIN
-    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < STRIP; k++)
-        if (k < s.len) rv[k] = r[i0 + k * pitch];
-    if (PREC != PREC_NONE) precond_strip<PREC>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            double pv;
-            if (PREC != PREC_NONE) {
-                z[i] = zv[k];
-                pv = zv[k];
-            } else {
-                pv = rv[k];
-            }
-            p[i] = pv;
-            dd_add_prod(acc[0], rv[k], pv);
-        }
-    }
-    STRIP_END
-    block_reduce<1>(acc, red);
-}
-
-// tea_leaf_cg.cuknl:43-68: w = A p, sum p*w
-__global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Ext e, long pitch, const double *__restrict__ p,
-                                                     double *__restrict__ w, const double *__restrict__ Kx,
-                                                     const double *__restrict__ Ky, Red red)
-{
-    DD acc[1] = {{0, 0}};
-    STRIP_BEGIN
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            const double wv = apply_A(p, Kx, Ky, i, pitch);
-            w[i] = wv;
-            dd_add_prod(acc[0], p[i], wv);
-        }
-    }
-    STRIP_END
-    block_reduce<1>(acc, red);
-}
-
-// tea_leaf_cg.cuknl:72-137: u += a p; r -= a w; z = M^-1 r; sum r*z (r*r)
-template <int PREC>
-__global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Ext e, long pitch, double alpha, double *__restrict__ u,
-                                                      const double *__restrict__ p, double *__restrict__ r,
-                                                      const double *__restrict__ w, double *__restrict__ z,
-                                                      const double *__restrict__ Mi, const double *__restrict__ cp,
-                                                      const double *__restrict__ bfp, const double *__restrict__ Kx,
-                                                      const double *__restrict__ Ky, Red red)
-{
-    DD acc[1] = {{0, 0}};
-    STRIP_BEGIN
-    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            u[i] += alpha * p[i];
-            rv[k] = r[i] - alpha * w[i];
-            r[i] = rv[k];
-        }
-    }
-    if (PREC != PREC_NONE) precond_strip<PREC>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            if (PREC != PREC_NONE) {
-                z[i0 + k * pitch] = zv[k];
-                dd_add_prod(acc[0], rv[k], zv[k]);
-            } else {
-                dd_add_prod(acc[0], rv[k], rv[k]);
-            }
-        }
-    }
-    STRIP_END
-    block_reduce<1>(acc, red);
-}
-
-// tea_leaf_cg.cuknl:141-161 and tea_leaf_ppcg.cuknl:301-313: p = src + beta p
-__global__ __launch_bounds__(BLOCK) void k_calc_p(Ext e, long pitch, double beta, double *__restrict__ p,
-                                                  const double *__restrict__ src)
-{
-    STRIP_BEGIN
-#pragma unroll
-    for (int k = 0; k < STRIP; k++)
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            p[i] = src[i] + beta * p[i];
-        }
-    STRIP_END
-}
 
 // ---- solver: Chebyshev --------------------------------------------------------------
 
@@ -733,8 +560,7 @@ template <int PREC, int FIRST>
 __global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const double *__restrict__ u,
                                                    const double *__restrict__ u0, double *__restrict__ p,
                                                    double *__restrict__ r, double *__restrict__ w,
-                                                   const double *__restrict__ Mi, const double *__restrict__ cp,
-                                                   const double *__restrict__ bfp, const double *__restrict__ Kx,
+                                                   const double *__restrict__ Mi, const double *__restrict__ Kx,
                                                    const double *__restrict__ Ky, double theta, double alpha,
                                                    double beta)
 {
@@ -750,7 +576,7 @@ __global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const doub
             r[i] = rv[k];
         }
     }
-    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
+    if (PREC == PREC_BLOCK) precond_strip<double, PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, Kx, Ky);
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
@@ -808,204 +634,6 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Ext e, long pitch, const doubl
 
 // ---- solver: PPCG ---------------------------------------------------------------------
 
-// tea_leaf_ppcg.cuknl:319-420
-template <int PREC>
-__global__ __launch_bounds__(BLOCK) void k_ppcg_init(Ext e, long pitch, double *__restrict__ p,
-                                                     const double *__restrict__ r, double *__restrict__ z,
-                                                     const double *__restrict__ Mi, const double *__restrict__ cp,
-                                                     const double *__restrict__ bfp, const double *__restrict__ Kx,
-                                                     const double *__restrict__ Ky, int step, Red red)
-{
-    DD acc[1] = {{0, 0}};
-    STRIP_BEGIN
-    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < STRIP; k++)
-        if (k < s.len) rv[k] = r[i0 + k * pitch];
-    if (PREC != PREC_NONE && (step == 1 || step == 2))
-        precond_strip<PREC>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            if (step == 1) {
-                // p = z = 0, then z = M^-1 r and p = z (or p = r)
-                if (PREC != PREC_NONE) {
-                    z[i] = zv[k];
-                    p[i] = zv[k];
-                } else {
-                    z[i] = 0.0;
-                    p[i] = rv[k];
-                }
-                dd_add_prod(acc[0], rv[k], p[i]);
-            } else if (step == 2) {
-                if (PREC != PREC_NONE) z[i] = zv[k];
-            } else {
-                const double pv = z[i];
-                p[i] = pv;
-                dd_add_prod(acc[0], rv[k], pv);
-            }
-        }
-    }
-    STRIP_END
-    block_reduce<1>(acc, red);
-}
-
-// tea_leaf_ppcg.cuknl:5-55 (init_sd, not called by the reference driver)
-template <int PREC>
-__global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd(Ext e, long pitch, const double *__restrict__ r,
-                                                        double *__restrict__ sd, const double *__restrict__ Mi,
-                                                        const double *__restrict__ cp,
-                                                        const double *__restrict__ bfp,
-                                                        const double *__restrict__ Kx,
-                                                        const double *__restrict__ Ky, double theta)
-{
-    STRIP_BEGIN
-    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < STRIP; k++)
-        if (k < s.len) rv[k] = r[i0 + k * pitch];
-    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            if (PREC == PREC_BLOCK) sd[i] = zv[k] / theta;
-            else if (PREC == PREC_DIAG) sd[i] = rv[k] * Mi[i] / theta;
-            else sd[i] = rv[k] / theta;
-        }
-    }
-    STRIP_END
-}
-
-// tea_leaf_ppcg.cuknl:58-89: sd = (z or r) * (1/theta); rtemp = r; utemp = sd
-__global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd_new(Ext e, long pitch, const double *__restrict__ r,
-                                                            double *__restrict__ sd,
-                                                            const double *__restrict__ src,
-                                                            double *__restrict__ rtemp,
-                                                            double *__restrict__ utemp, double theta_r)
-{
-    STRIP_BEGIN
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            const double sdv = src[i] * theta_r;
-            sd[i] = sdv;
-            rtemp[i] = r[i];
-            utemp[i] = sdv;
-        }
-    }
-    STRIP_END
-}
-
-// tea_leaf_ppcg.cuknl:154-173: rtemp -= A sd
-__global__ __launch_bounds__(BLOCK) void k_ppcg_update_r(Ext e, long pitch, double *__restrict__ rtemp,
-                                                         const double *__restrict__ Kx,
-                                                         const double *__restrict__ Ky,
-                                                         const double *__restrict__ sd)
-{
-    STRIP_BEGIN
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            const double result = apply_A(sd, Kx, Ky, i, pitch);
-            rtemp[i] -= result;
-        }
-    }
-    STRIP_END
-}
-
-// tea_leaf_ppcg.cuknl:178-244: sd = alpha sd + beta M^-1 rtemp; utemp += sd
-template <int PREC>
-__global__ __launch_bounds__(BLOCK) void k_ppcg_calc_sd(Ext e, long pitch, double *__restrict__ sd,
-                                                        const double *__restrict__ rtemp,
-                                                        double *__restrict__ utemp, const double *__restrict__ Mi,
-                                                        const double *__restrict__ cp,
-                                                        const double *__restrict__ bfp,
-                                                        const double *__restrict__ Kx,
-                                                        const double *__restrict__ Ky, double alpha, double beta)
-{
-    STRIP_BEGIN
-    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < STRIP; k++)
-        if (k < s.len) rv[k] = rtemp[i0 + k * pitch];
-    if (PREC == PREC_BLOCK) precond_strip<PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, cp, bfp, Kx, Ky);
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            double sdv;
-            if (PREC == PREC_BLOCK) sdv = alpha * sd[i] + beta * zv[k];
-            else if (PREC == PREC_DIAG) sdv = alpha * sd[i] + beta * rv[k] * Mi[i];
-            else sdv = alpha * sd[i] + beta * rv[k];
-            sd[i] = sdv;
-            utemp[i] += sdv;
-        }
-    }
-    STRIP_END
-}
-
-// One PPCG inner step in one pass (tea_leaf_ppcg.cuknl:154-173 then :178-244):
-// rtemp -= A sd; sd' = alpha sd + beta M^-1 rtemp; utemp += sd'.  sd is read
-// from sd_in and the new sd written to sd_out, so every cell sees the previous
-// step's sd exactly as the two separate launches of the reference do.
-template <int PREC>
-__global__ __launch_bounds__(BLOCK) void k_ppcg_inner_fused(Ext e, long pitch, const double *__restrict__ sd_in,
-                                                            double *__restrict__ sd_out, double *__restrict__ rtemp,
-                                                            double *__restrict__ utemp,
-                                                            const double *__restrict__ Mi,
-                                                            const double *__restrict__ cp,
-                                                            const double *__restrict__ bfp,
-                                                            const double *__restrict__ Kx,
-                                                            const double *__restrict__ Ky, double alpha, double beta)
-{
-    STRIP_BEGIN
-    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            const double result = apply_A(sd_in, Kx, Ky, i, pitch);
-            rv[k] = rtemp[i] - result;
-            rtemp[i] = rv[k];
-        }
-    }
-    if (PREC == PREC_BLOCK && s.len > 0) block_solve_strip_regs(rv, zv, s.len, Kx, Ky, i0, pitch);
-#pragma unroll
-    for (int k = 0; k < STRIP; k++) {
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            double sdv;
-            if (PREC == PREC_BLOCK) sdv = alpha * sd_in[i] + beta * zv[k];
-            else if (PREC == PREC_DIAG) sdv = alpha * sd_in[i] + beta * rv[k] * Mi[i];
-            else sdv = alpha * sd_in[i] + beta * rv[k];
-            sd_out[i] = sdv;
-            utemp[i] += sdv;
-        }
-    }
-    STRIP_END
-}
-
-// tea_leaf_ppcg.cuknl:279-297: sum (r - r_store)*z
-__global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const double *__restrict__ rstore,
-                                                    const double *__restrict__ r, const double *__restrict__ z,
-                                                    Red red)
-{
-    DD acc[1] = {{0, 0}};
-    STRIP_BEGIN
-#pragma unroll
-    for (int k = 0; k < STRIP; k++)
-        if (k < s.len) {
-            const long i = i0 + k * pitch;
-            dd_add_prod(acc[0], r[i] - rstore[i], z[i]);
-        }
-    STRIP_END
-    block_reduce<1>(acc, red);
-}
-
 }  // namespace
 
 // =============================================================================
@@ -1030,6 +658,7 @@ void release_all()
     if (g.d_partials) (void)hipFree(g.d_partials);
     if (g.h_out) (void)hipHostFree(g.h_out);
     if (g.d_halo_buf) (void)hipFree(g.d_halo_buf);
+    if (g.slab32) (void)hipFree(g.slab32);
     if (g.stream) (void)hipStreamDestroy(g.stream);
     Profiler keep = g.prof;
     g = Chunk();
@@ -1166,7 +795,7 @@ EXPORT void generate_chunk_kernel_cuda_(const int *number_of_states, const doubl
 EXPORT void set_field_kernel_cuda_(void)
 {
     CHECK_INIT();
-    launch("set_field", k_copy, interior(), g.pitch, F(F_ENERGY1), (const double *)F(F_ENERGY0));
+    launch("set_field", k_copy<double>, interior(), g.pitch, F(F_ENERGY1), (const double *)F(F_ENERGY0));
 }
 
 EXPORT void field_summary_kernel_cuda_(double *vol, double *mass, double *ie, double *temp)
@@ -1308,54 +937,193 @@ EXPORT void tea_leaf_calc_2norm_kernel_cuda_(int *norm_array, double *norm)
     else if (*norm_array == 1) { a = F(F_R); b = F(F_R); }
     else if (*norm_array == 2) { a = F(F_R); b = (g.prec != PREC_NONE) ? F(X_Z) : F(F_R); }
     else { die("Invalid value '%d' for norm_array passed, should be [0, 1, 2]", *norm_array); return; }
-    if (!launch("calc_2norm", k_dot, interior(), g.pitch, a, b, red_ctx())) return;
+    if (!launch("calc_2norm", k_dot<double>, interior(), g.pitch, a, b, red_ctx())) return;
     fetch(norm, 1);
 }
+
+namespace {
+
+// Fields in the precision of the current solver mode: FP64 (the reference's
+// path) or FP32 (the inner solver of the mixed-precision refinement).
+template <typename T> T *FT(int id);
+template <> double *FT<double>(int id) { return g.fs[id]; }
+template <> float *FT<float>(int id) { return g.fs32[id]; }
+template <typename T> long pitch_of() { return std::is_same<T, float>::value ? g.pitch32 : g.pitch; }
+
+// Profiler name of a launch: FP32 launches carry a "_f32" suffix.
+template <typename T> const char *nm(const char *base)
+{
+    if (!std::is_same<T, float>::value) return base;
+    static std::map<std::string, std::string> names;
+    auto it = names.find(base);
+    if (it == names.end()) it = names.emplace(base, std::string(base) + "_f32").first;
+    return it->second.c_str();
+}
+
+#define PREC_SWITCH_T(T, PREC_VALUE, NAME, KERNEL, ...)                                           \
+    ((PREC_VALUE) == PREC_BLOCK ? launch(nm<T>(NAME), KERNEL<T, PREC_BLOCK>, __VA_ARGS__)          \
+     : (PREC_VALUE) == PREC_DIAG ? launch(nm<T>(NAME), KERNEL<T, PREC_DIAG>, __VA_ARGS__)          \
+                                 : launch(nm<T>(NAME), KERNEL<T, PREC_NONE>, __VA_ARGS__))
+
+template <typename T> void cg_init_impl(double *rro)
+{
+    const Ext in = interior();
+    const long pt = pitch_of<T>();
+    if (g.prec == PREC_BLOCK && std::is_same<T, double>::value)
+        launch("block_init", k_block_init<T>, in, pt, FT<T>(X_CP), FT<T>(X_BFP), (const T *)FT<T>(X_KX),
+               (const T *)FT<T>(X_KY));
+    else if (g.prec == PREC_DIAG)
+        launch(nm<T>("init_jac_diag"), k_jac_diag<T>, in, pt, FT<T>(X_MI), (const T *)FT<T>(X_KX),
+               (const T *)FT<T>(X_KY));
+    if (!PREC_SWITCH_T(T, g.prec, "k_cg_init_p", k_cg_init_p, in, pt, FT<T>(F_P), (const T *)FT<T>(F_R), FT<T>(X_Z),
+                       (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), red_ctx()))
+        return;
+    fetch(rro, 1);
+}
+
+template <typename T> void cg_calc_w_impl(double *pw)
+{
+    if (!launch(nm<T>("cg_calc_w"), k_cg_calc_w<T>, interior(), pitch_of<T>(), (const T *)FT<T>(F_P), FT<T>(X_W),
+                (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), red_ctx()))
+        return;
+    fetch(pw, 1);
+}
+
+template <typename T> void cg_calc_ur_impl(double alpha, double *rrn)
+{
+    if (!PREC_SWITCH_T(T, g.prec, "k_cg_calc_ur", k_cg_calc_ur, interior(), pitch_of<T>(), alpha, FT<T>(F_U),
+                       (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z), (const T *)FT<T>(X_MI),
+                       (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), red_ctx()))
+        return;
+    fetch(rrn, 1);
+}
+
+template <typename T> void cg_calc_p_impl(double beta)
+{
+    launch(nm<T>("cg_calc_p"), k_calc_p<T>, interior(), pitch_of<T>(), beta, FT<T>(F_P),
+           (const T *)(g.prec != PREC_NONE ? FT<T>(X_Z) : FT<T>(F_R)));
+}
+
+template <typename T> void ppcg_init_impl(int st, double *rro)
+{
+    if (!PREC_SWITCH_T(T, g.prec, "k_ppcg_init", k_ppcg_init, interior(), pitch_of<T>(), FT<T>(F_P),
+                       (const T *)FT<T>(F_R), FT<T>(X_Z), (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX),
+                       (const T *)FT<T>(X_KY), st, red_ctx()))
+        return;
+    double v = 0.0;
+    if (!fetch(&v, 1)) return;
+    *rro = (st == 2) ? 0.0 : v;
+}
+
+template <typename T> void ppcg_init_sd_impl(double theta)
+{
+    PREC_SWITCH_T(T, g.prec, "k_ppcg_init_sd", k_ppcg_init_sd, interior(), pitch_of<T>(), (const T *)FT<T>(F_R),
+                  FT<T>(F_SD), (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), theta);
+}
+
+template <typename T> void ppcg_init_sd_new_impl(double theta)
+{
+    launch(nm<T>("ppcg_init_sd_new"), k_ppcg_init_sd_new<T>, interior(), pitch_of<T>(), (const T *)FT<T>(F_R),
+           FT<T>(F_SD), (const T *)(g.prec != PREC_NONE ? FT<T>(X_Z) : FT<T>(F_R)), FT<T>(X_RTEMP),
+           FT<T>(X_UTEMP), theta);
+}
+
+template <typename T> void ppcg_inner_impl(const Ext &e, double alpha, double beta)
+{
+    const long pt = pitch_of<T>();
+    if (fused_inner()) {
+        bool &dirty = std::is_same<T, double>::value ? g.sd_halo_dirty : g.sd32_halo_dirty;
+        if (dirty) {
+            const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+            Timed t("sd_halo_ring");
+            hipLaunchKernelGGL(k_copy_ring<T>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream,
+                               FT<T>(X_SD2), (const T *)FT<T>(F_SD), pt, g.h, g.nx, g.ny);
+            dirty = false;
+        }
+        // the new sd goes to the spare buffer, which then becomes the sd field
+        if (!PREC_SWITCH_T(T, g.prec, "k_ppcg_inner_fused", k_ppcg_inner_fused, e, pt, (const T *)FT<T>(F_SD),
+                           FT<T>(X_SD2), FT<T>(X_RTEMP), FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
+                           (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), alpha, beta))
+            return;
+        if (std::is_same<T, double>::value) {
+            std::swap(g.f[F_SD], g.f[X_SD2]);
+            std::swap(g.fs[F_SD], g.fs[X_SD2]);
+        } else {
+            std::swap(g.fs32[F_SD], g.fs32[X_SD2]);
+        }
+        return;
+    }
+    if (!launch(nm<T>("ppcg_update_r"), k_ppcg_update_r<T>, e, pt, FT<T>(X_RTEMP), (const T *)FT<T>(X_KX),
+                (const T *)FT<T>(X_KY), (const T *)FT<T>(F_SD)))
+        return;
+    PREC_SWITCH_T(T, g.prec, "k_ppcg_calc_sd", k_ppcg_calc_sd, e, pt, FT<T>(F_SD), (const T *)FT<T>(X_RTEMP),
+                  FT<T>(X_UTEMP), (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), alpha,
+                  beta);
+}
+
+template <typename T> void ppcg_store_r_impl()
+{
+    launch(nm<T>("ppcg_store_r"), k_copy<T>, interior(), pitch_of<T>(), FT<T>(X_RSTORE), (const T *)FT<T>(F_R));
+}
+
+template <typename T> void ppcg_update_z_impl()
+{
+    launch(nm<T>("ppcg_update_z"), k_copy<T>, interior(), pitch_of<T>(), FT<T>(X_Z), (const T *)FT<T>(X_UTEMP));
+}
+
+template <typename T> void ppcg_calc_rrn_impl(double *norm)
+{
+    if (!launch(nm<T>("ppcg_calc_rrn"), k_calc_rrn<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_RSTORE),
+                (const T *)FT<T>(F_R), (const T *)FT<T>(X_Z), red_ctx()))
+        return;
+    fetch(norm, 1);
+}
+
+template <typename T> void ppcg_calc_2norm_impl(double *norm)
+{
+    if (!launch(nm<T>("ppcg_calc_2norm"), k_dot<T>, interior(), pitch_of<T>(), (const T *)FT<T>(F_R),
+                (const T *)FT<T>(X_Z), red_ctx()))
+        return;
+    fetch(norm, 1);
+}
+
+template <typename T> void ppcg_calc_p_impl(double beta)
+{
+    launch(nm<T>("ppcg_calc_p"), k_calc_p<T>, interior(), pitch_of<T>(), beta, FT<T>(F_P), (const T *)FT<T>(X_Z));
+}
+
+}  // namespace
+
+#define BY_PRECISION(fn, ...) (g.fp32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+#define REQUIRE_FP64()                                                                            \
+    do { if (g.fp32) { die("%s: not available in the FP32 solver mode", __func__); return; } } while (0)
 
 EXPORT void tea_leaf_cg_init_kernel_cuda_(double *rro)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    const Ext in = interior();
-    if (g.prec == PREC_BLOCK)
-        launch("block_init", k_block_init, in, g.pitch, F(X_CP), F(X_BFP), (const double *)F(X_KX),
-               (const double *)F(X_KY));
-    else if (g.prec == PREC_DIAG)
-        launch("init_jac_diag", k_jac_diag, in, g.pitch, F(X_MI), (const double *)F(X_KX), (const double *)F(X_KY));
-    if (!PREC_SWITCH(g.prec, k_cg_init_p, in, g.pitch, F(F_P), (const double *)F(F_R), F(X_Z),
-                     (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
-                     (const double *)F(X_KX), (const double *)F(X_KY), red_ctx()))
-        return;
-    fetch(rro, 1);
+    BY_PRECISION(cg_init_impl, rro);
 }
 
 EXPORT void tea_leaf_cg_calc_w_kernel_cuda_(double *pw)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    if (!launch("cg_calc_w", k_cg_calc_w, interior(), g.pitch, (const double *)F(F_P), F(X_W),
-                (const double *)F(X_KX), (const double *)F(X_KY), red_ctx()))
-        return;
-    fetch(pw, 1);
+    BY_PRECISION(cg_calc_w_impl, pw);
 }
 
 EXPORT void tea_leaf_cg_calc_ur_kernel_cuda_(double *alpha, double *rrn)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    if (!PREC_SWITCH(g.prec, k_cg_calc_ur, interior(), g.pitch, *alpha, F(F_U), (const double *)F(F_P), F(F_R),
-                     (const double *)F(X_W), F(X_Z), (const double *)F(X_MI), (const double *)F(X_CP),
-                     (const double *)F(X_BFP), (const double *)F(X_KX), (const double *)F(X_KY), red_ctx()))
-        return;
-    fetch(rrn, 1);
+    BY_PRECISION(cg_calc_ur_impl, *alpha, rrn);
 }
 
 EXPORT void tea_leaf_cg_calc_p_kernel_cuda_(double *beta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(S_CG_FAMILY);
-    launch("cg_calc_p", k_calc_p, interior(), g.pitch, *beta, F(F_P),
-           (const double *)(g.prec != PREC_NONE ? F(X_Z) : F(F_R)));
+    BY_PRECISION(cg_calc_p_impl, *beta);
 }
 
 namespace {
@@ -1373,8 +1141,8 @@ bool cheby_p(double theta, double alpha, double beta)
     const Ext in = interior();
     auto args = [&](auto kernel) {
         return launch(name, kernel, in, g.pitch, (const double *)F(F_U), (const double *)F(X_U0), F(F_P), F(F_R),
-                      F(X_W), (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
-                      (const double *)F(X_KX), (const double *)F(X_KY), theta, alpha, beta);
+                      F(X_W), (const double *)F(X_MI), (const double *)F(X_KX), (const double *)F(X_KY), theta,
+                      alpha, beta);
     };
     if (g.prec == PREC_BLOCK) return args(k_cheby_p<PREC_BLOCK, FIRST>);
     if (g.prec == PREC_DIAG) return args(k_cheby_p<PREC_DIAG, FIRST>);
@@ -1387,6 +1155,7 @@ EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const doub
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
+    REQUIRE_FP64();
     upload_coefs(ch_alphas, ch_betas, *n_coefs);
     if (!cheby_p<1>(*theta, 0.0, 0.0)) return;
     launch("cheby_calc_u", k_add, interior(), g.pitch, F(F_U), (const double *)F(F_P));
@@ -1396,6 +1165,7 @@ EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
+    REQUIRE_FP64();
     const int k = *cheby_calc_step - 1;
     if (k < 0 || k >= (int)g.ch_a.size()) {
         die("cheby_iterate: step %d outside the %d uploaded coefficients", k + 1, (int)g.ch_a.size());
@@ -1409,8 +1179,9 @@ EXPORT void tea_leaf_jacobi_solve_kernel_cuda_(double *error)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_JACOBI);
+    REQUIRE_FP64();
     const Ext in = interior();
-    launch("jacobi_copy_u", k_copy, in, g.pitch, F(X_MI), (const double *)F(F_U));
+    launch("jacobi_copy_u", k_copy<double>, in, g.pitch, F(X_MI), (const double *)F(F_U));
     if (!launch("jacobi_solve", k_jacobi, in, g.pitch, (const double *)F(X_KX), (const double *)F(X_KY),
                 (const double *)F(X_U0), F(F_U), (const double *)F(X_MI), red_ctx()))
         return;
@@ -1429,31 +1200,21 @@ EXPORT void tea_leaf_ppcg_init_kernel_cuda_(const int *step, double *rro)
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     const int st = *step;
     if (st < 1 || st > 3) { die("ppcg_init: invalid step %d", st); return; }
-    if (!PREC_SWITCH(g.prec, k_ppcg_init, interior(), g.pitch, F(F_P), (const double *)F(F_R), F(X_Z),
-                     (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
-                     (const double *)F(X_KX), (const double *)F(X_KY), st, red_ctx()))
-        return;
-    double v = 0.0;
-    if (!fetch(&v, 1)) return;
-    *rro = (st == 2) ? 0.0 : v;
+    BY_PRECISION(ppcg_init_impl, st, rro);
 }
 
 EXPORT void tea_leaf_ppcg_init_sd_kernel_cuda_(const double *theta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    PREC_SWITCH(g.prec, k_ppcg_init_sd, interior(), g.pitch, (const double *)F(F_R), F(F_SD),
-                (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
-                (const double *)F(X_KX), (const double *)F(X_KY), *theta);
+    BY_PRECISION(ppcg_init_sd_impl, *theta);
 }
 
 EXPORT void tea_leaf_ppcg_init_sd_new_kernel_cuda_(const double *theta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    const double theta_r = 1.0 / *theta;
-    launch("ppcg_init_sd_new", k_ppcg_init_sd_new, interior(), g.pitch, (const double *)F(F_R), F(F_SD),
-           (const double *)(g.prec != PREC_NONE ? F(X_Z) : F(F_R)), F(X_RTEMP), F(X_UTEMP), theta_r);
+    BY_PRECISION(ppcg_init_sd_new_impl, *theta);
 }
 
 EXPORT void tea_leaf_ppcg_inner_kernel_cuda_(int *ppcg_cur_step, int *bounds_extra, int *chunk_neighbours)
@@ -1474,69 +1235,41 @@ EXPORT void tea_leaf_ppcg_inner_kernel_cuda_(int *ppcg_cur_step, int *bounds_ext
     e.x1 = h + g.nx - 1 + ((chunk_neighbours[TEA_CHUNK_RIGHT - 1] == TEA_EXTERNAL_FACE) ? 0 : be);
     e.y0 = (chunk_neighbours[TEA_CHUNK_BOTTOM - 1] == TEA_EXTERNAL_FACE) ? h : h - be;
     e.y1 = h + g.ny - 1 + ((chunk_neighbours[TEA_CHUNK_TOP - 1] == TEA_EXTERNAL_FACE) ? 0 : be);
-    if (fused_inner()) {
-        if (g.sd_halo_dirty) {
-            const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
-            Timed t("sd_halo_ring");
-            hipLaunchKernelGGL(k_copy_ring, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream, F(X_SD2),
-                               (const double *)F(F_SD), g.pitch, g.h, g.nx, g.ny);
-            g.sd_halo_dirty = false;
-        }
-        // the new sd goes to the spare buffer, which then becomes the sd field
-        if (!PREC_SWITCH(g.prec, k_ppcg_inner_fused, e, g.pitch, (const double *)F(F_SD), F(X_SD2), F(X_RTEMP),
-                         F(X_UTEMP), (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
-                         (const double *)F(X_KX), (const double *)F(X_KY), g.ch_a[k], g.ch_b[k]))
-            return;
-        std::swap(g.f[F_SD], g.f[X_SD2]);
-        std::swap(g.fs[F_SD], g.fs[X_SD2]);
-        return;
-    }
-    if (!launch("ppcg_update_r", k_ppcg_update_r, e, g.pitch, F(X_RTEMP), (const double *)F(X_KX),
-                (const double *)F(X_KY), (const double *)F(F_SD)))
-        return;
-    PREC_SWITCH(g.prec, k_ppcg_calc_sd, e, g.pitch, F(F_SD), (const double *)F(X_RTEMP), F(X_UTEMP),
-                (const double *)F(X_MI), (const double *)F(X_CP), (const double *)F(X_BFP),
-                (const double *)F(X_KX), (const double *)F(X_KY), g.ch_a[k], g.ch_b[k]);
+    BY_PRECISION(ppcg_inner_impl, e, g.ch_a[k], g.ch_b[k]);
 }
 
 EXPORT void tea_leaf_ppcg_store_r_kernel_cuda_(void)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    launch("ppcg_store_r", k_copy, interior(), g.pitch, F(X_RSTORE), (const double *)F(F_R));
+    BY_PRECISION(ppcg_store_r_impl);
 }
 
 EXPORT void tea_leaf_ppcg_update_z_kernel_cuda_(void)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    launch("ppcg_update_z", k_copy, interior(), g.pitch, F(X_Z), (const double *)F(X_UTEMP));
+    BY_PRECISION(ppcg_update_z_impl);
 }
 
 EXPORT void tea_leaf_ppcg_calc_rrn_kernel_cuda_(double *norm)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    if (!launch("ppcg_calc_rrn", k_calc_rrn, interior(), g.pitch, (const double *)F(X_RSTORE),
-                (const double *)F(F_R), (const double *)F(X_Z), red_ctx()))
-        return;
-    fetch(norm, 1);
+    BY_PRECISION(ppcg_calc_rrn_impl, norm);
 }
 
 EXPORT void tea_leaf_ppcg_calc_2norm_kernel_cuda_(double *norm)
 {
     CHECK_INIT();
-    if (!launch("ppcg_calc_2norm", k_dot, interior(), g.pitch, (const double *)F(F_R), (const double *)F(X_Z),
-                red_ctx()))
-        return;
-    fetch(norm, 1);
+    BY_PRECISION(ppcg_calc_2norm_impl, norm);
 }
 
 EXPORT void tea_leaf_ppcg_calc_p_kernel_cuda_(double *beta)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
-    launch("ppcg_calc_p", k_calc_p, interior(), g.pitch, *beta, F(F_P), (const double *)F(X_Z));
+    BY_PRECISION(ppcg_calc_p_impl, *beta);
 }
 
 // =============================================================================
@@ -1624,3 +1357,67 @@ EXPORT void tea_engine_synchronize(void)
     if (g.prof.on) g.prof.drain(false);
 }
 EXPORT void tea_engine_finalize(void) { release_all(); }
+
+// ---- mixed precision -------------------------------------------------------------
+
+EXPORT int tea_engine_set_precision(int bits)
+{
+    if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return -1; }
+    if (bits != 32 && bits != 64) { die("tea_engine_set_precision: %d bits is not 32 or 64", bits); return -1; }
+    if (bits == 32 && !g.slab32) {
+        g.lead32 = (g.h + 31) / 32 * 32;                       // 128-byte aligned interior
+        g.pitch32 = (long)((g.lead32 + g.nx + g.h + 31) / 32 * 32);
+        const size_t elems = ((size_t)g.pitch32 * (size_t)g.rows + 32 + 63) / 64 * 64;
+        HIP_OK_R(hipMalloc(&g.slab32, elems * N_SLOTS * sizeof(float)), -1);
+        HIP_OK_R(hipMemsetAsync(g.slab32, 0, elems * N_SLOTS * sizeof(float), g.stream), -1);
+        for (int k = 1; k < N_SLOTS; k++) g.fs32[k] = g.slab32 + elems * k + (g.lead32 - g.h);
+    }
+    g.fp32 = bits == 32;
+    return 0;
+}
+
+EXPORT void tea_engine_mp_setup(void)
+{
+    CHECK_INIT();
+    if (!g.slab32) { die("tea_engine_mp_setup: call tea_engine_set_precision(32) first"); return; }
+    launch("mp_coef", k_mp_coef, Ext{0, g.nx + 2 * g.h - 1, 0, g.ny + 2 * g.h - 1}, g.pitch, g.pitch32,
+           (const double *)F(X_KX), (const double *)F(X_KY), g.fs32[X_KX], g.fs32[X_KY]);
+}
+
+EXPORT void tea_engine_mp_begin(double *rr)
+{
+    CHECK_INIT();
+    if (!g.slab32) { die("tea_engine_mp_begin: call tea_engine_set_precision(32) first"); return; }
+    if (!launch("mp_begin", k_mp_begin, interior(), g.pitch, g.pitch32, (const double *)F(F_R), g.fs32[F_R],
+                g.fs32[F_U], red_ctx()))
+        return;
+    fetch(rr, 1);
+}
+
+EXPORT void tea_engine_mp_end(void)
+{
+    CHECK_INIT();
+    if (!g.slab32) { die("tea_engine_mp_end: call tea_engine_set_precision(32) first"); return; }
+    launch("mp_end", k_mp_end, interior(), g.pitch, g.pitch32, F(F_U), (const float *)g.fs32[F_U]);
+}
+
+EXPORT int tea_engine_get_field32(int id, float *host)
+{
+    if (!g.initialised || !g.slab32 || id < 1 || id >= X_COUNT) return -1;
+    const size_t w = (size_t)g.nx + 2 * g.h;
+    HIP_OK_R(hipMemcpy2DAsync(host, w * sizeof(float), g.fs32[id], g.pitch32 * sizeof(float), w * sizeof(float),
+                              (size_t)g.rows, hipMemcpyDeviceToHost, g.stream), -1);
+    HIP_OK_R(hipStreamSynchronize(g.stream), -1);
+    return 0;
+}
+
+EXPORT int tea_engine_set_field32(int id, const float *host)
+{
+    if (!g.initialised || !g.slab32 || id < 1 || id >= X_COUNT) return -1;
+    const size_t w = (size_t)g.nx + 2 * g.h;
+    HIP_OK_R(hipMemcpy2DAsync(g.fs32[id], g.pitch32 * sizeof(float), host, w * sizeof(float), w * sizeof(float),
+                              (size_t)g.rows, hipMemcpyHostToDevice, g.stream), -1);
+    if (id == F_SD) g.sd32_halo_dirty = true;
+    HIP_OK_R(hipStreamSynchronize(g.stream), -1);
+    return 0;
+}

@@ -181,62 +181,43 @@ struct TileLoop {
 };
 
 // 5-point operator (I - dt div K grad) x at flat index i, in the expression
-// order of tea_leaf_cg.cuknl:53-57.
-__device__ __forceinline__ double apply_A(const double *__restrict__ x, const double *__restrict__ Kx,
-                                          const double *__restrict__ Ky, long i, long pitch)
+// order of tea_leaf_cg.cuknl:53-57, evaluated in the field type T (FP64, or
+// FP32 for the mixed-precision inner solver).
+template <typename T>
+__device__ __forceinline__ T apply_A(const T *__restrict__ x, const T *__restrict__ Kx, const T *__restrict__ Ky,
+                                     long i, long pitch)
 {
-    return (1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * x[i]
+    return (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * x[i]
            - (Ky[i + pitch] * x[i + pitch] + Ky[i] * x[i - pitch])
            - (Kx[i + 1] * x[i + 1] + Kx[i] * x[i - 1]);
 }
 
 // Diagonal of the operator at i (COEF_B of tea_block_jacobi.cuknl:6).
-__device__ __forceinline__ double diag_A(const double *__restrict__ Kx, const double *__restrict__ Ky, long i,
-                                         long pitch)
+template <typename T>
+__device__ __forceinline__ T diag_A(const T *__restrict__ Kx, const T *__restrict__ Ky, long i, long pitch)
 {
-    return 1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i]);
+    return T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i]);
 }
 
-// z = M_block^-1 rhs on one strip (Thomas algorithm, tea_block_jacobi.cuknl:11-46).
-// i0 = flat index of the strip's first cell; len in 1..4.
-__device__ __forceinline__ void block_solve_strip(const double (&rhs)[STRIP], double (&z)[STRIP], int len,
-                                                  const double *__restrict__ cp, const double *__restrict__ bfp,
-                                                  const double *__restrict__ Kx, const double *__restrict__ Ky,
-                                                  long i0, long pitch)
+// z = M_block^-1 rhs on one strip of len (1..4) rows starting at flat index i0
+// (Thomas algorithm, tea_block_jacobi.cuknl:11-46), with the factors cp, bfp
+// recomputed in registers from Kx, Ky.  The factor expressions are those of
+// block_init (tea_block_jacobi.cuknl:83-111), so every value equals the stored
+// one bit for bit, and the pass reads 16 bytes per cell less.
+template <typename T>
+__device__ __forceinline__ void block_solve_strip(const T (&rhs)[STRIP], T (&z)[STRIP], int len,
+                                                  const T *__restrict__ Kx, const T *__restrict__ Ky, long i0,
+                                                  long pitch)
 {
-    double dp[STRIP];
-    dp[0] = rhs[0] / diag_A(Kx, Ky, i0, pitch);
-#pragma unroll
-    for (int k = 1; k < STRIP; k++) {
-        if (k < len) {
-            const long i = i0 + k * pitch;
-            dp[k] = (rhs[k] - (-Ky[i]) * dp[k - 1]) * bfp[i];
-        }
-    }
-#pragma unroll
-    for (int k = STRIP - 1; k >= 0; k--) {
-        if (k == len - 1) z[k] = dp[k];
-        else if (k < len - 1) z[k] = dp[k] - cp[i0 + k * pitch] * z[k + 1];
-    }
-}
-
-// The same solve with the Thomas factors recomputed from Kx, Ky in registers
-// instead of read from cp and bfp: the factor expressions are those of
-// tea_block_jacobi.cuknl:83-111, so every value is bit-identical to the stored
-// one, and the pass moves 16 bytes per cell less.
-__device__ __forceinline__ void block_solve_strip_regs(const double (&rhs)[STRIP], double (&z)[STRIP], int len,
-                                                       const double *__restrict__ Kx,
-                                                       const double *__restrict__ Ky, long i0, long pitch)
-{
-    double cp[STRIP], bfp[STRIP], dp[STRIP];
-    const double b0 = diag_A(Kx, Ky, i0, pitch);
+    T cp[STRIP], bfp[STRIP], dp[STRIP];
+    const T b0 = diag_A(Kx, Ky, i0, pitch);
     cp[0] = (-Ky[i0 + pitch]) / b0;
     dp[0] = rhs[0] / b0;
 #pragma unroll
     for (int k = 1; k < STRIP; k++) {
         if (k < len) {
             const long i = i0 + k * pitch;
-            bfp[k] = 1.0 / (diag_A(Kx, Ky, i, pitch) - (-Ky[i]) * cp[k - 1]);
+            bfp[k] = T(1.0) / (diag_A(Kx, Ky, i, pitch) - (-Ky[i]) * cp[k - 1]);
             cp[k] = (-Ky[i + pitch]) * bfp[k];
             dp[k] = (rhs[k] - (-Ky[i]) * dp[k - 1]) * bfp[k];
         }

@@ -82,6 +82,10 @@ struct Backend {
     int (*last_error)(char *, int) = nullptr;
     void (*clear_error)(void) = nullptr;
     void (*synchronize)(void) = nullptr;
+    int (*set_precision)(int) = nullptr;
+    void (*mp_setup)(void) = nullptr;
+    void (*mp_begin)(double *) = nullptr;
+    void (*mp_end)(void) = nullptr;
 
     void check() const
     {
@@ -108,6 +112,10 @@ Backend load_backend(const char *path)
     b.last_error = reinterpret_cast<int (*)(char *, int)>(dlsym(b.handle, "tea_engine_last_error"));
     b.clear_error = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_clear_error"));
     b.synchronize = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_synchronize"));
+    b.set_precision = reinterpret_cast<int (*)(int)>(dlsym(b.handle, "tea_engine_set_precision"));
+    b.mp_setup = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_setup"));
+    b.mp_begin = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_mp_begin"));
+    b.mp_end = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_end"));
     return b;
 }
 
@@ -141,6 +149,13 @@ struct Deck {
     int halo_exchange_depth = 1;
     int test_problem = 0;
     std::vector<State> states;  // 1-based in the deck, 0-based here
+    // refinement on the FP64 true residual (no reference counterpart;
+    // read_input.f90 ignores these keys)
+    int refine_bits = 0;         // tl_inner_precision fp32|fp64; 0: the reference's solve
+    bool restart = false;        // tl_mixed_refinement restart|reliable
+    double reliable_delta = 0.1; // tl_reliable_delta: residual drop that triggers a replacement
+    double inner_eps = 1.0e-4;   // tl_inner_eps: restart scheme's inner tolerance (relative)
+    int max_refine = 30;         // tl_max_refinements: restart scheme's limit
 };
 
 // One logical deck line as parse_getline returns it (parse.f90:107-236):
@@ -267,6 +282,19 @@ Deck read_deck(const std::string &path)
             else if (w == "tl_coefficient_density") d.coefficient = CONDUCTIVITY;
             else if (w == "tl_coefficient_inverse_density") d.coefficient = RECIP_CONDUCTIVITY;
             else if (w == "test_problem") d.test_problem = getival(next("test_problem"));
+            else if (w == "tl_inner_precision") {
+                const std::string &p = next("tl_inner_precision");
+                if (p == "fp32") d.refine_bits = 32;
+                else if (p == "fp64") d.refine_bits = 64;
+                else fail("read_input: tl_inner_precision must be fp32 or fp64, not %s", p.c_str());
+            } else if (w == "tl_mixed_refinement") {
+                const std::string &p = next("tl_mixed_refinement");
+                if (p == "restart") d.restart = true;
+                else if (p == "reliable") d.restart = false;
+                else fail("read_input: tl_mixed_refinement must be reliable or restart, not %s", p.c_str());
+            } else if (w == "tl_reliable_delta") d.reliable_delta = getrval(next("tl_reliable_delta"));
+            else if (w == "tl_inner_eps") d.inner_eps = getrval(next("tl_inner_eps"));
+            else if (w == "tl_max_refinements") d.max_refine = getival(next("tl_max_refinements"));
             else if (w == "state") {
                 int s = getival(next("state"));
                 if (s < 1 || s > state_max) fail("read_input: bad state number %d", s);
@@ -299,6 +327,12 @@ Deck read_deck(const std::string &path)
         float c = (float)(d.x_cells * d.y_cells);
         d.tl_ppcg_inner_steps = 4 * (int)std::sqrt(std::sqrt(c));
     }
+    if (d.refine_bits && !(d.tl_use_cg || d.tl_use_ppcg))
+        fail("read_input: tl_inner_precision needs tl_use_cg or tl_use_ppcg");
+    if (d.refine_bits && d.reflective_boundary)
+        fail("read_input: tl_inner_precision does not support reflective_boundary");
+    if (d.refine_bits && !(d.reliable_delta > 0.0 && d.reliable_delta < 1.0))
+        fail("read_input: tl_reliable_delta must lie in (0, 1)");
     if (d.halo_exchange_depth > 1 && d.tl_preconditioner_type == TL_PREC_JAC_BLOCK && d.tl_use_ppcg)
         fail("read_input: Unable to use nonstandard halo depth with block jacobi preconditioner");
 
@@ -456,6 +490,30 @@ private:
     void update_halo(const int *fields, int depth);
     void field_summary();
     void tea_leaf();
+    struct Solve {
+        int n;                  // loop index at exit (iteration count + 1)
+        double error;           // the solver's own error measure at exit
+        double initial;         // sqrt of the initial residual the loop was given
+        int ppcg_inner;         // PPCG inner steps taken
+    };
+    Solve iterate(double initial_residual_sq, double eps);
+    Solve refine(double initial_residual_sq, int *refinements, int *iterations, int *inner);
+    Solve refine_reliable(double initial_residual_sq, int *updates, int *iterations, int *inner);
+
+    // residual replacement state of refine_reliable (off in the reference's solve)
+    struct Reliable {
+        bool on = false;
+        int bits = 64;          // precision of the iteration fields
+        double target = 0.0;    // stop when ||u0 - A u||_2 < target
+        double rr_ref = 0.0;    // ||r||^2 (FP64, true) at the last replacement
+        double q_ref = 0.0;     // the solver's rrn at the last replacement
+        bool have_q = false;
+        bool fp32_live = false; // the backend is in FP32 mode with u32 pending
+        bool converged = false;
+        int updates = 0;
+        double rr_final = 0.0;
+    } rel;
+    bool reliable_step(double q, bool refresh_z);
     void run_ppcg_inner_steps(int *ppcg_inner_iters);
     void log(const char *fmt, ...);
 };
@@ -597,14 +655,7 @@ void Run::run_ppcg_inner_steps(int *ppcg_inner_iters)
 // tea_solve.f90:37-509
 void Run::tea_leaf()
 {
-    int n;
-    double old_error, error = 0.0, exact_error = 0.0, initial_residual = 0.0;
-    double rro = 0.0, pw = 0.0, rrn = 0.0, alpha, beta;
-    const int max_iters = d.max_iters;
-    std::vector<double> cg_alphas(max_iters + 2, 0.0), cg_betas(max_iters + 2, 0.0);
-    std::vector<double> ch_alphas(max_iters + 2, 0.0), ch_betas(max_iters + 2, 0.0);
-    int est_itc = 0, cheby_calc_steps = 0, max_cheby_iters = 0, ppcg_inner_iters = 0;
-    bool ch_switch_check = false;
+    double exact_error = 0.0, initial_residual = 0.0;
     int fields[NUM_FIELDS];
 
     if (d.coefficient != RECIP_CONDUCTIVITY && d.coefficient != CONDUCTIVITY)
@@ -636,6 +687,73 @@ void Run::tea_leaf()
         b.tea_leaf_calc_2norm_kernel_cuda_(&which, &initial_residual);
         b.check();
     }
+    int refinements = 0, iterations = 0, ppcg_inner_iters = 0;
+    Solve sol;
+    if (d.refine_bits && d.restart) {
+        sol = refine(initial_residual, &refinements, &iterations, &ppcg_inner_iters);
+    } else if (d.refine_bits) {
+        sol = refine_reliable(initial_residual, &refinements, &iterations, &ppcg_inner_iters);
+    } else {
+        sol = iterate(initial_residual, d.eps);
+        iterations = sol.n - 1;
+        ppcg_inner_iters = sol.ppcg_inner;
+    }
+    const double error = sol.error;
+    initial_residual = sol.initial;
+
+    if (d.tl_check_result) {
+        std::fill(fields, fields + NUM_FIELDS, 0);
+        fields[FIELD_U - 1] = 1;
+        update_halo(fields, 1);
+        b.tea_leaf_calc_residual_cuda_();
+        b.check();
+        int which = 1;
+        b.tea_leaf_calc_2norm_kernel_cuda_(&which, &exact_error);
+        b.check();
+        exact_error = std::sqrt(exact_error);
+    }
+
+    log("Conduction error %s\n", fortran_e(error / initial_residual, 14, 7).c_str());
+    if (d.tl_check_result) log("EXACT error calculated as%s\n", fortran_e(exact_error / initial_residual, 14, 7).c_str());
+    log("Iteration count %8d\n", iterations);
+    if (d.tl_use_ppcg) log("PPCG Iteration count%8d (Total %8d)\n", ppcg_inner_iters, ppcg_inner_iters + iterations);
+    if (d.refine_bits) log("FP%d refinements %8d\n", d.refine_bits, refinements);
+
+    const int k = R->n_steps;
+    if (k < TEA_MAX_STEPS) {
+        R->iterations[k] = iterations;
+        R->refinements[k] = refinements;
+        R->ppcg_inner[k] = ppcg_inner_iters;
+        R->conduction_error[k] = error / initial_residual;
+        R->exact_error[k] = d.tl_check_result ? exact_error / initial_residual : -1.0;
+        R->initial_residual[k] = initial_residual;
+    }
+    R->eigmin = eigmin;
+    R->eigmax = eigmax;
+
+    b.tea_leaf_common_finalise_kernel_cuda_();
+    b.check();
+    std::fill(fields, fields + NUM_FIELDS, 0);
+    fields[FIELD_ENERGY1 - 1] = 1;
+    update_halo(fields, 1);
+}
+
+// The solver loop of tea_solve.f90:100-419 (after the initial residual): CG
+// pre-steps, the switch to Chebyshev or PPCG, and the iterations to
+// |error| < eps * sqrt(initial_residual_sq).
+Run::Solve Run::iterate(double initial_residual_sq, double eps)
+{
+    int n;
+    double old_error, error = 0.0, initial_residual = initial_residual_sq;
+    double rro = 0.0, pw = 0.0, rrn = 0.0, alpha, beta;
+    const int max_iters = d.max_iters;
+    std::vector<double> cg_alphas(max_iters + 2, 0.0), cg_betas(max_iters + 2, 0.0);
+    std::vector<double> ch_alphas(max_iters + 2, 0.0), ch_betas(max_iters + 2, 0.0);
+    int est_itc = 0, cheby_calc_steps = 0, max_cheby_iters = 0, ppcg_inner_iters = 0;
+    bool ch_switch_check = false, converged = false;
+    int fields[NUM_FIELDS];
+    std::fill(fields, fields + NUM_FIELDS, 0);
+
     old_error = initial_residual;
     initial_residual = std::sqrt(std::fabs(initial_residual));
     if (d.verbose_on) log(" Initial residual %.16E\n", initial_residual);
@@ -746,13 +864,17 @@ void Run::tea_leaf()
                 alpha = rro / pw;
                 b.tea_leaf_cg_calc_ur_kernel_cuda_(&alpha, &rrn);
                 b.check();
-                run_ppcg_inner_steps(&ppcg_inner_iters);
-                ppcg_inner_iters += d.tl_ppcg_inner_steps;
-                b.tea_leaf_ppcg_calc_rrn_kernel_cuda_(&rrn);
-                b.check();
-                beta = rrn / rro;
-                b.tea_leaf_ppcg_calc_p_kernel_cuda_(&beta);
-                b.check();
+                if (rel.on && reliable_step(rrn, true)) {
+                    converged = true;
+                } else {
+                    run_ppcg_inner_steps(&ppcg_inner_iters);
+                    ppcg_inner_iters += d.tl_ppcg_inner_steps;
+                    b.tea_leaf_ppcg_calc_rrn_kernel_cuda_(&rrn);
+                    b.check();
+                    beta = rrn / rro;
+                    b.tea_leaf_ppcg_calc_p_kernel_cuda_(&beta);
+                    b.check();
+                }
                 error = rrn;
                 rro = rrn;
             }
@@ -769,6 +891,8 @@ void Run::tea_leaf()
             cg_betas[n] = beta;
             b.tea_leaf_cg_calc_p_kernel_cuda_(&beta);
             b.check();
+            // a replaced r reaches z at the next cg_calc_ur
+            if (rel.on && reliable_step(rrn, false)) converged = true;
             error = rrn;
             rro = rrn;
         } else if (d.tl_use_jacobi) {
@@ -777,47 +901,176 @@ void Run::tea_leaf()
         }
 
         update_halo(fields, 1);
+        if (converged) break;
 
         error = std::sqrt(std::fabs(error));
         if (d.verbose_on) log(" Residual %.16E\n", error);
-        if (std::fabs(error) < d.eps * initial_residual) break;
+        if (std::fabs(error) < eps * initial_residual) break;
         old_error = error;
     }
     // a DO loop that runs to completion leaves n = max_iters + 1
 
-    if (d.tl_check_result) {
-        std::fill(fields, fields + NUM_FIELDS, 0);
-        fields[FIELD_U - 1] = 1;
-        update_halo(fields, 1);
+    return Solve{n, error, initial_residual, ppcg_inner_iters};
+}
+
+// Mixed-precision iterative refinement (no reference counterpart): the FP64
+// residual r = u0 - A u is measured on the device; while
+// ||r|| > tl_eps * ||r0|| the correction equation A d = r is solved by the
+// same CG/PPCG loop on FP32 fields (FP64 dot products and host numerics) to
+// tl_inner_eps relative, and u += d in FP64.  The eigenvalue estimate comes
+// from the first inner solve's CG pre-steps and is kept across solves.
+Run::Solve Run::refine(double initial_residual_sq, int *refinements, int *iterations, int *inner)
+{
+    if (!b.set_precision || !b.mp_setup || !b.mp_begin || !b.mp_end)
+        fail("backend has no mixed-precision entry points (tl_inner_precision fp32)");
+    const double ir = std::sqrt(std::fabs(initial_residual_sq));
+    if (b.set_precision(32) != 0) b.check();
+    b.set_precision(64);
+    b.mp_setup();
+    b.check();
+    double rr = initial_residual_sq;
+    double err = std::sqrt(std::fabs(rr));
+    *refinements = 0;
+    *iterations = 0;
+    *inner = 0;
+    for (int k = 0; k < d.max_refine; k++) {
+        if (k > 0) {
+            b.tea_leaf_calc_residual_cuda_();
+            b.check();
+            int which = 1;
+            b.tea_leaf_calc_2norm_kernel_cuda_(&which, &rr);
+            b.check();
+            err = std::sqrt(std::fabs(rr));
+        }
+        if (d.verbose_on) log(" Refinement %d true residual %s\n", k, fortran_e(err / ir, 14, 7).c_str());
+        if (err < d.eps * ir) break;
+        double rr32 = 0.0;
+        b.set_precision(32);
+        b.mp_begin(&rr32);
+        b.check();
+        Solve s = iterate(rr32, d.inner_eps);
+        b.mp_end();
+        b.check();
+        b.set_precision(64);
+        *refinements += 1;
+        *iterations += s.n - 1;
+        *inner += s.ppcg_inner;
+    }
+    if (*refinements == d.max_refine) {
         b.tea_leaf_calc_residual_cuda_();
         b.check();
         int which = 1;
-        b.tea_leaf_calc_2norm_kernel_cuda_(&which, &exact_error);
+        b.tea_leaf_calc_2norm_kernel_cuda_(&which, &rr);
         b.check();
-        exact_error = std::sqrt(exact_error);
+        err = std::sqrt(std::fabs(rr));
     }
+    return Solve{0, err, ir, *inner};
+}
 
-    log("Conduction error %s\n", fortran_e(error / initial_residual, 14, 7).c_str());
-    if (d.tl_check_result) log("EXACT error calculated as%s\n", fortran_e(exact_error / initial_residual, 14, 7).c_str());
-    log("Iteration count %8d\n", n - 1);
-    if (d.tl_use_ppcg) log("PPCG Iteration count%8d (Total %8d)\n", ppcg_inner_iters, ppcg_inner_iters + n - 1);
-
-    const int k = R->n_steps;
-    if (k < TEA_MAX_STEPS) {
-        R->iterations[k] = n - 1;
-        R->ppcg_inner[k] = ppcg_inner_iters;
-        R->conduction_error[k] = error / initial_residual;
-        R->exact_error[k] = d.tl_check_result ? exact_error / initial_residual : -1.0;
-        R->initial_residual[k] = initial_residual;
+// Refinement with residual replacement ("reliable updates": Sleijpen and
+// van der Vorst, Computing 56 (1996) 141-163; Clark et al., Comput. Phys.
+// Commun. 181 (2010) 1517-1528), no reference counterpart.  The CG/PPCG loop
+// runs on fields of `bits` precision (FP64 scalars and dot accumulators) on
+// the correction equation A d = r, where u32 holds d since the last
+// replacement.  Whenever the loop's own residual measure has dropped by
+// tl_reliable_delta since the last replacement, or predicts that the target
+// is met, the correction is folded into the FP64 u, the FP64 residual
+// r = u0 - A u is measured, and the loop stops if ||r||_2 < tl_eps ||r0||_2;
+// otherwise r replaces the loop's residual (in PPCG z = M^-1 r is recomputed
+// at once; CG forms it at its next cg_calc_ur), keeping the search direction
+// p, so the Krylov process is not restarted.
+// At 64 bits the same loop is CG/PPCG with residual replacement and a stop
+// test on the true residual.
+Run::Solve Run::refine_reliable(double initial_residual_sq, int *updates, int *iterations, int *inner)
+{
+    const int bits = d.refine_bits;
+    if (bits == 32 && (!b.set_precision || !b.mp_setup || !b.mp_begin || !b.mp_end))
+        fail("backend has no mixed-precision entry points (tl_inner_precision fp32)");
+    const double ir = std::sqrt(std::fabs(initial_residual_sq));
+    rel = Reliable();
+    rel.on = true;
+    rel.bits = bits;
+    rel.target = d.eps * ir;
+    rel.rr_ref = initial_residual_sq;
+    rel.rr_final = initial_residual_sq;
+    double rr = initial_residual_sq;
+    Solve s{1, ir, ir, 0};
+    if (!(ir < rel.target) && ir > 0.0) {
+        if (bits == 32) {
+            if (b.set_precision(32) != 0) b.check();
+            b.mp_setup();
+            b.check();
+            b.mp_begin(&rr);
+            b.check();
+            rel.fp32_live = true;
+        }
+        s = iterate(rr, 0.0);
+        if (rel.fp32_live) {  // stopped by tl_max_iters
+            b.mp_end();
+            b.check();
+            b.set_precision(64);
+            rel.fp32_live = false;
+        }
+        if (!rel.converged) {
+            b.tea_leaf_calc_residual_cuda_();
+            b.check();
+            int which = 1;
+            b.tea_leaf_calc_2norm_kernel_cuda_(&which, &rel.rr_final);
+            b.check();
+        }
     }
-    R->eigmin = eigmin;
-    R->eigmax = eigmax;
+    rel.on = false;
+    *updates = rel.updates;
+    *iterations = s.n - 1;
+    *inner = s.ppcg_inner;
+    return Solve{s.n, std::sqrt(std::fabs(rel.rr_final)), ir, s.ppcg_inner};
+}
 
-    b.tea_leaf_common_finalise_kernel_cuda_();
+bool Run::reliable_step(double q, bool refresh_z)
+{
+    if (!rel.have_q) {  // the first measure after a replacement is the reference level
+        rel.q_ref = q;
+        rel.have_q = true;
+        return false;
+    }
+    const double drop = std::sqrt(std::fabs(q / rel.q_ref));
+    const double predicted = std::sqrt(std::fabs(rel.rr_ref)) * drop;
+    if (drop >= d.reliable_delta && predicted >= rel.target) return false;
+    if (rel.bits == 32) {
+        b.mp_end();
+        b.check();
+        b.set_precision(64);
+        rel.fp32_live = false;
+    }
+    b.tea_leaf_calc_residual_cuda_();
     b.check();
-    std::fill(fields, fields + NUM_FIELDS, 0);
-    fields[FIELD_ENERGY1 - 1] = 1;
-    update_halo(fields, 1);
+    double rr = 0.0;
+    int which = 1;
+    b.tea_leaf_calc_2norm_kernel_cuda_(&which, &rr);
+    b.check();
+    rel.updates++;
+    rel.rr_final = rr;
+    if (d.verbose_on) log(" Replacement %d true residual %.16E\n", rel.updates, std::sqrt(std::fabs(rr)));
+    if (std::sqrt(std::fabs(rr)) < rel.target) {
+        rel.converged = true;
+        return true;
+    }
+    if (rel.bits == 32) {
+        b.set_precision(32);
+        double rr32 = 0.0;
+        b.mp_begin(&rr32);
+        b.check();
+        rel.fp32_live = true;
+    }
+    if (refresh_z) {
+        int st = 2;
+        double unused = 0.0;
+        b.tea_leaf_ppcg_init_kernel_cuda_(&st, &unused);  // z = M^-1 r for the new r
+        b.check();
+    }
+    rel.rr_ref = rr;
+    rel.q_ref = q;
+    return false;
 }
 
 // diffuse.f90:23-163

@@ -25,7 +25,7 @@ typedef struct {
     double exact_error[TEA_MAX_STEPS];            /* ||u0 - A u|| / ||r0|| (tl_check_result) */
     double initial_residual[TEA_MAX_STEPS];       /* ||r0|| */
     double step_seconds[TEA_MAX_STEPS];           /* wall time of the step */
-    double outer_seconds[TEA_MAX_STEPS];          /* mixed precision: outer refinement count (as double) */
+    int refinements[TEA_MAX_STEPS];               /* mixed precision: FP32 inner solves per step */
     int n_summaries;
     int summary_step[TEA_MAX_SUMMARIES];
     double summary[TEA_MAX_SUMMARIES][6];         /* time, volume, mass, density, energy, u */

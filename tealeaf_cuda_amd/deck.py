@@ -124,6 +124,11 @@ class Deck:
     profiler_on: bool = False
     inverse_density: bool = False
     test_problem: int | None = None
+    inner_precision: str | None = None  # "fp32" | "fp64": refinement on the FP64 true residual
+    mixed_refinement: str | None = None  # "reliable" (default) | "restart"
+    reliable_delta: float | None = None
+    inner_eps: float | None = None
+    max_refinements: int | None = None
     states: list = field(default_factory=standard_states)
     extra: list = field(default_factory=list)
 
@@ -163,6 +168,16 @@ class Deck:
             lines.append("tl_coefficient_inverse_density")
         if self.test_problem is not None:
             lines.append(f"test_problem {self.test_problem}")
+        if self.inner_precision is not None:
+            lines.append(f"tl_inner_precision={self.inner_precision}")
+        if self.mixed_refinement is not None:
+            lines.append(f"tl_mixed_refinement={self.mixed_refinement}")
+        if self.reliable_delta is not None:
+            lines.append(f"tl_reliable_delta={_f(self.reliable_delta)}")
+        if self.inner_eps is not None:
+            lines.append(f"tl_inner_eps={_f(self.inner_eps)}")
+        if self.max_refinements is not None:
+            lines.append(f"tl_max_refinements={self.max_refinements}")
         lines += list(self.extra)
         lines.append("*endtea")
         return "\n".join(lines) + "\n"

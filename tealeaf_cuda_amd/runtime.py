@@ -103,6 +103,12 @@ _EXT = {
     "tea_engine_stream": ([], ctypes.c_void_p),
     "tea_engine_synchronize": ([], None),
     "tea_engine_finalize": ([], None),
+    "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
+    "tea_engine_mp_setup": ([], None),
+    "tea_engine_mp_begin": ([_D], None),
+    "tea_engine_mp_end": ([], None),
+    "tea_engine_get_field32": ([ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "tea_engine_set_field32": ([ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
 }
 
 ABI_SYMBOLS = tuple(_ABI)
@@ -312,6 +318,34 @@ class Engine:
             out[name] = (c.value, ms.value)
         return out
 
+    # -- mixed precision
+    def set_precision(self, bits: int):
+        if self.lib.tea_engine_set_precision(int(bits)) != 0:
+            self._check()
+            raise TeaError(f"set_precision({bits}) failed")
+
+    def mp_setup(self):
+        self._call("tea_engine_mp_setup")
+
+    def mp_begin(self) -> float:
+        out = ctypes.c_double()
+        self._call("tea_engine_mp_begin", ctypes.byref(out))
+        return out.value
+
+    def mp_end(self):
+        self._call("tea_engine_mp_end")
+
+    def get_field32(self, field_id) -> np.ndarray:
+        a = np.empty(self.shape(), dtype=np.float32)
+        if self.lib.tea_engine_get_field32(int(field_id), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
+            raise TeaError(f"get_field32({field_id}) failed")
+        return a
+
+    def set_field32(self, field_id, a) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if self.lib.tea_engine_set_field32(int(field_id), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
+            raise TeaError(f"set_field32({field_id}) failed")
+
     def synchronize(self):
         self.lib.tea_engine_synchronize()
 
@@ -330,7 +364,7 @@ class _RunResult(ctypes.Structure):
         ("exact_error", ctypes.c_double * TEA_MAX_STEPS),
         ("initial_residual", ctypes.c_double * TEA_MAX_STEPS),
         ("step_seconds", ctypes.c_double * TEA_MAX_STEPS),
-        ("outer_seconds", ctypes.c_double * TEA_MAX_STEPS),
+        ("refinements", ctypes.c_int * TEA_MAX_STEPS),
         ("n_summaries", ctypes.c_int),
         ("summary_step", ctypes.c_int * TEA_MAX_SUMMARIES),
         ("summary", (ctypes.c_double * 6) * TEA_MAX_SUMMARIES),
@@ -364,6 +398,7 @@ class RunResult:
     exact_error: list
     initial_residual: list
     step_seconds: list
+    refinements: list = field(default_factory=list)
     summaries: list = field(default_factory=list)
     eigmin: float = 0.0
     eigmax: float = 0.0
@@ -414,6 +449,7 @@ def run_deck(deck_path, backend: str | os.PathLike | None = None, out_path=None,
         exact_error=list(res.exact_error[:n]),
         initial_residual=list(res.initial_residual[:n]),
         step_seconds=list(res.step_seconds[:n]),
+        refinements=list(res.refinements[:n]),
         eigmin=res.eigmin,
         eigmax=res.eigmax,
         total_seconds=res.total_seconds,

@@ -49,6 +49,22 @@ DECKS = {
     "random_layout_ppcg": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
                                states=random_layout(11, cells=128), x_cells=128, y_cells=128),
     "tiny_3x5": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=4, x_cells=3, y_cells=5),
+    "mixed_ppcg_block": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+                             inner_precision="fp32"),
+    "mixed_ppcg_diag_h2": dict(solver="ppcg", preconditioner="jac_diag", ppcg_inner_steps=9, halo_depth=2,
+                               inner_precision="fp32", reliable_delta=0.01),
+    "mixed_ppcg_none_ragged_h4": dict(solver="ppcg", preconditioner="none", ppcg_inner_steps=10, halo_depth=4,
+                                      x_cells=97, y_cells=66, inner_precision="fp32"),
+    "mixed_cg_block": dict(solver="cg", preconditioner="jac_block", inner_precision="fp32", reliable_delta=0.05),
+    "mixed_restart_ppcg_block": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+                                     inner_precision="fp32", mixed_refinement="restart", inner_eps=1e-5),
+    "mixed_restart_cg_diag": dict(solver="cg", preconditioner="jac_diag", inner_precision="fp32",
+                                  mixed_refinement="restart", inner_eps=1e-4),
+    "refined_fp64_ppcg_block": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+                                    inner_precision="fp64"),
+    "mixed_random_layout": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+                                states=random_layout(12, cells=128), x_cells=128, y_cells=128,
+                                inner_precision="fp32"),
 }
 
 
@@ -68,6 +84,7 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
     cpu = run_deck(path, backend=oracle_so)
     assert gpu.iterations == cpu.iterations
     assert gpu.ppcg_inner == cpu.ppcg_inner
+    assert gpu.refinements == cpu.refinements
     assert gpu.conduction_error == cpu.conduction_error
     assert gpu.exact_error == cpu.exact_error
     assert len(gpu.summaries) == len(cpu.summaries)
@@ -179,6 +196,68 @@ def test_halo_entry_points_bitwise(depth, oracle_so, oracle_env, tmp_path):
             for e in (g, c):
                 e.unpack(fields, off, depth, face, rnd)
             _compare(g, c, f"unpack face {face}")
+    finally:
+        g.finalize()
+        c.finalize()
+
+
+# ------------------------------------------------------------------ FP32 mode
+
+FP32_FIELDS = (rt.FIELD_U, rt.FIELD_P, rt.FIELD_SD, rt.FIELD_R, rt.XFIELD_U0, rt.XFIELD_W, rt.XFIELD_Z,
+               rt.XFIELD_RTEMP, rt.XFIELD_UTEMP, rt.XFIELD_RSTORE)
+
+
+def _randomise32(engines, seed):
+    rng = np.random.default_rng(seed)
+    shape = engines[0].shape()
+    for fid in FP32_FIELDS:
+        a = rng.standard_normal(shape).astype(np.float32)
+        for e in engines:
+            e.set_field32(fid, a)
+
+
+@pytest.mark.parametrize("prec", ["none", "jac_diag", "jac_block"])
+def test_fp32_entry_points_bitwise(prec, oracle_so, oracle_env, tmp_path):
+    """FP32 mode: every CG/PPCG entry point on random float fields, then
+    mp_begin / mp_end, compared bit for bit (FP32 fields and FP64 results)."""
+    oracle_env("dot2")
+    g, c = Engine(), Engine(oracle_so)
+    calls = dict(CALLS)
+    calls.update(PPCG_CALLS)
+    for k in ("calc_residual", "calc_2norm_0", "calc_2norm_1", "calc_2norm_2", "finalise", "field_summary"):
+        del calls[k]
+    try:
+        for k, (name, fn) in enumerate(sorted(calls.items())):
+            print(f"[gpu parity] fp32 entry point {prec}/{name}", flush=True)
+            for e in (g, c):
+                _prepare(e, tmp_path, "ppcg", prec)
+                e.set_precision(32)
+                e.mp_setup()
+            _randomise32((g, c), seed=k)
+            for e in (g, c):
+                e.cg_init()
+            _randomise32((g, c), seed=200 + k)
+            out_g, out_c = fn(g), fn(c)
+            assert out_g == out_c, name
+            for fid in range(1, 20):
+                if fid in (rt.XFIELD_CP, rt.XFIELD_BFP):
+                    continue  # FP32 mode keeps no Thomas factors: the solves recompute them in registers
+                assert np.array_equal(g.get_field32(fid), c.get_field32(fid), equal_nan=True), f"{name}: {fid}"
+        print("[gpu parity] fp32 mp_begin/mp_end", flush=True)
+        rng = np.random.default_rng(7)
+        r, u = rng.standard_normal(g.shape()), rng.standard_normal(g.shape())
+        outs = []
+        for e in (g, c):
+            e.set_precision(64)
+            e.set_field(rt.FIELD_R, r)
+            e.set_field(rt.FIELD_U, u)
+            e.set_precision(32)
+            outs.append(e.mp_begin())
+            e.set_field32(rt.FIELD_U, r.astype(np.float32) * np.float32(0.5))
+            e.mp_end()
+        assert outs[0] == outs[1]
+        _compare(g, c, "mp_end")
+        assert np.array_equal(g.get_field32(rt.FIELD_R), c.get_field32(rt.FIELD_R))
     finally:
         g.finalize()
         c.finalize()

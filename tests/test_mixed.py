@@ -1,0 +1,224 @@
+"""Mixed-precision refinement (tl_inner_precision=fp32) on the oracle.
+
+The refinement loop has no counterpart in the reference; what pins it:
+
+* the FP32 instantiation of every CG/PPCG kernel equals the FP64 one exactly
+  on inputs where float arithmetic is exact (small integers and dyadic
+  scalars), so the FP32 kernels compute the same expressions as the FP64
+  kernels that the reference's golden files pin; with the jac_diag / jac_block
+  preconditioners (divisions) they agree to float rounding;
+* mp_begin / mp_end are the roundings r32 = fl32(r) and u += u32 exactly;
+* both refinement schemes (restart: FP32 inner solves to tl_inner_eps;
+  reliable: residual replacement keeping the search direction) reach the FP64
+  true relative residual tl_eps on every step
+  (tl_check_result measures ||u0 - A u|| / ||r0|| in FP64 after the solve), and
+  its field summaries agree with a tightly converged FP64 run to the accuracy
+  that residual implies;
+* decks the refinement does not support are rejected with an error.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tealeaf_cuda_amd import Deck, Engine, StateSpec, TeaError, run_deck
+from tealeaf_cuda_amd import runtime as rt
+
+from test_oracle_math import interior, put_interior, setup_chunk
+
+LAYOUT = [StateSpec(100.0, 0.0001), StateSpec(0.1, 25.0, "rectangle", 0.0, 1.0, 1.0, 2.0),
+          StateSpec(0.1, 0.1, "rectangle", 1.0, 6.0, 1.0, 2.0), StateSpec(0.1, 0.1, "rectangle", 5.0, 6.0, 1.0, 8.0),
+          StateSpec(0.1, 0.1, "rectangle", 5.0, 10.0, 7.0, 8.0)]
+
+SOLVE_FIELDS = (rt.FIELD_U, rt.FIELD_P, rt.FIELD_SD, rt.FIELD_R, rt.XFIELD_U0, rt.XFIELD_W, rt.XFIELD_Z,
+                rt.XFIELD_RTEMP, rt.XFIELD_UTEMP, rt.XFIELD_RSTORE)
+
+
+def _integer_chunk(eng, tmp_path, solver, prec, seed, nx=13, ny=9):
+    """A chunk whose Kx/Ky and solver fields hold small integers (exact in float)."""
+    setup_chunk(eng, tmp_path, nx=nx, ny=ny, solver=solver, prec=prec, halo=2)
+    rng = np.random.default_rng(seed)
+    shape = eng.shape()
+    for fid in (rt.XFIELD_KX, rt.XFIELD_KY):
+        k = rng.integers(1, 5, size=shape).astype(np.float64)
+        eng.set_field(fid, k)
+    eng.set_precision(32)
+    eng.mp_setup()
+    vals = {}
+    for fid in SOLVE_FIELDS:
+        a = np.zeros(shape)
+        h = eng.h
+        a[h:h + ny, h:h + nx] = rng.integers(-8, 9, size=(ny, nx))
+        vals[fid] = a
+    return vals
+
+
+def _load(eng, vals, bits):
+    eng.set_precision(bits)
+    for fid, a in vals.items():
+        if bits == 64:
+            eng.set_field(fid, a)
+        else:
+            eng.set_field32(fid, a.astype(np.float32))
+
+
+def _read(eng, fid, bits):
+    return interior(eng, eng.get_field(fid) if bits == 64 else eng.get_field32(fid).astype(np.float64))
+
+
+EXACT_CALLS = {
+    "cg_init": lambda e: e.cg_init(),
+    "cg_calc_w": lambda e: e.cg_calc_w(),
+    "cg_calc_ur": lambda e: e.cg_calc_ur(0.5),
+    "cg_calc_p": lambda e: e.cg_calc_p(0.25),
+    "ppcg_init_1": lambda e: e.ppcg_init(1),
+    "ppcg_init_3": lambda e: e.ppcg_init(3),
+    "ppcg_init_sd_new": lambda e: e.ppcg_init_sd_new(2.0),
+    "ppcg_inner": lambda e: (e.ppcg_init_constants([0.5, 0.25, 0.125], [0.75, 0.5, 0.25]), e.ppcg_inner(2)),
+    "ppcg_store_r": lambda e: e.ppcg_store_r(),
+    "ppcg_calc_rrn": lambda e: e.ppcg_calc_rrn(),
+    "ppcg_calc_2norm": lambda e: e.ppcg_calc_2norm(),
+    "ppcg_calc_p": lambda e: e.ppcg_calc_p(0.5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EXACT_CALLS))
+def test_fp32_kernels_equal_fp64_on_exact_inputs(name, oracle_so, oracle_env, tmp_path):
+    oracle_env("dot2")
+    eng = Engine(oracle_so)
+    try:
+        vals = _integer_chunk(eng, tmp_path, "ppcg", "none", seed=len(name))
+        outs = {}
+        fields = {}
+        for bits in (64, 32):
+            _load(eng, vals, bits)
+            outs[bits] = EXACT_CALLS[name](eng)
+            fields[bits] = {fid: _read(eng, fid, bits) for fid in SOLVE_FIELDS}
+        assert outs[32] == outs[64]
+        for fid in SOLVE_FIELDS:
+            assert np.array_equal(fields[32][fid], fields[64][fid]), f"{name}: field {fid}"
+    finally:
+        eng.finalize()
+
+
+@pytest.mark.parametrize("prec", ["jac_diag", "jac_block"])
+def test_fp32_preconditioners_agree_to_float_rounding(prec, oracle_so, oracle_env, tmp_path):
+    oracle_env("dot2")
+    eng = Engine(oracle_so)
+    try:
+        vals = _integer_chunk(eng, tmp_path, "ppcg", prec, seed=5)
+        z = {}
+        for bits in (64, 32):
+            _load(eng, vals, bits)
+            rro = eng.cg_init()
+            z[bits] = (_read(eng, rt.XFIELD_Z, bits), rro)
+        np.testing.assert_allclose(z[32][0], z[64][0], rtol=2e-6, atol=2e-6 * np.abs(z[64][0]).max())
+        assert z[32][1] == pytest.approx(z[64][1], rel=1e-5)
+    finally:
+        eng.finalize()
+
+
+def test_mp_begin_end_are_roundings(oracle_so, oracle_env, tmp_path):
+    oracle_env("dot2")
+    eng = Engine(oracle_so)
+    try:
+        setup_chunk(eng, tmp_path, nx=11, ny=7, solver="cg")
+        rng = np.random.default_rng(3)
+        r = rng.standard_normal(eng.shape())
+        u = rng.standard_normal(eng.shape())
+        eng.set_field(rt.FIELD_R, r)
+        eng.set_field(rt.FIELD_U, u)
+        eng.set_precision(32)
+        rr = eng.mp_begin()
+        r32 = interior(eng, eng.get_field32(rt.FIELD_R))
+        assert np.array_equal(r32, interior(eng, r).astype(np.float32))
+        assert not interior(eng, eng.get_field32(rt.FIELD_U)).any()
+        x = r32.astype(np.float64).ravel()
+        assert rr == pytest.approx(float(np.dot(x, x)), rel=1e-15)
+        d = rng.standard_normal(eng.shape()).astype(np.float32)
+        eng.set_field32(rt.FIELD_U, d)
+        eng.mp_end()
+        assert np.array_equal(interior(eng, eng.get_field(rt.FIELD_U)),
+                              interior(eng, u) + interior(eng, d).astype(np.float64))
+    finally:
+        eng.finalize()
+
+
+def _run(tmp_path, oracle_so, name, **kw):
+    base = dict(x_cells=96, y_cells=80, end_step=3, eps=1e-10, check_result=True, summary_frequency=1,
+                states=LAYOUT)
+    base.update(kw)
+    return run_deck(Deck(**base).write(tmp_path / f"{name}.in"), backend=oracle_so,
+                    out_path=tmp_path / f"{name}.out")
+
+
+MIXED_CASES = [
+    ("ppcg", "jac_block", dict(mixed_refinement="restart", inner_eps=1e-6)),
+    ("ppcg", "jac_diag", dict(mixed_refinement="restart", inner_eps=1e-4)),
+    ("cg", "jac_block", dict(mixed_refinement="restart", inner_eps=1e-5)),
+    ("cg", "none", dict(mixed_refinement="restart", inner_eps=1e-3)),
+    ("ppcg", "jac_block", dict()),
+    ("ppcg", "none", dict(reliable_delta=0.01)),
+    ("cg", "jac_diag", dict(reliable_delta=0.3)),
+    ("cg", "jac_block", dict(mixed_refinement="reliable")),
+]
+
+
+@pytest.mark.parametrize("solver,prec,kw", MIXED_CASES)
+def test_mixed_reaches_fp64_true_residual(solver, prec, kw, oracle_so, oracle_env, tmp_path):
+    oracle_env("dot2")
+    inner = dict(ppcg_inner_steps=10) if solver == "ppcg" else {}
+    mixed = _run(tmp_path, oracle_so, "mixed", solver=solver, preconditioner=prec, inner_precision="fp32",
+                 **kw, **inner)
+    ref = _run(tmp_path, oracle_so, "fp64", solver=solver, preconditioner=prec, eps=1e-14, **inner)
+    assert all(k >= 2 for k in mixed.refinements)
+    for k, e in enumerate(mixed.exact_error):
+        assert 0 < e < 1e-10, (k, e)
+        assert mixed.conduction_error[k] == e          # the refinement loop stops on the true residual
+    assert len(mixed.summaries) == len(ref.summaries)
+    for a, b in zip(mixed.summaries, ref.summaries):
+        np.testing.assert_allclose(a.values(), b.values(), rtol=1e-9)
+    text = (tmp_path / "mixed.out").read_text()
+    assert "FP32 refinements" in text
+
+
+def test_reliable_keeps_the_krylov_process(oracle_so, oracle_env, tmp_path):
+    """Residual replacement keeps p, so the FP32 loop needs about the
+    iterations of the FP64 loop with the same true-residual stop (the restart
+    scheme needs more), and both reach the target."""
+    oracle_env("dot2")
+    base = dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10, x_cells=128, y_cells=128)
+    r32 = _run(tmp_path, oracle_so, "r32", inner_precision="fp32", **base)
+    r64 = _run(tmp_path, oracle_so, "r64", inner_precision="fp64", **base)
+    rst = _run(tmp_path, oracle_so, "rst", inner_precision="fp32", mixed_refinement="restart", inner_eps=1e-4, **base)
+    for a, b in zip(r32.iterations, r64.iterations):
+        assert abs(a - b) <= max(2, 0.1 * b)
+    assert sum(rst.iterations) > sum(r32.iterations)
+    for r in (r32, r64, rst):
+        assert max(r.exact_error) < 1e-10
+    assert all(k >= 1 for k in r64.refinements)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(solver="chebyshev", preconditioner="none"), "tl_inner_precision"),
+    (dict(solver="jacobi", preconditioner="none"), "tl_inner_precision"),
+    (dict(solver="ppcg", preconditioner="none", reflective_boundary=True), "reflective"),
+])
+def test_mixed_rejects_unsupported_decks(kw, msg, oracle_so, tmp_path):
+    with pytest.raises(TeaError, match=msg):
+        _run(tmp_path, oracle_so, "bad", inner_precision="fp32", end_step=1, **kw)
+
+
+def test_fp32_mode_rejects_chebyshev_and_jacobi_calls(oracle_so, tmp_path):
+    eng = Engine(oracle_so)
+    try:
+        setup_chunk(eng, tmp_path, solver="chebyshev")
+        eng.set_precision(32)
+        with pytest.raises(TeaError):
+            eng.cheby_init([0.2], [0.9], 1.3)
+        with pytest.raises(TeaError):
+            eng.jacobi_solve()
+        with pytest.raises(TeaError):
+            eng.set_precision(16)
+    finally:
+        eng.finalize()

@@ -1,21 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark: ms per timestep to an FP64 relative residual of 1e-10 at 4096^2.
 
-Workload: the reference's 4-rectangle problem (tea.in:2-6 geometry on [0,10]^2)
-on a 4096 x 4096 mesh, plain FP64 PPCG with 10 inner steps and the block-Jacobi
-preconditioner, tl_eps 1e-10, dt 0.004, one chunk on one GPU.  A step is one
-timestep: solver set-up, the CG pre-steps / PPCG iterations to tl_eps, and the
-finalisation, all through the host driver over libtea_engine.so.
+Workload: a seeded layout of random axis-aligned rectangles (density
+log-uniform in [0.1, 100], energy log-uniform in [1e-4, 25], on a density-100 /
+energy-1e-4 background) on a 4096 x 4096 mesh of [0,10]^2, PPCG with 10 inner
+steps and the block-Jacobi preconditioner, dt 0.004, one chunk on one GPU.  A
+step is one timestep: solver set-up, the CG pre-steps / PPCG iterations, and
+the finalisation, all through the host driver over libtea_engine.so.
 
-W warm-up steps run first (the first holds the CG eigenvalue estimate), then
-exactly K timed steps.  The timed region is bracketed by device
-synchronisation; HIP events recorded on the engine's own stream give the GPU
-time of the region, and the engine's per-launch event profiler gives each
-kernel's average duration for the roofline figure.
+Three solves of the same deck are timed, each W warm-up steps (the first holds
+the CG eigenvalue estimate) then exactly K timed steps:
 
-The CPU baseline runs the same deck through the same driver over the CPU
+* headline: FP32 fields with FP64 dot accumulators, FP64 residual replacement,
+  stopping when the FP64 true residual ||u0 - A u||_2 < 1e-10 ||r0||_2;
+* fp64_true: the same loop on FP64 fields, the same stop;
+* fp64_reference: plain FP64 PPCG stopping on the reference's own recurrence
+  test at tl_eps 1e-10 (its true residual is reported beside it).
+
+The timed region is bracketed by device synchronisation; HIP events recorded
+on the engine's own stream give the GPU time of the region, and the engine's
+per-launch event profiler gives each kernel's average duration for the
+roofline figure.
+
+The CPU baseline runs the plain FP64 deck through the same driver over the CPU
 oracle (OpenMP, plain reductions) on a bounded sample of PPCG iterations and
-extrapolates to the GPU run's iteration counts.
+extrapolates to the GPU plain run's iteration counts.
 """
 from __future__ import annotations
 
@@ -38,6 +47,7 @@ HBM_PEAK_GBS = 8000.0
 ALGO_BYTES = {
     # one whole inner step (the reference's update_r + calc_sd_new launches, jac_block)
     "k_ppcg_inner_fused": 112,
+    "k_ppcg_inner_fused_f32": 56,     # the same arrays in FP32
     "ppcg_update_r": 40,              # rtemp, sd, Kx, Ky read; rtemp written
     "k_ppcg_calc_sd": 72,             # jac_block: rtemp, sd, utemp, cp, bfp, Kx, Ky read; sd, utemp written
     "cg_calc_w": 32,                  # p, Kx, Ky read; w written
@@ -51,21 +61,29 @@ ALGO_BYTES = {
 }
 
 
+LAYOUT_SEED = 4096
+VARIANTS = {
+    "headline": dict(inner_precision="fp32"),
+    "fp64_true": dict(inner_precision="fp64"),
+    "fp64_reference": dict(),
+}
+
+
 def make_deck(path, n, steps, solver="ppcg", prec="jac_block", eps=1e-10, max_iters=10000, inner=10,
-              epslim=None):
-    from tealeaf_cuda_amd import Deck
+              epslim=None, **kw):
+    from tealeaf_cuda_amd import Deck, random_layout
 
     d = Deck(x_cells=n, y_cells=n, end_step=steps, solver=solver, preconditioner=prec, eps=eps,
              max_iters=max_iters, halo_depth=1, ppcg_inner_steps=inner, summary_frequency=0,
-             check_result=True, ch_cg_epslim=epslim)
+             check_result=True, ch_cg_epslim=epslim, states=random_layout(LAYOUT_SEED, cells=n), **kw)
     return d.write(path)
 
 
-def run_gpu(n, warmup, steps, tmp):
+def run_gpu(n, warmup, steps, tmp, variant):
     import torch
     import tealeaf_cuda_amd as tl
 
-    deck = make_deck(Path(tmp) / "tea.in", n, warmup + steps)
+    deck = make_deck(Path(tmp) / f"tea_{variant}.in", n, warmup + steps, **VARIANTS[variant])
     eng = tl.Engine()  # same library instance the driver loads (RTLD_LOCAL handles are shared per path)
     ev = {}
 
@@ -90,6 +108,8 @@ def run_gpu(n, warmup, steps, tmp):
     gpu_ms = ev["start"].elapsed_time(ev["stop"])
     wall_ms = (ev["t1"] - ev["t0"]) * 1e3
     timed = slice(warmup, warmup + steps)
+    print(f"[bench] {variant}: {gpu_ms / steps:.1f} ms/step, iterations {res.iterations[timed]}, "
+          f"true residual {res.exact_error[timed]}", file=sys.stderr, flush=True)
     return res, gpu_ms, wall_ms, ev["profile"], timed
 
 
@@ -153,6 +173,7 @@ def cpu_baseline(n, gpu_res, timed, tmp):
     times = {}
     for iters in (lo, hi):
         deck = make_deck(Path(tmp) / f"cpu_{iters}.in", n, 1, max_iters=iters, epslim=1e30)
+        print(f"[bench] cpu baseline sample: {iters} iterations", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         tl.run_deck(deck, backend=oracle)
         times[iters] = time.perf_counter() - t0
@@ -185,11 +206,25 @@ def main():
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     n = args.size
+    runs = {}
     with tempfile.TemporaryDirectory() as tmp:
-        res, gpu_ms, wall_ms, profile, timed = run_gpu(n, args.warmup, args.steps, tmp)
+        for variant in VARIANTS:
+            runs[variant] = run_gpu(n, args.warmup, args.steps, tmp, variant)
+        res, gpu_ms, wall_ms, profile, timed = runs["headline"]
         ms_per_step = gpu_ms / args.steps
         dominant, roof = roofline(profile, n)
-        cpu = None if args.no_cpu_baseline else cpu_baseline(n, res, timed, tmp)
+        _, plain_roof = roofline(runs["fp64_reference"][3], n)
+        cpu = None if args.no_cpu_baseline else cpu_baseline(n, runs["fp64_reference"][0], timed, tmp)
+
+    def summary(variant):
+        r, g_ms, w_ms, _, t = runs[variant]
+        return {"ms_per_step": round(g_ms / args.steps, 3), "wall_ms_per_step": round(w_ms / args.steps, 3),
+                "iterations": r.iterations[t], "ppcg_inner": r.ppcg_inner[t],
+                "residual_replacements": r.refinements[t], "exact_rel_residual": r.exact_error[t]}
+
+    comparison = {v: summary(v) for v in VARIANTS}
+    for v in ("fp64_true", "fp64_reference"):
+        comparison[f"speedup_vs_{v}"] = round(comparison[v]["ms_per_step"] / comparison["headline"]["ms_per_step"], 3)
     line = {
         "metric": METRIC,
         "value": round(ms_per_step, 3),
@@ -201,21 +236,25 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 fields, f64 accumulators and residual",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FP64 PPCG, jac_block, 10 inner steps, {n}x{n} cells, 4-rectangle layout, "
-                        "tl_eps 1e-10, dt 0.004, halo_depth 1",
+            "workload": f"PPCG, jac_block, 10 inner steps, FP32 fields with FP64 residual replacement, "
+                        f"{n}x{n} cells, random-rectangle layout seed {LAYOUT_SEED}, "
+                        "stop at FP64 true relative residual 1e-10, dt 0.004, halo_depth 1",
             "cells": n * n,
             "solver": "ppcg",
             "preconditioner": "jac_block",
             "tl_eps": 1e-10,
+            "layout_seed": LAYOUT_SEED,
         },
         "wall_ms_per_step": round(wall_ms / args.steps, 3),
         "iterations": res.iterations[timed],
         "ppcg_inner": res.ppcg_inner[timed],
         "exact_rel_residual": res.exact_error[timed],
+        "comparison": comparison,
         "roofline": roof,
+        "roofline_fp64_reference": plain_roof,
         "dominant_kernel": dominant,
         "cpu_baseline": cpu,
         "kernels": {k: {"launches": c, "total_ms": round(ms, 3)} for k, (c, ms) in sorted(profile.items())},

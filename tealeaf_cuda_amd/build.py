@@ -1,7 +1,8 @@
 """Build recipes for the native libraries of the package.
 
-* ``libtea_engine.so`` -- the HIP kernel layer (C-ABI of include/tea_leaf_engine.h),
-  compiled for gfx950 with hipcc;
+* ``libtea_engine.so`` -- the HIP kernel layer (C-ABI of include/tea_leaf_engine.h)
+  and the warm-start network and trainer (include/tea_warmstart.h), compiled for
+  gfx950 with hipcc;
 * ``libtea_driver.so`` -- the host driver (deck reader, time loop, solver control).
 
 Both are built in-tree, next to this file, so they travel with the source tree.
@@ -49,7 +50,7 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
 
 
 def build_engine() -> Path:
-    sources = [CSRC / "engine.hip"]
+    sources = [CSRC / "engine.hip", CSRC / "warmstart.hip"]
     tmp = ENGINE_SO.with_suffix(".so.tmp")
     _run([HIPCC, *HIP_FLAGS, "-o", str(tmp), *map(str, sources)])
     os.replace(tmp, ENGINE_SO)

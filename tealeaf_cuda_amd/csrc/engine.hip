@@ -30,6 +30,7 @@
 #include "../../include/tea_leaf_engine.h"
 #include "engine_device.hpp"
 #include "engine_solver.hpp"
+#include "ws_net.hpp"
 
 using namespace tea;
 
@@ -145,6 +146,17 @@ struct Chunk {
     // across before the next fused step.
     bool sd_halo_dirty = false;
     bool sd32_halo_dirty = false;  // the same for the FP32 shadow (set_field32)
+
+    // learned warm start: network, inference buffers, u0 of the two previous
+    // steps (field layout: the same flat index as the fields)
+    bool ws_on = false;
+    bool ws_use_net = false;
+    ws::Net ws_net;
+    ws::Acts ws_acts;
+    float *ws_planes = nullptr;   // 3 x ny x nx features
+    double *ws_slab = nullptr;
+    double *ws_up1 = nullptr, *ws_up2 = nullptr;
+    int ws_nprev = 0;             // previous u0 fields held (0..2)
 
     Profiler prof;
 };
@@ -550,6 +562,128 @@ __global__ __launch_bounds__(BLOCK) void k_finalise(Ext e, long pitch, const dou
     STRIP_END
 }
 
+// ---- learned warm start (no reference counterpart) ---------------------------------
+
+// sums of (u0 - u_prev)^2 and u0^2 over the interior
+__global__ __launch_bounds__(BLOCK) void k_ws_stats(Ext e, long pitch, const double *__restrict__ u0,
+                                                    const double *__restrict__ up, Red red)
+{
+    DD acc[2] = {{0, 0}, {0, 0}};
+    STRIP_BEGIN
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const double d = u0[i] - up[i];
+        dd_add_prod(acc[0], d, d);
+        dd_add_prod(acc[1], u0[i], u0[i]);
+    }
+    STRIP_END
+    block_reduce<2>(acc, red);
+}
+
+// network features on dense planes [3][ny][nx]
+__global__ __launch_bounds__(BLOCK) void k_ws_features(Ext e, long pitch, int h, int nx, long n,
+                                                       const double *__restrict__ dens, const double *__restrict__ u0,
+                                                       const double *__restrict__ up, double inv_s, double inv_su,
+                                                       float *__restrict__ planes)
+{
+    STRIP_BEGIN
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const long o = (long)(s.r0 + k - h) * nx + (s.c - h);
+        planes[o] = (float)((log10(dens[i]) - 0.5) / 1.5);
+        planes[n + o] = (float)((u0[i] - up[i]) * inv_s);
+        planes[2 * n + o] = (float)(u0[i] * inv_su);
+    }
+    STRIP_END
+}
+
+// Search directions of the warm start at interior cell i:
+//   v0 = u0 - up1, v1 = up1 - up2 (when nv > 1 previous steps), v2 = scale * net
+// (when net != nullptr); the network plane is dense, its cells outside the
+// interior count as 0 (their faces carry K = 0 under the zero-flux boundary).
+struct WsDirs {
+    const double *u0, *up1, *up2;
+    const float *net;
+    double scale;
+    int h, nx, ny;
+};
+
+__device__ __forceinline__ double ws_net_at(const WsDirs &d, int c, int r)
+{
+    if (c < d.h || c >= d.h + d.nx || r < d.h || r >= d.h + d.ny) return 0.0;
+    return d.scale * (double)d.net[(long)(r - d.h) * d.nx + (c - d.h)];
+}
+
+// Gram matrix and right-hand side of the energy-minimising combination:
+// G[a][b] = v_a . A v_b and g[a] = v_a . r0 with r0 = u0 - A u0 (the residual
+// of the u = u^n guess), packed as G00 G01 G02 G11 G12 G22 g0 g1 g2.
+__global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
+                                                   const double *__restrict__ Ky, int two, Red red)
+{
+    DD acc[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) acc[j] = DD{0.0, 0.0};
+    STRIP_BEGIN
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const int c = s.c, r = s.r0 + k;
+        const double a_u0 = apply_A(d.u0, Kx, Ky, i, pitch);
+        const double a_p1 = apply_A(d.up1, Kx, Ky, i, pitch);
+        double v[3], av[3];
+        v[0] = d.u0[i] - d.up1[i];
+        av[0] = a_u0 - a_p1;
+        v[1] = 0.0;
+        av[1] = 0.0;
+        if (two) {
+            v[1] = d.up1[i] - d.up2[i];
+            av[1] = a_p1 - apply_A(d.up2, Kx, Ky, i, pitch);
+        }
+        v[2] = 0.0;
+        av[2] = 0.0;
+        if (d.net) {
+            const double q = ws_net_at(d, c, r);
+            v[2] = q;
+            av[2] = (1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * q -
+                    (Ky[i + pitch] * ws_net_at(d, c, r + 1) + Ky[i] * ws_net_at(d, c, r - 1)) -
+                    (Kx[i + 1] * ws_net_at(d, c + 1, r) + Kx[i] * ws_net_at(d, c - 1, r));
+        }
+        const double r0 = d.u0[i] - a_u0;
+        dd_add_prod(acc[0], v[0], av[0]);
+        dd_add_prod(acc[1], v[0], av[1]);
+        dd_add_prod(acc[2], v[0], av[2]);
+        dd_add_prod(acc[3], v[1], av[1]);
+        dd_add_prod(acc[4], v[1], av[2]);
+        dd_add_prod(acc[5], v[2], av[2]);
+        dd_add_prod(acc[6], v[0], r0);
+        dd_add_prod(acc[7], v[1], r0);
+        dd_add_prod(acc[8], v[2], r0);
+    }
+    STRIP_END
+    block_reduce<9>(acc, red);
+}
+
+// u = u0 + c0 v0 + c1 v1 + c2 v2; then up2 = up1, up1 = u0 (apply == 0: only
+// the shift of the previous fields)
+__global__ __launch_bounds__(BLOCK) void k_ws_apply(Ext e, long pitch, double *__restrict__ u, WsDirs d,
+                                                    double *__restrict__ up1, double *__restrict__ up2, double c0,
+                                                    double c1, double c2, int apply)
+{
+    STRIP_BEGIN
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const double u0 = d.u0[i], p1 = up1[i];
+        if (apply) {
+            double v = u0 + c0 * (u0 - p1);
+            if (c1 != 0.0) v += c1 * (p1 - up2[i]);
+            if (c2 != 0.0) v += c2 * ws_net_at(d, s.c, s.r0 + k);
+            u[i] = v;
+        }
+        up2[i] = p1;
+        up1[i] = u0;
+    }
+    STRIP_END
+}
+
 // ---- solver: CG -------------------------------------------------------------------
 
 // ---- solver: Chebyshev --------------------------------------------------------------
@@ -659,6 +793,10 @@ void release_all()
     if (g.h_out) (void)hipHostFree(g.h_out);
     if (g.d_halo_buf) (void)hipFree(g.d_halo_buf);
     if (g.slab32) (void)hipFree(g.slab32);
+    ws::net_free(g.ws_net);
+    ws::acts_free(g.ws_acts);
+    if (g.ws_planes) (void)hipFree(g.ws_planes);
+    if (g.ws_slab) (void)hipFree(g.ws_slab);
     if (g.stream) (void)hipStreamDestroy(g.stream);
     Profiler keep = g.prof;
     g = Chunk();
@@ -1418,6 +1556,152 @@ EXPORT int tea_engine_set_field32(int id, const float *host)
     HIP_OK_R(hipMemcpy2DAsync(g.fs32[id], g.pitch32 * sizeof(float), host, w * sizeof(float), w * sizeof(float),
                               (size_t)g.rows, hipMemcpyHostToDevice, g.stream), -1);
     if (id == F_SD) g.sd32_halo_dirty = true;
+    HIP_OK_R(hipStreamSynchronize(g.stream), -1);
+    return 0;
+}
+
+// =============================================================================
+// Learned warm start (no reference counterpart; tea_warmstart.h)
+// =============================================================================
+
+EXPORT int tea_engine_warm_start_load(const float *params, int n)
+{
+    if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return -1; }
+    if (params && n != ws::NPARAM) {
+        die("tea_engine_warm_start_load: expected %d parameters, got %d", ws::NPARAM, n);
+        return -1;
+    }
+    if (params && !ws::shape_ok(g.ny, g.nx)) {
+        die("tea_engine_warm_start_load: the network needs x_cells %% 64 == 0 and y_cells %% 4 == 0 (%d x %d)", g.nx,
+            g.ny);
+        return -1;
+    }
+    if (!g.ws_slab) {
+        const size_t elems = (size_t)g.pitch * (size_t)g.rows + 16;
+        HIP_OK_R(hipMalloc(&g.ws_slab, 2 * elems * sizeof(double)), -1);
+        HIP_OK_R(hipMemset(g.ws_slab, 0, 2 * elems * sizeof(double)), -1);
+        g.ws_up1 = g.ws_slab + (g.fs[F_U] - g.f[F_U]);
+        g.ws_up2 = g.ws_up1 + elems;
+    }
+    if (params) {
+        if (!g.ws_net.params) {
+            HIP_OK_R(ws::net_alloc(g.ws_net), -1);
+            HIP_OK_R(ws::acts_alloc(g.ws_acts, 1, g.ny, g.nx, false), -1);
+            HIP_OK_R(hipMalloc(&g.ws_planes, 3 * (size_t)g.nx * g.ny * sizeof(float)), -1);
+        }
+        HIP_OK_R(ws::net_upload(g.ws_net, params, g.stream), -1);
+    }
+    g.ws_on = true;
+    g.ws_use_net = params != nullptr;
+    g.ws_nprev = 0;
+    return 0;
+}
+
+EXPORT void tea_engine_warm_start_reset(void) { g.ws_nprev = 0; }
+
+namespace {
+// Solves the k x k system G c = b (G symmetric positive semi-definite) by
+// Cholesky, dropping a direction whose pivot falls below 1e-12 of its
+// diagonal (linearly dependent on the earlier ones); c = 0 for dropped ones.
+void ws_solve(int k, double G[3][3], const double *b, double *c)
+{
+    double L[3][3] = {{0}};
+    bool use[3] = {false, false, false};
+    for (int j = 0; j < k; j++) {
+        double dj = G[j][j];
+        for (int m = 0; m < j; m++) dj -= L[j][m] * L[j][m];
+        if (!(G[j][j] > 0.0) || !(dj > 1e-12 * G[j][j])) continue;
+        use[j] = true;
+        L[j][j] = std::sqrt(dj);
+        for (int i = j + 1; i < k; i++) {
+            double v = G[i][j];
+            for (int m = 0; m < j; m++) v -= L[i][m] * L[j][m];
+            L[i][j] = v / L[j][j];
+        }
+    }
+    double y[3] = {0, 0, 0};
+    for (int i = 0; i < k; i++) {
+        if (!use[i]) continue;
+        double v = b[i];
+        for (int m = 0; m < i; m++) v -= L[i][m] * y[m];
+        y[i] = v / L[i][i];
+    }
+    for (int i = k - 1; i >= 0; i--) {
+        c[i] = 0.0;
+        if (!use[i]) continue;
+        double v = y[i];
+        for (int m = i + 1; m < k; m++) v -= L[m][i] * c[m];
+        c[i] = v / L[i][i];
+    }
+}
+}  // namespace
+
+EXPORT void tea_engine_warm_start_apply(int *applied)
+{
+    CHECK_INIT();
+    *applied = 0;
+    if (!g.ws_on) { die("tea_engine_warm_start_apply: tea_engine_warm_start_load was not called"); return; }
+    const Ext in = interior();
+    WsDirs d{(const double *)F(X_U0), g.ws_up1, g.ws_up2, nullptr, 0.0, g.h, g.nx, g.ny};
+    double c[3] = {0.0, 0.0, 0.0};
+    int apply = 0;
+    if (g.ws_nprev >= 1) {
+        if (g.ws_use_net) {
+            if (!launch("ws_stats", k_ws_stats, in, g.pitch, d.u0, (const double *)g.ws_up1, red_ctx())) return;
+            double sums[2];
+            if (!fetch(sums, 2)) return;
+            const double cells = (double)g.nx * (double)g.ny;
+            const double sc = std::sqrt(sums[0] / cells), su = std::sqrt(sums[1] / cells);
+            if (sc > 0.0 && su > 0.0) {
+                const long n = (long)g.nx * g.ny;
+                if (!launch("ws_features", k_ws_features, in, g.pitch, g.h, g.nx, n, (const double *)F(F_DENSITY),
+                            d.u0, (const double *)g.ws_up1, 1.0 / sc, 1.0 / su, g.ws_planes))
+                    return;
+                {
+                    Timed t("ws_network");
+                    ws::im2col(g.ws_acts, g.ws_planes, g.stream);
+                    ws::forward(g.ws_net, g.ws_acts, g.stream);
+                }
+                HIP_OK(hipGetLastError());
+                // the network predicts (du - (u0 - up1)) / s: direction s * net
+                d.net = g.ws_acts.out;
+                d.scale = sc;
+            }
+        }
+        const int two = g.ws_nprev >= 2 ? 1 : 0;
+        if (!launch("ws_gram", k_ws_gram, in, g.pitch, d, (const double *)F(X_KX), (const double *)F(X_KY), two,
+                    red_ctx()))
+            return;
+        double r[9];
+        if (!fetch(r, 9)) return;
+        double G[3][3] = {{r[0], r[1], r[2]}, {r[1], r[3], r[4]}, {r[2], r[4], r[5]}};
+        double b[3] = {r[6], r[7], r[8]};
+        // active directions in order: v0, [v1], [v2]
+        int idx[3], k = 0;
+        idx[k++] = 0;
+        if (two) idx[k++] = 1;
+        if (d.net) idx[k++] = 2;
+        double Gs[3][3], bs[3], cs[3];
+        for (int a = 0; a < k; a++) {
+            bs[a] = b[idx[a]];
+            for (int q = 0; q < k; q++) Gs[a][q] = G[idx[a]][idx[q]];
+        }
+        ws_solve(k, Gs, bs, cs);
+        for (int a = 0; a < k; a++) c[idx[a]] = cs[a];
+        apply = 1;
+    }
+    if (!launch("ws_apply", k_ws_apply, in, g.pitch, F(F_U), d, g.ws_up1, g.ws_up2, c[0], c[1], c[2], apply)) return;
+    g.ws_nprev = g.ws_nprev < 2 ? g.ws_nprev + 1 : 2;
+    *applied = apply;
+}
+
+EXPORT int tea_engine_copy_field(int id, double *dst)
+{
+    if (!g.initialised || id < 1 || id >= X_COUNT || !dst) return -1;
+    const double *src = g.fs[id] + (long)g.h + (long)g.h * g.pitch;
+    HIP_OK_R(hipMemcpy2DAsync(dst, (size_t)g.nx * sizeof(double), src, (size_t)g.pitch * sizeof(double),
+                              (size_t)g.nx * sizeof(double), (size_t)g.ny, hipMemcpyDeviceToDevice, g.stream),
+             -1);
     HIP_OK_R(hipStreamSynchronize(g.stream), -1);
     return 0;
 }

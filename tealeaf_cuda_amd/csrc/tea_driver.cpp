@@ -86,6 +86,8 @@ struct Backend {
     void (*mp_setup)(void) = nullptr;
     void (*mp_begin)(double *) = nullptr;
     void (*mp_end)(void) = nullptr;
+    int (*warm_start_load)(const float *, int) = nullptr;
+    void (*warm_start_apply)(int *) = nullptr;
 
     void check() const
     {
@@ -116,6 +118,8 @@ Backend load_backend(const char *path)
     b.mp_setup = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_setup"));
     b.mp_begin = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_mp_begin"));
     b.mp_end = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_end"));
+    b.warm_start_load = reinterpret_cast<int (*)(const float *, int)>(dlsym(b.handle, "tea_engine_warm_start_load"));
+    b.warm_start_apply = reinterpret_cast<void (*)(int *)>(dlsym(b.handle, "tea_engine_warm_start_apply"));
     return b;
 }
 
@@ -156,7 +160,30 @@ struct Deck {
     double reliable_delta = 0.1; // tl_reliable_delta: residual drop that triggers a replacement
     double inner_eps = 1.0e-4;   // tl_inner_eps: restart scheme's inner tolerance (relative)
     int max_refine = 30;         // tl_max_refinements: restart scheme's limit
+    std::string warm_start;      // tl_warm_start <weight file>: learned initial guess
 };
+
+// The text after `key` on the deck line that starts with it, case kept (the
+// keyword parser lower-cases and truncates lines; a file path must not be).
+std::string raw_value(const std::string &path, const char *key)
+{
+    std::ifstream in(path);
+    std::string l;
+    const size_t kl = std::strlen(key);
+    while (std::getline(in, l)) {
+        size_t a = l.find_first_not_of(" \t");
+        if (a == std::string::npos || l.size() - a < kl) continue;
+        std::string head = l.substr(a, kl);
+        for (auto &ch : head) ch = (char)std::tolower((unsigned char)ch);
+        if (head != key) continue;
+        size_t b = a + kl;
+        while (b < l.size() && (l[b] == ' ' || l[b] == '\t' || l[b] == '=')) b++;
+        size_t e = b;
+        while (e < l.size() && l[e] != ' ' && l[e] != '\t' && l[e] != '\r') e++;
+        return l.substr(b, e - b);
+    }
+    return std::string();
+}
 
 // One logical deck line as parse_getline returns it (parse.f90:107-236):
 // control characters blanked, text after '!' or ';' dropped, lower-cased,
@@ -293,6 +320,10 @@ Deck read_deck(const std::string &path)
                 else if (p == "reliable") d.restart = false;
                 else fail("read_input: tl_mixed_refinement must be reliable or restart, not %s", p.c_str());
             } else if (w == "tl_reliable_delta") d.reliable_delta = getrval(next("tl_reliable_delta"));
+            else if (w == "tl_warm_start") {
+                next("tl_warm_start");
+                d.warm_start = raw_value(path, "tl_warm_start");
+            }
             else if (w == "tl_inner_eps") d.inner_eps = getrval(next("tl_inner_eps"));
             else if (w == "tl_max_refinements") d.max_refine = getival(next("tl_max_refinements"));
             else if (w == "state") {
@@ -331,6 +362,10 @@ Deck read_deck(const std::string &path)
         fail("read_input: tl_inner_precision needs tl_use_cg or tl_use_ppcg");
     if (d.refine_bits && d.reflective_boundary)
         fail("read_input: tl_inner_precision does not support reflective_boundary");
+    if (!d.warm_start.empty() && !(d.tl_use_cg || d.tl_use_ppcg))
+        fail("read_input: tl_warm_start needs tl_use_cg or tl_use_ppcg");
+    if (!d.warm_start.empty() && d.reflective_boundary)
+        fail("read_input: tl_warm_start does not support reflective_boundary");
     if (d.refine_bits && !(d.reliable_delta > 0.0 && d.reliable_delta < 1.0))
         fail("read_input: tl_reliable_delta must lie in (0, 1)");
     if (d.halo_exchange_depth > 1 && d.tl_preconditioner_type == TL_PREC_JAC_BLOCK && d.tl_use_ppcg)
@@ -489,6 +524,7 @@ private:
 
     void update_halo(const int *fields, int depth);
     void field_summary();
+    void load_warm_start();
     void tea_leaf();
     struct Solve {
         int n;                  // loop index at exit (iteration count + 1)
@@ -497,8 +533,10 @@ private:
         int ppcg_inner;         // PPCG inner steps taken
     };
     Solve iterate(double initial_residual_sq, double eps);
-    Solve refine(double initial_residual_sq, int *refinements, int *iterations, int *inner);
-    Solve refine_reliable(double initial_residual_sq, int *updates, int *iterations, int *inner);
+    Solve refine(double initial_residual_sq, double target_residual_sq, int *refinements, int *iterations,
+                 int *inner);
+    Solve refine_reliable(double initial_residual_sq, double target_residual_sq, int *updates, int *iterations,
+                          int *inner);
 
     // residual replacement state of refine_reliable (off in the reference's solve)
     struct Reliable {
@@ -510,6 +548,8 @@ private:
         bool have_q = false;
         bool fp32_live = false; // the backend is in FP32 mode with u32 pending
         bool converged = false;
+        bool stagnated = false; // the true residual stopped falling: FP64's attainable accuracy
+        int stalls = 0;
         int updates = 0;
         double rr_final = 0.0;
     } rel;
@@ -621,6 +661,36 @@ void Run::start()
     b.set_field_kernel_cuda_();
     b.check();
     field_summary();
+
+    if (!d.warm_start.empty()) load_warm_start();
+}
+
+// Weight file: "TLWS0001", int32 count, count float32 (little-endian); the
+// value "extrapolate" selects the same energy-minimising guess without a
+// network (previous-step differences only).
+void Run::load_warm_start()
+{
+    if (!b.warm_start_load || !b.warm_start_apply) fail("backend has no warm-start entry points (tl_warm_start)");
+    if (d.warm_start == "extrapolate") {
+        if (b.warm_start_load(nullptr, 0) != 0) b.check();
+        b.check();
+        log("Warm start from previous steps (no network)\n");
+        return;
+    }
+    std::ifstream in(d.warm_start, std::ios::binary);
+    if (!in) fail("tl_warm_start: cannot open %s", d.warm_start.c_str());
+    char magic[8];
+    int32_t n = 0;
+    in.read(magic, 8);
+    in.read(reinterpret_cast<char *>(&n), 4);
+    if (!in || std::memcmp(magic, "TLWS0001", 8) != 0 || n <= 0 || n > (1 << 24))
+        fail("tl_warm_start: %s is not a weight file", d.warm_start.c_str());
+    std::vector<float> params(n);
+    in.read(reinterpret_cast<char *>(params.data()), (std::streamsize)n * 4);
+    if (!in) fail("tl_warm_start: %s is truncated", d.warm_start.c_str());
+    if (b.warm_start_load(params.data(), n) != 0) b.check();
+    b.check();
+    log("Warm start network %s (%d parameters)\n", d.warm_start.c_str(), n);
 }
 
 // tea_leaf_ppcg.f90:174-230
@@ -680,6 +750,20 @@ void Run::tea_leaf()
     fields[FIELD_U - 1] = 1;
     update_halo(fields, 1);
 
+    // With a warm start the target stays tl_eps times the residual of the
+    // u = u^n guess; the solve starts from the network's guess.
+    double target_residual = -1.0;
+    int warm = 0;
+    if (!d.warm_start.empty()) {
+        b.tea_leaf_calc_residual_cuda_();
+        b.check();
+        int which = 1;
+        b.tea_leaf_calc_2norm_kernel_cuda_(&which, &target_residual);
+        b.check();
+        b.warm_start_apply(&warm);
+        b.check();
+        update_halo(fields, 1);
+    }
     b.tea_leaf_calc_residual_cuda_();
     b.check();
     {
@@ -687,14 +771,18 @@ void Run::tea_leaf()
         b.tea_leaf_calc_2norm_kernel_cuda_(&which, &initial_residual);
         b.check();
     }
+    if (target_residual < 0.0) target_residual = initial_residual;
+    if (warm)
+        log("Warm start: residual %s of the u^n guess\n",
+            fortran_e(std::sqrt(std::fabs(initial_residual)) / std::sqrt(std::fabs(target_residual)), 14, 7).c_str());
     int refinements = 0, iterations = 0, ppcg_inner_iters = 0;
     Solve sol;
     if (d.refine_bits && d.restart) {
-        sol = refine(initial_residual, &refinements, &iterations, &ppcg_inner_iters);
+        sol = refine(initial_residual, target_residual, &refinements, &iterations, &ppcg_inner_iters);
     } else if (d.refine_bits) {
-        sol = refine_reliable(initial_residual, &refinements, &iterations, &ppcg_inner_iters);
+        sol = refine_reliable(initial_residual, target_residual, &refinements, &iterations, &ppcg_inner_iters);
     } else {
-        sol = iterate(initial_residual, d.eps);
+        sol = iterate(target_residual, d.eps);
         iterations = sol.n - 1;
         ppcg_inner_iters = sol.ppcg_inner;
     }
@@ -723,6 +811,7 @@ void Run::tea_leaf()
     if (k < TEA_MAX_STEPS) {
         R->iterations[k] = iterations;
         R->refinements[k] = refinements;
+        R->warm_start[k] = warm;
         R->ppcg_inner[k] = ppcg_inner_iters;
         R->conduction_error[k] = error / initial_residual;
         R->exact_error[k] = d.tl_check_result ? exact_error / initial_residual : -1.0;
@@ -919,11 +1008,12 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
 // same CG/PPCG loop on FP32 fields (FP64 dot products and host numerics) to
 // tl_inner_eps relative, and u += d in FP64.  The eigenvalue estimate comes
 // from the first inner solve's CG pre-steps and is kept across solves.
-Run::Solve Run::refine(double initial_residual_sq, int *refinements, int *iterations, int *inner)
+Run::Solve Run::refine(double initial_residual_sq, double target_residual_sq, int *refinements, int *iterations,
+                       int *inner)
 {
     if (!b.set_precision || !b.mp_setup || !b.mp_begin || !b.mp_end)
         fail("backend has no mixed-precision entry points (tl_inner_precision fp32)");
-    const double ir = std::sqrt(std::fabs(initial_residual_sq));
+    const double ir = std::sqrt(std::fabs(target_residual_sq));
     if (b.set_precision(32) != 0) b.check();
     b.set_precision(64);
     b.mp_setup();
@@ -981,12 +1071,14 @@ Run::Solve Run::refine(double initial_residual_sq, int *refinements, int *iterat
 // p, so the Krylov process is not restarted.
 // At 64 bits the same loop is CG/PPCG with residual replacement and a stop
 // test on the true residual.
-Run::Solve Run::refine_reliable(double initial_residual_sq, int *updates, int *iterations, int *inner)
+Run::Solve Run::refine_reliable(double initial_residual_sq, double target_residual_sq, int *updates,
+                                int *iterations, int *inner)
 {
     const int bits = d.refine_bits;
     if (bits == 32 && (!b.set_precision || !b.mp_setup || !b.mp_begin || !b.mp_end))
         fail("backend has no mixed-precision entry points (tl_inner_precision fp32)");
-    const double ir = std::sqrt(std::fabs(initial_residual_sq));
+    const double ir = std::sqrt(std::fabs(target_residual_sq));
+    const double r0 = std::sqrt(std::fabs(initial_residual_sq));
     rel = Reliable();
     rel.on = true;
     rel.bits = bits;
@@ -994,8 +1086,8 @@ Run::Solve Run::refine_reliable(double initial_residual_sq, int *updates, int *i
     rel.rr_ref = initial_residual_sq;
     rel.rr_final = initial_residual_sq;
     double rr = initial_residual_sq;
-    Solve s{1, ir, ir, 0};
-    if (!(ir < rel.target) && ir > 0.0) {
+    Solve s{1, r0, ir, 0};
+    if (!(r0 < rel.target) && r0 > 0.0) {
         if (bits == 32) {
             if (b.set_precision(32) != 0) b.check();
             b.mp_setup();
@@ -1053,6 +1145,17 @@ bool Run::reliable_step(double q, bool refresh_z)
     if (d.verbose_on) log(" Replacement %d true residual %.16E\n", rel.updates, std::sqrt(std::fabs(rr)));
     if (std::sqrt(std::fabs(rr)) < rel.target) {
         rel.converged = true;
+        return true;
+    }
+    // The true residual came out more than 4x above what the loop's own
+    // measure predicted: the recurrence no longer tracks it.  Three such
+    // replacements in a row mean the target lies below what FP64 resolves.
+    rel.stalls = std::sqrt(std::fabs(rr)) > 4.0 * predicted ? rel.stalls + 1 : 0;
+    if (rel.stalls >= 3) {
+        rel.converged = true;
+        rel.stagnated = true;
+        log("True residual stagnated at %s of the target\n",
+            fortran_e(std::sqrt(std::fabs(rr)) / rel.target, 14, 7).c_str());
         return true;
     }
     if (rel.bits == 32) {

@@ -25,7 +25,8 @@ typedef struct {
     double exact_error[TEA_MAX_STEPS];            /* ||u0 - A u|| / ||r0|| (tl_check_result) */
     double initial_residual[TEA_MAX_STEPS];       /* ||r0|| */
     double step_seconds[TEA_MAX_STEPS];           /* wall time of the step */
-    int refinements[TEA_MAX_STEPS];               /* mixed precision: FP32 inner solves per step */
+    int refinements[TEA_MAX_STEPS];               /* refinement: residual replacements / inner solves */
+    int warm_start[TEA_MAX_STEPS];                /* 1 if the learned initial guess was used */
     int n_summaries;
     int summary_step[TEA_MAX_SUMMARIES];
     double summary[TEA_MAX_SUMMARIES][6];         /* time, volume, mass, density, energy, u */

@@ -129,6 +129,7 @@ class Deck:
     reliable_delta: float | None = None
     inner_eps: float | None = None
     max_refinements: int | None = None
+    warm_start: str | None = None        # weight file of the learned initial guess
     states: list = field(default_factory=standard_states)
     extra: list = field(default_factory=list)
 
@@ -178,6 +179,8 @@ class Deck:
             lines.append(f"tl_inner_eps={_f(self.inner_eps)}")
         if self.max_refinements is not None:
             lines.append(f"tl_max_refinements={self.max_refinements}")
+        if self.warm_start is not None:
+            lines.append(f"tl_warm_start {self.warm_start}")
         lines += list(self.extra)
         lines.append("*endtea")
         return "\n".join(lines) + "\n"

@@ -286,6 +286,7 @@ class Engine:
         return (self.ny + 2 * self.h, self.nx + 2 * self.h)
 
     def get_field(self, field_id) -> np.ndarray:
+        self.refresh()  # the driver may have (re)initialised the chunk since
         a = np.empty(self.shape(), dtype=np.float64)
         if self.lib.tea_engine_get_field(int(field_id), a.ctypes.data_as(_D)) != 0:
             self._check()
@@ -293,6 +294,7 @@ class Engine:
         return a
 
     def set_field(self, field_id, a) -> None:
+        self.refresh()
         a = np.ascontiguousarray(a, dtype=np.float64)
         if a.shape != self.shape():
             raise ValueError(f"field shape {a.shape} != {self.shape()}")
@@ -336,13 +338,17 @@ class Engine:
         self._call("tea_engine_mp_end")
 
     def get_field32(self, field_id) -> np.ndarray:
+        self.refresh()  # the driver may have (re)initialised the chunk since
         a = np.empty(self.shape(), dtype=np.float32)
         if self.lib.tea_engine_get_field32(int(field_id), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
             raise TeaError(f"get_field32({field_id}) failed")
         return a
 
     def set_field32(self, field_id, a) -> None:
+        self.refresh()
         a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.shape != self.shape():
+            raise ValueError(f"field shape {a.shape} != {self.shape()}")
         if self.lib.tea_engine_set_field32(int(field_id), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
             raise TeaError(f"set_field32({field_id}) failed")
 
@@ -365,6 +371,7 @@ class _RunResult(ctypes.Structure):
         ("initial_residual", ctypes.c_double * TEA_MAX_STEPS),
         ("step_seconds", ctypes.c_double * TEA_MAX_STEPS),
         ("refinements", ctypes.c_int * TEA_MAX_STEPS),
+        ("warm_start", ctypes.c_int * TEA_MAX_STEPS),
         ("n_summaries", ctypes.c_int),
         ("summary_step", ctypes.c_int * TEA_MAX_SUMMARIES),
         ("summary", (ctypes.c_double * 6) * TEA_MAX_SUMMARIES),
@@ -399,6 +406,7 @@ class RunResult:
     initial_residual: list
     step_seconds: list
     refinements: list = field(default_factory=list)
+    warm_start: list = field(default_factory=list)
     summaries: list = field(default_factory=list)
     eigmin: float = 0.0
     eigmax: float = 0.0
@@ -450,6 +458,7 @@ def run_deck(deck_path, backend: str | os.PathLike | None = None, out_path=None,
         initial_residual=list(res.initial_residual[:n]),
         step_seconds=list(res.step_seconds[:n]),
         refinements=list(res.refinements[:n]),
+        warm_start=list(res.warm_start[:n]),
         eigmin=res.eigmin,
         eigmax=res.eigmax,
         total_seconds=res.total_seconds,

@@ -47,3 +47,18 @@ def test_library_exports_every_declared_symbol(lib, oracle_so):
 def test_driver_library_loads(driver_built):
     so = ctypes.CDLL(str(DRIVER_SO))
     assert hasattr(so, "tea_driver_run") and hasattr(so, "tea_driver_set_step_callback")
+
+
+WS_HEADER = REPO / "include" / "tea_warmstart.h"
+
+
+def test_engine_exports_the_warm_start_interface():
+    from tealeaf_cuda_amd.warmstart import WARMSTART_SYMBOLS
+
+    text = re.sub(r"/\*.*?\*/", "", WS_HEADER.read_text(), flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+char\s*\*|void\s*\*|void|int)\s*(\w+)\s*\(", text, flags=re.M)
+    assert set(names) == set(WARMSTART_SYMBOLS)
+    if not ENGINE_SO.exists():
+        pytest.fail("libtea_engine.so has not been built (run __graft_entry__.build())")
+    so = ctypes.CDLL(str(ENGINE_SO))
+    assert not [n for n in names if not hasattr(so, n)]

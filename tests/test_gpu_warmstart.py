@@ -1,0 +1,203 @@
+"""GPU tests of the learned warm start: the HIP network kernels against the
+CPU oracle (oracle/cnn_oracle.py, bfloat16 rounding emulated at the points
+where the kernels store BF16), training on the GPU, the engine's inference
+on its own fields, and whole decks that use it.
+
+Tolerances: the GPU accumulates in FP32 in its own order, so a BF16 value
+that sits at a rounding boundary can round the other way (one BF16 unit,
+2^-8 relative); relative norm errors of 5e-3 (forward) and 2e-2 (gradient
+blocks) leave room for a few such flips and catch any wrong term.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch  # noqa: F401  (first: torch's HIP runtime must be the process's one)
+
+from conftest import REPO
+from tealeaf_cuda_amd import Deck, Engine, run_deck
+from tealeaf_cuda_amd import runtime as rt
+from tealeaf_cuda_amd import warmstart as wsm
+
+sys.path.insert(0, str(REPO / "oracle"))
+import cnn_oracle as co  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed, w6_scale=0.05):
+    rng = np.random.default_rng(seed)
+    p = co.init_params(seed).astype(np.float64)
+    p[co.OFF_W6:co.OFF_B6] = rng.standard_normal(co.OFF_B6 - co.OFF_W6) * w6_scale
+    p[co.OFF_B1:co.OFF_HID] = rng.standard_normal(co.CH) * 0.1
+    p[co.OFF_B6] = 0.01
+    return p.astype(np.float32)
+
+
+def _samples(seed, B, H, W):
+    rng = np.random.default_rng(seed)
+    s = rng.standard_normal((B, 6, H, W)).astype(np.float32)
+    s[:, 4:] = rng.uniform(0.1, 3.0, (B, 2, H, W))
+    return s, torch.from_numpy(s).cuda()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_forward_matches_oracle():
+    print("[gpu warmstart] forward vs oracle", flush=True)
+    B, H, W = 2, 8, 64
+    t = wsm.Trainer(B, H, W, margin=1)
+    try:
+        p = _params(1)
+        t.set_params(p)
+        s, sd = _samples(2, B, H, W)
+        out = t.forward(sd)
+        ref, _ = co.forward(p, s, mode="bf16")
+        assert _rel(out, ref) < 5e-3
+        exact, _ = co.forward(p, s)
+        assert _rel(out, exact) < 5e-2
+    finally:
+        t.close()
+
+
+@pytest.mark.parametrize("loss_kind", ["mse", "energy"])
+def test_gradient_matches_oracle(loss_kind):
+    print(f"[gpu warmstart] gradient vs oracle ({loss_kind})", flush=True)
+    B, H, W = 2, 8, 64
+    t = wsm.Trainer(B, H, W, margin=1, loss=loss_kind)
+    try:
+        p = _params(3)
+        t.set_params(p)
+        s, sd = _samples(4, B, H, W)
+        loss, g = t.gradient(sd)
+        rl, rg = co.loss_and_grad(p, s, margin=1, mode="bf16", loss=loss_kind)
+        assert loss == pytest.approx(rl, rel=1e-3)
+        blocks = [("w1", co.OFF_W1, co.OFF_B1), ("b1", co.OFF_B1, co.OFF_HID)]
+        for l in range(4):
+            o = co.OFF_HID + l * co.HID
+            blocks += [(f"w{l + 2}", o, o + co.TAPS * co.CH * co.CH), (f"b{l + 2}", o + co.TAPS * co.CH * co.CH,
+                                                                      o + co.HID)]
+        blocks += [("w6", co.OFF_W6, co.OFF_B6), ("b6", co.OFF_B6, co.NPARAM)]
+        for name, lo, hi in blocks:
+            assert _rel(g[lo:hi], rg[lo:hi]) < 2e-2, name
+    finally:
+        t.close()
+
+
+def test_training_fits_a_learnable_target():
+    """Target = a fixed linear stencil of the features: Adam on the GPU must
+    cut the loss by a large factor."""
+    print("[gpu warmstart] training", flush=True)
+    B, H, W = 4, 16, 64
+    rng = np.random.default_rng(5)
+    s = rng.standard_normal((B, 6, H, W)).astype(np.float32)
+    s[:, 4:] = 0.5
+    f = s[:, :3]
+    s[:, 3] = 0.5 * f[:, 1] - 0.25 * np.roll(f[:, 1], 1, axis=2) + 0.3 * f[:, 0] * (f[:, 2] > 0)
+    sd = torch.from_numpy(s).cuda()
+    t = wsm.Trainer(B, H, W, margin=1, loss="mse")
+    try:
+        t.set_params(co.init_params(6))
+        first = t.step(sd, 1e-3, want_loss=True)
+        for _ in range(300):
+            t.step(sd, 1e-3)
+        last, _ = t.gradient(sd)
+        assert last < 0.05 * first, (first, last)
+    finally:
+        t.close()
+
+
+def _operator(eng, h, nx, ny):
+    """The interior 5-point operator I - dt div K grad from the engine's Kx/Ky."""
+    kx, ky = eng.get_field(rt.XFIELD_KX), eng.get_field(rt.XFIELD_KY)
+
+    def A(v):
+        p = np.zeros(eng.shape())
+        p[h:h + ny, h:h + nx] = v
+        c = (slice(h, h + ny), slice(h, h + nx))
+        kr, kl = kx[h:h + ny, h + 1:h + nx + 1], kx[c]
+        ku, kd = ky[h + 1:h + ny + 1, h:h + nx], ky[c]
+        return ((1 + ku + kd + kr + kl) * v - ku * p[h + 1:h + ny + 1, h:h + nx] - kd * p[h - 1:h + ny - 1, h:h + nx]
+                - kr * p[h:h + ny, h + 1:h + nx + 1] - kl * p[h:h + ny, h - 1:h + nx - 1])
+    return A
+
+
+@pytest.mark.parametrize("with_net", [True, False])
+def test_engine_apply_matches_oracle(with_net, tmp_path):
+    """tea_engine_warm_start_apply on engine fields: the guess u0 + sum c_a v_a
+    with the energy-minimising coefficients, computed here from the oracle's
+    network output and a dense restatement of the operator."""
+    import ctypes
+
+    print(f"[gpu warmstart] engine apply (network {with_net})", flush=True)
+    from test_oracle_math import setup_chunk
+
+    lib = wsm.library()
+    eng = Engine()
+    try:
+        nx, ny = 64, 8
+        setup_chunk(eng, tmp_path, nx=nx, ny=ny, solver="cg", halo=1)
+        p = _params(7, w6_scale=0.2)
+        if with_net:
+            assert lib.tea_engine_warm_start_load(p.ctypes.data_as(wsm._F), p.size) == 0
+        else:
+            assert lib.tea_engine_warm_start_load(None, 0) == 0
+        rng = np.random.default_rng(8)
+        h = eng.h
+        shape = eng.shape()
+        A = _operator(eng, h, nx, ny)
+
+        def put(fid, vals):
+            a = np.zeros(shape)
+            a[h:h + ny, h:h + nx] = vals
+            eng.set_field(fid, a)
+
+        rho = 10.0 ** rng.uniform(-1, 2, (ny, nx))
+        put(rt.FIELD_DENSITY, rho)
+        us = [rng.uniform(1.0, 3.0, (ny, nx))]
+        for _ in range(2):
+            us.append(us[-1] + rng.standard_normal((ny, nx)) * 0.1)
+        applied = ctypes.c_int(-1)
+        for k, u0 in enumerate(us):
+            put(rt.XFIELD_U0, u0)
+            put(rt.FIELD_U, u0)
+            lib.tea_engine_warm_start_apply(ctypes.byref(applied))
+            assert applied.value == (1 if k else 0)
+            if k == 0:
+                continue
+            dirs = [u0 - us[k - 1]]
+            if k >= 2:
+                dirs.append(us[k - 1] - us[k - 2])
+            if with_net:
+                d = dirs[0]
+                sc, su = np.sqrt(np.mean(d * d)), np.sqrt(np.mean(u0 * u0))
+                planes = np.stack([(np.log10(rho) - 0.5) / 1.5, d / sc, u0 / su]).astype(np.float32)[None]
+                net, _ = co.forward(p, planes, mode="bf16")
+                dirs.append(sc * net[0])
+            AV = [A(v) for v in dirs]
+            r0 = u0 - A(u0)
+            G = np.array([[np.sum(a * b) for b in AV] for a in dirs])
+            c = np.linalg.solve(G, np.array([np.sum(v * r0) for v in dirs]))
+            expect = u0 + sum(ci * v for ci, v in zip(c, dirs))
+            u = eng.get_field(rt.FIELD_U)[h:h + ny, h:h + nx]
+            tol = (2e-2 if with_net else 1e-10) * np.abs(expect - u0).max()
+            assert np.abs(u - expect).max() <= tol, (k, np.abs(u - expect).max(), tol)
+    finally:
+        eng.finalize()
+
+
+def test_warm_start_deck_reaches_the_same_target(tmp_path):
+    """A deck with tl_warm_start: the guess is used from step 2 on and every
+    step still reaches ||u0 - A u|| < 1e-10 ||u0 - A u^n||."""
+    print("[gpu warmstart] deck", flush=True)
+    wpath = wsm.save_weights(tmp_path / "w.tlws", _params(9, w6_scale=0.01))
+    for prec, ws in (("fp64", str(wpath)), ("fp32", str(wpath)), ("fp64", "extrapolate")):
+        deck = wsm.solver_deck(128, 21, 4, inner_precision=prec, check_result=True, warm_start=ws)
+        r = run_deck(deck.write(tmp_path / f"ws_{prec}.in"), out_path=tmp_path / f"ws_{prec}.out")
+        assert r.warm_start == [0, 1, 1, 1]
+        assert all(0 < e < 1e-10 for e in r.exact_error), r.exact_error
+        assert "Warm start" in (tmp_path / f"ws_{prec}.out").read_text()

@@ -222,3 +222,16 @@ def test_fp32_mode_rejects_chebyshev_and_jacobi_calls(oracle_so, tmp_path):
             eng.set_precision(16)
     finally:
         eng.finalize()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_unreachable_target_stops_on_stagnation(prec, oracle_so, oracle_env, tmp_path):
+    """tl_eps below FP64's attainable accuracy: the refinement loop notices the
+    true residual no longer follows the recurrence and stops long before
+    tl_max_iters, at the best true residual it reached."""
+    oracle_env("dot2")
+    r = _run(tmp_path, oracle_so, "stag", solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+             inner_precision=prec, eps=1e-18, max_iters=5000, end_step=2)
+    assert all(n < 1000 for n in r.iterations), r.iterations
+    assert all(e < 1e-13 for e in r.exact_error), r.exact_error
+    assert "stagnated" in (tmp_path / "stag.out").read_text()

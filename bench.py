@@ -8,12 +8,18 @@ steps and the block-Jacobi preconditioner, dt 0.004, one chunk on one GPU.  A
 step is one timestep: solver set-up, the CG pre-steps / PPCG iterations, and
 the finalisation, all through the host driver over libtea_engine.so.
 
-Three solves of the same deck are timed, each W warm-up steps (the first holds
-the CG eigenvalue estimate) then exactly K timed steps:
+Five solves of the same deck are timed, each W warm-up steps (the first holds
+the CG eigenvalue estimate) then exactly K timed steps.  Every solve but the
+last stops when the FP64 true residual ||u0 - A u||_2 < 1e-10 ||r0||_2, r0
+the residual of the u = u^n guess:
 
-* headline: FP32 fields with FP64 dot accumulators, FP64 residual replacement,
-  stopping when the FP64 true residual ||u0 - A u||_2 < 1e-10 ||r0||_2;
-* fp64_true: the same loop on FP64 fields, the same stop;
+* headline: learned warm start + FP32 fields with FP64 dot accumulators and
+  FP64 residual replacement (the network's weights were trained on the GPU
+  from plain FP64 PPCG solves at this mesh size);
+* fp32_extrapolate: the same without the network (the energy-minimising
+  combination of the previous steps' differences only);
+* fp32: FP32 refinement from the u = u^n guess;
+* fp64_true: plain FP64 PPCG with the same true-residual stop;
 * fp64_reference: plain FP64 PPCG stopping on the reference's own recurrence
   test at tl_eps 1e-10 (its true residual is reported beside it).
 
@@ -62,11 +68,20 @@ ALGO_BYTES = {
 
 
 LAYOUT_SEED = 4096
-VARIANTS = {
-    "headline": dict(inner_precision="fp32"),
-    "fp64_true": dict(inner_precision="fp64"),
-    "fp64_reference": dict(),
-}
+
+
+def weights_for(n):
+    return REPO / "tealeaf_cuda_amd" / "weights" / f"warmstart_{n}.tlws"
+
+
+def variants(n):
+    return {
+        "headline": dict(inner_precision="fp32", warm_start=str(weights_for(n))),
+        "fp32_extrapolate": dict(inner_precision="fp32", warm_start="extrapolate"),
+        "fp32": dict(inner_precision="fp32"),
+        "fp64_true": dict(inner_precision="fp64"),
+        "fp64_reference": dict(),
+    }
 
 
 def make_deck(path, n, steps, solver="ppcg", prec="jac_block", eps=1e-10, max_iters=10000, inner=10,
@@ -83,7 +98,7 @@ def run_gpu(n, warmup, steps, tmp, variant):
     import torch
     import tealeaf_cuda_amd as tl
 
-    deck = make_deck(Path(tmp) / f"tea_{variant}.in", n, warmup + steps, **VARIANTS[variant])
+    deck = make_deck(Path(tmp) / f"tea_{variant}.in", n, warmup + steps, **variants(n)[variant])
     eng = tl.Engine()  # same library instance the driver loads (RTLD_LOCAL handles are shared per path)
     ev = {}
 
@@ -196,25 +211,37 @@ def cpu_baseline(n, gpu_res, timed, tmp):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=19)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated variants to run (profiling); the first one then gives value")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs one chunk on one GPU; multi-GPU runs are not part of this build "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     n = args.size
+    if not weights_for(n).exists():
+        print(f"bench.py: no warm-start weights for {n}^2 ({weights_for(n)})", file=sys.stderr)
+        sys.exit(2)
+    chosen = [v for v in variants(n) if args.only is None or v in args.only.split(",")]
+    if not chosen:
+        print(f"bench.py: --only names none of {list(variants(n))}", file=sys.stderr)
+        sys.exit(2)
+    main_variant = chosen[0]
     runs = {}
     with tempfile.TemporaryDirectory() as tmp:
-        for variant in VARIANTS:
+        for variant in chosen:
             runs[variant] = run_gpu(n, args.warmup, args.steps, tmp, variant)
-        res, gpu_ms, wall_ms, profile, timed = runs["headline"]
+        res, gpu_ms, wall_ms, profile, timed = runs[main_variant]
         ms_per_step = gpu_ms / args.steps
         dominant, roof = roofline(profile, n)
-        _, plain_roof = roofline(runs["fp64_reference"][3], n)
-        cpu = None if args.no_cpu_baseline else cpu_baseline(n, runs["fp64_reference"][0], timed, tmp)
+        plain_roof = roofline(runs["fp64_reference"][3], n)[1] if "fp64_reference" in runs else None
+        cpu = None
+        if not args.no_cpu_baseline and "fp64_reference" in runs:
+            cpu = cpu_baseline(n, runs["fp64_reference"][0], timed, tmp)
 
     def summary(variant):
         r, g_ms, w_ms, _, t = runs[variant]
@@ -222,9 +249,10 @@ def main():
                 "iterations": r.iterations[t], "ppcg_inner": r.ppcg_inner[t],
                 "residual_replacements": r.refinements[t], "exact_rel_residual": r.exact_error[t]}
 
-    comparison = {v: summary(v) for v in VARIANTS}
-    for v in ("fp64_true", "fp64_reference"):
-        comparison[f"speedup_vs_{v}"] = round(comparison[v]["ms_per_step"] / comparison["headline"]["ms_per_step"], 3)
+    comparison = {v: summary(v) for v in runs}
+    for v in [v for v in runs if v != main_variant]:
+        comparison[f"speedup_vs_{v}"] = round(comparison[v]["ms_per_step"] / comparison[main_variant]["ms_per_step"],
+                                              3)
     line = {
         "metric": METRIC,
         "value": round(ms_per_step, 3),
@@ -239,8 +267,8 @@ def main():
         "dtype": "f32 fields, f64 accumulators and residual",
         "data": "synthetic",
         "config": {
-            "workload": f"PPCG, jac_block, 10 inner steps, FP32 fields with FP64 residual replacement, "
-                        f"{n}x{n} cells, random-rectangle layout seed {LAYOUT_SEED}, "
+            "workload": f"PPCG, jac_block, 10 inner steps, learned warm start, FP32 fields with FP64 residual "
+                        f"replacement, {n}x{n} cells, random-rectangle layout seed {LAYOUT_SEED}, "
                         "stop at FP64 true relative residual 1e-10, dt 0.004, halo_depth 1",
             "cells": n * n,
             "solver": "ppcg",
@@ -248,6 +276,7 @@ def main():
             "tl_eps": 1e-10,
             "layout_seed": LAYOUT_SEED,
         },
+        "variant": main_variant,
         "wall_ms_per_step": round(wall_ms / args.steps, 3),
         "iterations": res.iterations[timed],
         "ppcg_inner": res.ppcg_inner[timed],

@@ -212,6 +212,21 @@ void tea_engine_synchronize(void);
 /* Releases every device allocation; the next call must be initialise_cuda_. */
 void tea_engine_finalize(void);
 
+/* ---- fused PPCG outer iteration -------------------------------------------- *
+ * No counterpart in the reference: each call is the composition of reference
+ * entry points given beside it, with identical results, in fewer passes over
+ * HBM and fewer host round trips.  Used by the host driver for PPCG decks
+ * without reflective boundaries (the reference driver's update_halo of p
+ * between ppcg_calc_p and cg_calc_w is a no-op there).  Follow the
+ * precision mode of tea_engine_set_precision.
+ *   tea_engine_ppcg_begin: [ppcg_calc_p(beta) if with_p], cg_calc_w (pw_rro[0]),
+ *                          ppcg_store_r, ppcg_calc_2norm (pw_rro[1])
+ *   tea_engine_ppcg_ur:    cg_calc_ur(alpha) (rrn), ppcg_init_sd_new(theta)
+ *   tea_engine_ppcg_end:   ppcg_update_z, ppcg_calc_rrn (rrn) */
+void tea_engine_ppcg_begin(int with_p, double beta, double *pw_rro);
+void tea_engine_ppcg_ur(double alpha, double theta, double *rrn);
+void tea_engine_ppcg_end(double *rrn);
+
 /* ---- mixed precision: FP32 inner solver under FP64 refinement ------------ *
  * tea_engine_set_precision(32) switches the CG and PPCG entry points
  * (cg_init, cg_calc_w/ur/p, ppcg_init, init_sd(_new), inner, store_r,

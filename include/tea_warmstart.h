@@ -56,6 +56,8 @@ int tea_engine_warm_start_load(const float *params, int n);
 void tea_engine_warm_start_apply(int *applied);
 /* Forgets the previous steps (the next apply only records u^n). */
 void tea_engine_warm_start_reset(void);
+/* c0, c1, c2 of the last guess (all 0 when it only recorded u^n). */
+void tea_engine_warm_start_coefficients(double *c);
 /* Copies the interior of field field_id (TEA_FIELD_* / TEA_XFIELD_*) into a
  * device array of ny x nx doubles (row-major, x fastest).  Synchronous. */
 int tea_engine_copy_field(int field_id, double *device_dst);
@@ -84,12 +86,16 @@ int tea_ws_trainer_step(void *trainer, const float *samples, float lr, double *l
 int tea_ws_trainer_sync(void *trainer);
 
 /* Training samples from dense device fields (ny x nx doubles each: density,
- * u^{n-1}, u^n, u^{n+1}, Kx, Ky): crop k of crop x crop cells at device
- * origins [2k] = x, [2k+1] = y goes to out + k*6*crop*crop; cells outside the
- * mesh are zero.  scales[0] = s, scales[1] = s_u (host).  Synchronous. */
-int tea_ws_make_samples(const double *rho, const double *u_prev, const double *u0, const double *u_next,
-                        const double *kx, const double *ky, int nx, int ny, const int *origins, int n, int crop,
-                        float *out, double *scales);
+ * u^{n-2}, u^{n-1}, u^n, u^{n+1}, Kx, Ky): crop k of crop x crop cells at
+ * device origins [2k] = x, [2k+1] = y goes to out + k*6*crop*crop; cells
+ * outside the mesh are zero.  The target plane is the part of the step that
+ * the guess u^n + c0 (u^n - u^{n-1}) + c1 (u^{n-1} - u^{n-2}) misses, over s:
+ * (u^{n+1} - u^n - c0 (u^n - u^{n-1}) - c1 (u^{n-1} - u^{n-2})) / s
+ * (c0 = 1, c1 = 0: the linear extrapolation).  scales[0] = s, scales[1] = s_u
+ * (host).  Synchronous. */
+int tea_ws_make_samples(const double *rho, const double *u_prev2, const double *u_prev, const double *u0,
+                        const double *u_next, const double *kx, const double *ky, int nx, int ny, double c0,
+                        double c1, const int *origins, int n, int crop, float *out, double *scales);
 
 #ifdef __cplusplus
 }

@@ -843,6 +843,27 @@ EXPORT void tea_engine_finalize(void) { free_all(); }
 
 /* ---------------- mixed precision (no reference counterpart) ---------------------------- */
 
+/* Fused PPCG outer iteration of the engine: the compositions it stands for */
+EXPORT void tea_engine_ppcg_begin(int with_p, double beta, double *pw_rro)
+{
+    if (with_p) tea_leaf_ppcg_calc_p_kernel_cuda_(&beta);
+    tea_leaf_cg_calc_w_kernel_cuda_(&pw_rro[0]);
+    tea_leaf_ppcg_store_r_kernel_cuda_();
+    tea_leaf_ppcg_calc_2norm_kernel_cuda_(&pw_rro[1]);
+}
+
+EXPORT void tea_engine_ppcg_ur(double alpha, double theta, double *rrn)
+{
+    tea_leaf_cg_calc_ur_kernel_cuda_(&alpha, rrn);
+    tea_leaf_ppcg_init_sd_new_kernel_cuda_(&theta);
+}
+
+EXPORT void tea_engine_ppcg_end(double *rrn)
+{
+    tea_leaf_ppcg_update_z_kernel_cuda_();
+    tea_leaf_ppcg_calc_rrn_kernel_cuda_(rrn);
+}
+
 EXPORT int tea_engine_set_precision(int bits)
 {
     if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return -1; }

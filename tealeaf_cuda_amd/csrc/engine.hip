@@ -45,6 +45,7 @@ enum FieldId {
     F_DENSITY = 1, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD, F_R,
     X_U0, X_W, X_Z, X_MI, X_KX, X_KY, X_RTEMP, X_UTEMP, X_RSTORE, X_CP, X_BFP, X_VOLUME, X_COUNT,
     X_SD2 = X_COUNT,    // second sd buffer of the fused inner step (internal)
+    X_P2,               // second p buffer of the fused outer iteration (internal)
     N_SLOTS
 };
 enum { SOLVER_JACOBI = 0, SOLVER_CG = 1, SOLVER_CHEBY = 2, SOLVER_PPCG = 3 };
@@ -157,6 +158,7 @@ struct Chunk {
     double *ws_slab = nullptr;
     double *ws_up1 = nullptr, *ws_up2 = nullptr;
     int ws_nprev = 0;             // previous u0 fields held (0..2)
+    double ws_coef[3] = {0, 0, 0};  // coefficients of the last guess
 
     Profiler prof;
 };
@@ -1230,6 +1232,47 @@ template <typename T> void ppcg_calc_p_impl(double beta)
     launch(nm<T>("ppcg_calc_p"), k_calc_p<T>, interior(), pitch_of<T>(), beta, FT<T>(F_P), (const T *)FT<T>(X_Z));
 }
 
+// Fused outer iteration (see k_ppcg_begin / k_ppcg_ur_sd / k_ppcg_end).
+template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
+{
+    const long pt = pitch_of<T>();
+    const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+    {
+        Timed t("p_halo_ring");
+        hipLaunchKernelGGL(k_copy_ring<T>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream,
+                           FT<T>(X_P2), (const T *)FT<T>(F_P), pt, g.h, g.nx, g.ny);
+    }
+    if (!launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
+                FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
+                (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx()))
+        return;
+    if (std::is_same<T, double>::value) {
+        std::swap(g.f[F_P], g.f[X_P2]);
+        std::swap(g.fs[F_P], g.fs[X_P2]);
+    } else {
+        std::swap(g.fs32[F_P], g.fs32[X_P2]);
+    }
+    fetch(out, 2);
+}
+
+template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
+{
+    if (!PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, theta, FT<T>(F_U),
+                       (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z), (const T *)FT<T>(X_MI),
+                       (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD), FT<T>(X_RTEMP),
+                       FT<T>(X_UTEMP), red_ctx()))
+        return;
+    fetch(rrn, 1);
+}
+
+template <typename T> void ppcg_end_impl(double *rrn)
+{
+    if (!launch(nm<T>("ppcg_end"), k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP), FT<T>(X_Z),
+                (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()))
+        return;
+    fetch(rrn, 1);
+}
+
 }  // namespace
 
 #define BY_PRECISION(fn, ...) (g.fp32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
@@ -1413,6 +1456,29 @@ EXPORT void tea_leaf_ppcg_calc_p_kernel_cuda_(double *beta)
 // =============================================================================
 // Extensions
 // =============================================================================
+
+// Fused PPCG outer iteration (no reference counterpart): the same arithmetic as
+// the entry points named in include/tea_leaf_engine.h, in fewer passes.
+EXPORT void tea_engine_ppcg_begin(int with_p, double beta, double *pw_rro)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    BY_PRECISION(ppcg_begin_impl, with_p, beta, pw_rro);
+}
+
+EXPORT void tea_engine_ppcg_ur(double alpha, double theta, double *rrn)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    BY_PRECISION(ppcg_ur_impl, alpha, theta, rrn);
+}
+
+EXPORT void tea_engine_ppcg_end(double *rrn)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    BY_PRECISION(ppcg_end_impl, rrn);
+}
 
 EXPORT void tea_engine_set_deck_path(const char *path) { g_deck_path = path ? path : "tea.in"; }
 EXPORT void tea_engine_set_error_mode(int mode) { g_error_mode = mode; }
@@ -1690,9 +1756,15 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
         for (int a = 0; a < k; a++) c[idx[a]] = cs[a];
         apply = 1;
     }
+    for (int a = 0; a < 3; a++) g.ws_coef[a] = c[a];
     if (!launch("ws_apply", k_ws_apply, in, g.pitch, F(F_U), d, g.ws_up1, g.ws_up2, c[0], c[1], c[2], apply)) return;
     g.ws_nprev = g.ws_nprev < 2 ? g.ws_nprev + 1 : 2;
     *applied = apply;
+}
+
+EXPORT void tea_engine_warm_start_coefficients(double *c)
+{
+    for (int a = 0; a < 3; a++) c[a] = g.ws_coef[a];
 }
 
 EXPORT int tea_engine_copy_field(int id, double *dst)

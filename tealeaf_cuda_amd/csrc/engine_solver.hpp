@@ -398,6 +398,119 @@ __global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const T *
     block_reduce<1>(acc, red);
 }
 
+// ---- fused PPCG outer iteration (no reference counterpart: each kernel below is
+// the composition of reference kernels named in its comment, with their
+// expressions unchanged, so results are bit-identical to the separate launches)
+
+// [ppcg_calc_p: p' = z + beta p] (with_p), cg_calc_w: w = A p', p'.w,
+// ppcg_store_r: rstore = r, ppcg_calc_2norm: r.z.  p' goes to the spare
+// buffer pn (the caller swaps it in): p is read at neighbours that other
+// lanes rewrite.  Cells outside the interior keep their p, as calc_p only
+// updates the interior.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int with_p, double beta_in,
+                                                      const T *__restrict__ p, T *__restrict__ pn,
+                                                      const T *__restrict__ z, T *__restrict__ w,
+                                                      const T *__restrict__ Kx, const T *__restrict__ Ky,
+                                                      const T *__restrict__ r, T *__restrict__ rstore, Red red)
+{
+    const T beta = (T)beta_in;
+    DD acc[2] = {{0, 0}, {0, 0}};
+    auto pv = [&](long j, int c, int rr) -> T {
+        const bool in = c >= e.x0 && c <= e.x1 && rr >= e.y0 && rr <= e.y1;
+        return (with_p && in) ? z[j] + beta * p[j] : p[j];
+    };
+    STRIP_BEGIN
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const int c = s.c, rr = s.r0 + k;
+            const T pc = pv(i, c, rr);
+            const T wv = (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * pc
+                         - (Ky[i + pitch] * pv(i + pitch, c, rr + 1) + Ky[i] * pv(i - pitch, c, rr - 1))
+                         - (Kx[i + 1] * pv(i + 1, c + 1, rr) + Kx[i] * pv(i - 1, c - 1, rr));
+            pn[i] = pc;
+            w[i] = wv;
+            const T rv = r[i];
+            rstore[i] = rv;
+            dd_add_prod(acc[0], (double)pc, (double)wv);
+            dd_add_prod(acc[1], (double)rv, (double)z[i]);
+        }
+    }
+    STRIP_END
+    block_reduce<2>(acc, red);
+}
+
+// cg_calc_ur (u += a p; r -= a w; z = M^-1 r; r.z or r.r) then
+// ppcg_init_sd_new (sd = (z or r) / theta; rtemp = r; utemp = sd).
+template <typename T, int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd(Ext e, long pitch, double alpha_in, double theta_in,
+                                                      T *__restrict__ u, const T *__restrict__ p,
+                                                      T *__restrict__ r, const T *__restrict__ w,
+                                                      T *__restrict__ z, const T *__restrict__ Mi,
+                                                      const T *__restrict__ Kx, const T *__restrict__ Ky,
+                                                      T *__restrict__ sd, T *__restrict__ rtemp,
+                                                      T *__restrict__ utemp, Red red)
+{
+    const T alpha = (T)alpha_in;
+    const T theta_r = T(1.0) / (T)theta_in;
+    DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
+    T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            u[i] += alpha * p[i];
+            rv[k] = r[i] - alpha * w[i];
+            r[i] = rv[k];
+        }
+    }
+    if (PREC != PREC_NONE) precond_strip<T, PREC>(rv, zv, s.len, i0, pitch, Mi, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            T src;
+            if (PREC != PREC_NONE) {
+                z[i] = zv[k];
+                dd_add_prod(acc[0], (double)rv[k], (double)zv[k]);
+                src = zv[k];
+            } else {
+                dd_add_prod(acc[0], (double)rv[k], (double)rv[k]);
+                src = rv[k];
+            }
+            const T sdv = src * theta_r;
+            sd[i] = sdv;
+            rtemp[i] = rv[k];
+            utemp[i] = sdv;
+        }
+    }
+    STRIP_END
+    block_reduce<1>(acc, red);
+}
+
+// ppcg_update_z (z = utemp) then ppcg_calc_rrn ((r - rstore).z).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_end(Ext e, long pitch, const T *__restrict__ utemp,
+                                                    T *__restrict__ z, const T *__restrict__ r,
+                                                    const T *__restrict__ rstore, Red red)
+{
+    DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const T zv = utemp[i];
+            z[i] = zv;
+            dd_add_prod(acc[0], (double)(r[i] - rstore[i]), (double)zv);
+        }
+    STRIP_END
+    block_reduce<1>(acc, red);
+}
+
 // ---- mixed precision glue (no reference counterpart) -------------------------------
 
 // FP32 copies of the FP64 face coefficients over the whole array (halo included).

@@ -88,6 +88,9 @@ struct Backend {
     void (*mp_end)(void) = nullptr;
     int (*warm_start_load)(const float *, int) = nullptr;
     void (*warm_start_apply)(int *) = nullptr;
+    void (*ppcg_begin)(int, double, double *) = nullptr;
+    void (*ppcg_ur)(double, double, double *) = nullptr;
+    void (*ppcg_end)(double *) = nullptr;
 
     void check() const
     {
@@ -120,6 +123,9 @@ Backend load_backend(const char *path)
     b.mp_end = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_end"));
     b.warm_start_load = reinterpret_cast<int (*)(const float *, int)>(dlsym(b.handle, "tea_engine_warm_start_load"));
     b.warm_start_apply = reinterpret_cast<void (*)(int *)>(dlsym(b.handle, "tea_engine_warm_start_apply"));
+    b.ppcg_begin = reinterpret_cast<void (*)(int, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_begin"));
+    b.ppcg_ur = reinterpret_cast<void (*)(double, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_ur"));
+    b.ppcg_end = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_ppcg_end"));
     return b;
 }
 
@@ -553,8 +559,15 @@ private:
         int updates = 0;
         double rr_final = 0.0;
     } rel;
-    bool reliable_step(double q, bool refresh_z);
-    void run_ppcg_inner_steps(int *ppcg_inner_iters);
+    bool reliable_step(double q, bool refresh_z, bool *replaced);
+    void run_ppcg_inner_steps(int *ppcg_inner_iters, bool fused = false);
+    // the fused outer iteration (tea_engine_ppcg_begin/ur/end) is used when the
+    // backend has it and no halo update separates the fused calls
+    bool fused_outer() const
+    {
+        const char *env = std::getenv("TEA_FUSED_OUTER");
+        return b.ppcg_begin && b.ppcg_ur && b.ppcg_end && !d.reflective_boundary && !(env && env[0] == '0');
+    }
     void log(const char *fmt, ...);
 };
 
@@ -694,15 +707,19 @@ void Run::load_warm_start()
 }
 
 // tea_leaf_ppcg.f90:174-230
-void Run::run_ppcg_inner_steps(int *ppcg_inner_iters)
+// fused: sd/rtemp/utemp were set by tea_engine_ppcg_ur and z = utemp comes
+// with tea_engine_ppcg_end (no reflective halo update to interleave)
+void Run::run_ppcg_inner_steps(int *ppcg_inner_iters, bool fused)
 {
     int fields[NUM_FIELDS] = {0};
     fields[FIELD_U - 1] = 1;
     update_halo(fields, 1);
     if (d.tl_ppcg_inner_steps < 0) return;
     double th = theta;
-    b.tea_leaf_ppcg_init_sd_new_kernel_cuda_(&th);
-    b.check();
+    if (!fused) {
+        b.tea_leaf_ppcg_init_sd_new_kernel_cuda_(&th);
+        b.check();
+    }
     const int h = d.halo_exchange_depth;
     for (int cur = 1; cur <= d.tl_ppcg_inner_steps; cur += h) {
         int f[NUM_FIELDS] = {0};
@@ -717,8 +734,10 @@ void Run::run_ppcg_inner_steps(int *ppcg_inner_iters)
             if (inner_step == d.tl_ppcg_inner_steps) break;
         }
     }
-    b.tea_leaf_ppcg_update_z_kernel_cuda_();
-    b.check();
+    if (!fused) {
+        b.tea_leaf_ppcg_update_z_kernel_cuda_();
+        b.check();
+    }
     (void)ppcg_inner_iters;
 }
 
@@ -840,6 +859,9 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
     std::vector<double> ch_alphas(max_iters + 2, 0.0), ch_betas(max_iters + 2, 0.0);
     int est_itc = 0, cheby_calc_steps = 0, max_cheby_iters = 0, ppcg_inner_iters = 0;
     bool ch_switch_check = false, converged = false;
+    const bool fused = d.tl_use_ppcg && fused_outer();
+    bool p_pending = false;       // fused: ppcg_calc_p(beta_pending) still to apply
+    double beta_pending = 0.0;
     int fields[NUM_FIELDS];
     std::fill(fields, fields + NUM_FIELDS, 0);
 
@@ -943,6 +965,37 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                         b.check();
                     }
                 }
+            } else if (d.tl_use_ppcg && fused) {
+                // the same calls as below, fused; ppcg_calc_p of the previous
+                // iteration is folded into this iteration's first pass
+                double two[2];
+                b.ppcg_begin(p_pending ? 1 : 0, beta_pending, two);
+                b.check();
+                p_pending = false;
+                pw = two[0];
+                rro = two[1];
+                alpha = rro / pw;
+                b.ppcg_ur(alpha, theta, &rrn);
+                b.check();
+                bool replaced = false;
+                if (rel.on && reliable_step(rrn, true, &replaced)) {
+                    converged = true;
+                } else {
+                    if (replaced) {  // sd, rtemp, utemp from the replaced r and z
+                        double th = theta;
+                        b.tea_leaf_ppcg_init_sd_new_kernel_cuda_(&th);
+                        b.check();
+                    }
+                    run_ppcg_inner_steps(&ppcg_inner_iters, true);
+                    ppcg_inner_iters += d.tl_ppcg_inner_steps;
+                    b.ppcg_end(&rrn);
+                    b.check();
+                    beta = rrn / rro;
+                    p_pending = true;
+                    beta_pending = beta;
+                }
+                error = rrn;
+                rro = rrn;
             } else if (d.tl_use_ppcg) {
                 b.tea_leaf_cg_calc_w_kernel_cuda_(&pw);
                 b.check();
@@ -953,7 +1006,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                 alpha = rro / pw;
                 b.tea_leaf_cg_calc_ur_kernel_cuda_(&alpha, &rrn);
                 b.check();
-                if (rel.on && reliable_step(rrn, true)) {
+                if (rel.on && reliable_step(rrn, true, nullptr)) {
                     converged = true;
                 } else {
                     run_ppcg_inner_steps(&ppcg_inner_iters);
@@ -981,7 +1034,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
             b.tea_leaf_cg_calc_p_kernel_cuda_(&beta);
             b.check();
             // a replaced r reaches z at the next cg_calc_ur
-            if (rel.on && reliable_step(rrn, false)) converged = true;
+            if (rel.on && reliable_step(rrn, false, nullptr)) converged = true;
             error = rrn;
             rro = rrn;
         } else if (d.tl_use_jacobi) {
@@ -1118,8 +1171,9 @@ Run::Solve Run::refine_reliable(double initial_residual_sq, double target_residu
     return Solve{s.n, std::sqrt(std::fabs(rel.rr_final)), ir, s.ppcg_inner};
 }
 
-bool Run::reliable_step(double q, bool refresh_z)
+bool Run::reliable_step(double q, bool refresh_z, bool *replaced)
 {
+    if (replaced) *replaced = false;
     if (!rel.have_q) {  // the first measure after a replacement is the reference level
         rel.q_ref = q;
         rel.have_q = true;
@@ -1147,10 +1201,13 @@ bool Run::reliable_step(double q, bool refresh_z)
         rel.converged = true;
         return true;
     }
-    // The true residual came out more than 4x above what the loop's own
-    // measure predicted: the recurrence no longer tracks it.  Three such
-    // replacements in a row mean the target lies below what FP64 resolves.
-    rel.stalls = std::sqrt(std::fabs(rr)) > 4.0 * predicted ? rel.stalls + 1 : 0;
+    // A replacement stalls when the true residual came out more than 4x above
+    // what the loop's own measure predicted (the recurrence no longer tracks
+    // it) or fell by less than 1% since the previous replacement.  Three in a
+    // row mean the target lies below what FP64 resolves.
+    const double now = std::sqrt(std::fabs(rr));
+    const bool stall = now > 4.0 * predicted || now > 0.99 * std::sqrt(std::fabs(rel.rr_ref));
+    rel.stalls = stall ? rel.stalls + 1 : 0;
     if (rel.stalls >= 3) {
         rel.converged = true;
         rel.stagnated = true;
@@ -1171,6 +1228,7 @@ bool Run::reliable_step(double q, bool refresh_z)
         b.tea_leaf_ppcg_init_kernel_cuda_(&st, &unused);  // z = M^-1 r for the new r
         b.check();
     }
+    if (replaced) *replaced = true;
     rel.rr_ref = rr;
     rel.q_ref = q;
     return false;

@@ -361,12 +361,13 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 //   0: (log10 rho - 0.5) / 1.5
 //   1: (u0 - u_prev) / s
 //   2: u0 / s_u
-//   3: target ((u_next - u0) - (u0 - u_prev)) / s
+//   3: target ((u_next - u0) - c0 (u0 - u_prev) - c1 (u_prev - u_prev2)) / s
 //   4, 5: Kx (left face), Ky (bottom face) of the timestep operator
-__global__ void k_make_samples(const double *__restrict__ rho, const double *__restrict__ u_prev,
-                               const double *__restrict__ u0, const double *__restrict__ u_next,
-                               const double *__restrict__ kx, const double *__restrict__ ky, int nx, int ny,
-                               double inv_s, double inv_su, const int *__restrict__ origins, int n, int crop,
+__global__ void k_make_samples(const double *__restrict__ rho, const double *__restrict__ u_prev2,
+                               const double *__restrict__ u_prev, const double *__restrict__ u0,
+                               const double *__restrict__ u_next, const double *__restrict__ kx,
+                               const double *__restrict__ ky, int nx, int ny, double c0, double c1, double inv_s,
+                               double inv_su, const int *__restrict__ origins, int n, int crop,
                                float *__restrict__ out)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
@@ -386,7 +387,7 @@ __global__ void k_make_samples(const double *__restrict__ rho, const double *__r
     o[0] = (float)((log10(rho[i]) - 0.5) / 1.5);
     o[per] = (float)(dprev * inv_s);
     o[2 * per] = (float)(u0[i] * inv_su);
-    o[3 * per] = (float)(((u_next[i] - u0[i]) - dprev) * inv_s);
+    o[3 * per] = (float)(((u_next[i] - u0[i]) - c0 * dprev - c1 * (u_prev[i] - u_prev2[i])) * inv_s);
     o[4 * per] = (float)kx[i];
     o[5 * per] = (float)ky[i];
 }
@@ -739,12 +740,12 @@ EXPORT int tea_ws_trainer_sync(void *h)
     return 0;
 }
 
-EXPORT int tea_ws_make_samples(const double *rho, const double *u_prev, const double *u0, const double *u_next,
-                               const double *kx, const double *ky, int nx, int ny, const int *origins, int n,
-                               int crop, float *out, double *scales)
+EXPORT int tea_ws_make_samples(const double *rho, const double *u_prev2, const double *u_prev, const double *u0,
+                               const double *u_next, const double *kx, const double *ky, int nx, int ny, double c0,
+                               double c1, const int *origins, int n, int crop, float *out, double *scales)
 {
-    if (!rho || !u_prev || !u0 || !u_next || !kx || !ky || !origins || !out || !scales || nx < 1 || ny < 1 ||
-        n < 1 || crop < 1)
+    if (!rho || !u_prev2 || !u_prev || !u0 || !u_next || !kx || !ky || !origins || !out || !scales || nx < 1 ||
+        ny < 1 || n < 1 || crop < 1)
         return ws_fail("tea_ws_make_samples: bad argument");
     const long cells = (long)nx * ny;
     const int blocks = 1024;
@@ -764,8 +765,9 @@ EXPORT int tea_ws_make_samples(const double *rho, const double *u_prev, const do
     scales[1] = std::sqrt(c / (double)cells);
     if (!(scales[0] > 0.0) || !(scales[1] > 0.0)) return ws_fail("tea_ws_make_samples: zero field scale");
     const long total = (long)n * crop * crop;
-    hipLaunchKernelGGL(ws::k_make_samples, dim3(ws::nblocks(total)), dim3(ws::BLOCK), 0, nullptr, rho, u_prev, u0,
-                       u_next, kx, ky, nx, ny, 1.0 / scales[0], 1.0 / scales[1], origins, n, crop, out);
+    hipLaunchKernelGGL(ws::k_make_samples, dim3(ws::nblocks(total)), dim3(ws::BLOCK), 0, nullptr, rho, u_prev2,
+                       u_prev, u0, u_next, kx, ky, nx, ny, c0, c1, 1.0 / scales[0], 1.0 / scales[1], origins, n, crop,
+                       out);
     WS_OK(hipGetLastError());
     WS_OK(hipDeviceSynchronize());
     return 0;

@@ -103,6 +103,9 @@ _EXT = {
     "tea_engine_stream": ([], ctypes.c_void_p),
     "tea_engine_synchronize": ([], None),
     "tea_engine_finalize": ([], None),
+    "tea_engine_ppcg_begin": ([ctypes.c_int, ctypes.c_double, _D], None),
+    "tea_engine_ppcg_ur": ([ctypes.c_double, ctypes.c_double, _D], None),
+    "tea_engine_ppcg_end": ([_D], None),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
     "tea_engine_mp_begin": ([_D], None),
@@ -319,6 +322,22 @@ class Engine:
             self.lib.tea_engine_profile_get(name.encode(), ctypes.byref(c), ctypes.byref(ms))
             out[name] = (c.value, ms.value)
         return out
+
+    # -- fused PPCG outer iteration
+    def ppcg_begin(self, with_p: bool, beta: float):
+        out = (ctypes.c_double * 2)()
+        self._call("tea_engine_ppcg_begin", int(bool(with_p)), float(beta), out)
+        return out[0], out[1]
+
+    def ppcg_ur(self, alpha: float, theta: float) -> float:
+        out = ctypes.c_double()
+        self._call("tea_engine_ppcg_ur", float(alpha), float(theta), ctypes.byref(out))
+        return out.value
+
+    def ppcg_end(self) -> float:
+        out = ctypes.c_double()
+        self._call("tea_engine_ppcg_end", ctypes.byref(out))
+        return out.value
 
     # -- mixed precision
     def set_precision(self, bits: int):

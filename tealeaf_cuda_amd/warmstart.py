@@ -44,6 +44,7 @@ _WS = {
     "tea_engine_warm_start_load": ([_F, ctypes.c_int], ctypes.c_int),
     "tea_engine_warm_start_apply": ([_I], None),
     "tea_engine_warm_start_reset": ([], None),
+    "tea_engine_warm_start_coefficients": ([_D], None),
     "tea_engine_copy_field": ([ctypes.c_int, _P], ctypes.c_int),
     "tea_ws_trainer_create": ([ctypes.c_int] * 5, _P),
     "tea_ws_trainer_destroy": ([_P], None),
@@ -53,8 +54,8 @@ _WS = {
     "tea_ws_trainer_gradient": ([_P, _P, _F, _D], ctypes.c_int),
     "tea_ws_trainer_step": ([_P, _P, ctypes.c_float, _D], ctypes.c_int),
     "tea_ws_trainer_sync": ([_P], ctypes.c_int),
-    "tea_ws_make_samples": ([_P] * 6 + [ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P, _D],
-                            ctypes.c_int),
+    "tea_ws_make_samples": ([_P] * 7 + [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P,
+                                        ctypes.c_int, ctypes.c_int, _P, _D], ctypes.c_int),
 }
 WARMSTART_SYMBOLS = tuple(_WS)
 NPLANES = 6                 # training sample planes: f0, f1, f2, target, Kx, Ky
@@ -207,19 +208,26 @@ class GenStats:
 
 
 def generate(seeds, n: int, steps: int, crops_per_step: int, store: SampleStore, tmpdir, rng=None,
-             log=print) -> GenStats:
-    """Runs the plain solver over `seeds` and fills `store` with crops."""
+             log=print, residual_of: str = "galerkin") -> GenStats:
+    """Runs the solver over `seeds` and fills `store` with crops.  The target
+    is the part of each step that a guess misses: residual_of "extrapolation"
+    (u^n + (u^n - u^{n-1})) or "galerkin" (the engine's energy-minimising
+    combination of the previous-step differences, which the solves then also
+    start from; the solution is the same to the solver tolerance)."""
     lib = library()
     rng = rng or np.random.default_rng(0)
     st = GenStats()
     dev = store.device
-    bufs = {k: torch.empty((n, n), dtype=torch.float64, device=dev) for k in ("rho", "prev", "u0", "u", "kx", "ky")}
+    bufs = {k: torch.empty((n, n), dtype=torch.float64, device=dev)
+            for k in ("rho", "prev2", "prev", "u0", "u", "kx", "ky")}
+    coef = (ctypes.c_double * 3)()
+    extra = {"warm_start": "extrapolate"} if residual_of == "galerkin" else {}
     origins = torch.empty(2 * crops_per_step, dtype=torch.int32, device=dev)
     scales = (ctypes.c_double * 2)()
     crop = store.crop
     for k, seed in enumerate(seeds):
-        deck = solver_deck(n, seed, steps).write(Path(tmpdir) / "gen.in")
-        state = {"have_prev": False}
+        deck = solver_deck(n, seed, steps, **extra).write(Path(tmpdir) / "gen.in")
+        state = {"nprev": 0}
 
         def on_step(step):
             for fid, key in ((FIELD_DENSITY, "rho"), (XFIELD_U0, "u0"), (FIELD_U, "u"), (XFIELD_KX, "kx"),
@@ -228,21 +236,29 @@ def generate(seeds, n: int, steps: int, crops_per_step: int, store: SampleStore,
                     continue
                 if lib.tea_engine_copy_field(fid, ctypes.c_void_p(bufs[key].data_ptr())) != 0:
                     raise TeaError("tea_engine_copy_field failed")
-            if state["have_prev"] and store.count < store.capacity:
+            if state["nprev"] >= 1 and store.count < store.capacity:
+                if residual_of == "galerkin":
+                    lib.tea_engine_warm_start_coefficients(coef)
+                    c0, c1 = coef[0], coef[1]
+                else:
+                    c0, c1 = 1.0, 0.0
+                if state["nprev"] < 2:
+                    c1 = 0.0
                 m = min(crops_per_step, store.capacity - store.count)
                 o = rng.integers(0, n - crop + 1, size=(m, 2)).astype(np.int32)
                 origins[:2 * m].copy_(torch.from_numpy(o.ravel()))
                 out = store.data[store.count:store.count + m]
                 rc = lib.tea_ws_make_samples(*(ctypes.c_void_p(bufs[x].data_ptr())
-                                               for x in ("rho", "prev", "u0", "u", "kx", "ky")),
-                                             n, n, ctypes.c_void_p(origins.data_ptr()), m, crop,
+                                               for x in ("rho", "prev2", "prev", "u0", "u", "kx", "ky")),
+                                             n, n, c0, c1, ctypes.c_void_p(origins.data_ptr()), m, crop,
                                              ctypes.c_void_p(out.data_ptr()), scales)
                 if rc == 0:
                     store.count += m
                     st.samples += m
+            bufs["prev2"].copy_(bufs["prev"] if state["nprev"] >= 1 else bufs["u0"])
             bufs["prev"].copy_(bufs["u0"])
             torch.cuda.synchronize()
-            state["have_prev"] = True
+            state["nprev"] += 1
 
         t0 = time.perf_counter()
         res = run_deck(deck, on_step=on_step)
@@ -337,6 +353,8 @@ def main(argv=None):
     tr.add_argument("--iters", type=int, default=20000)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--loss", choices=sorted(LOSSES), default="energy")
+    tr.add_argument("--target", choices=["galerkin", "extrapolation"], default="galerkin",
+                    help="the guess whose remaining error the network learns")
     tr.add_argument("--init-seed", type=int, default=0)
     tr.add_argument("--out", required=True)
     ev = sub.add_parser("evaluate", help="iterations to 1e-10 with and without the warm start")
@@ -353,7 +371,8 @@ def main(argv=None):
             seeds = _range(a.seeds)
             store = SampleStore(capacity=len(seeds) * (a.steps - 1) * a.crops, crop=a.crop)
             t0 = time.perf_counter()
-            st = generate(seeds, a.size, a.steps, a.crops, store, tmp, rng=np.random.default_rng(a.init_seed), log=log)
+            st = generate(seeds, a.size, a.steps, a.crops, store, tmp, rng=np.random.default_rng(a.init_seed), log=log,
+                          residual_of=a.target)
             gen_s = time.perf_counter() - t0
             trainer = Trainer(a.batch, a.crop, a.crop, margin=6, loss=a.loss)
             init = _initial_params(a.init_seed)
@@ -367,7 +386,7 @@ def main(argv=None):
             print(json.dumps({"weights": str(a.out), "size": a.size, "seeds": [seeds.start, seeds.stop],
                               "steps": a.steps, "samples": store.count, "solver_steps": st.steps,
                               "data_seconds": round(gen_s, 1), "train_iters": a.iters, "batch": a.batch,
-                              "train_seconds": round(train_s, 1), "loss": a.loss,
+                              "train_seconds": round(train_s, 1), "loss": a.loss, "target": a.target,
                               "loss_history": [(s, float(l)) for s, l in hist]}))
         else:
             variants = {"plain": None, "extrapolate": "extrapolate", "network": str(Path(a.weights).resolve())}

@@ -92,6 +92,22 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
         assert a.values() == b.values(), (a, b)
 
 
+@pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg"])
+def test_deck_bitwise_unfused_outer(name, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
+    """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0) on the GPU
+    gives the same results as the oracle."""
+    oracle_env("dot2")
+    monkeypatch.setenv("TEA_FUSED_OUTER", "0")
+    print(f"[gpu parity] unfused deck {name}", flush=True)
+    path = _deck(**DECKS[name]).write(tmp_path / "tea.in")
+    gpu = run_deck(path)
+    cpu = run_deck(path, backend=oracle_so)
+    assert gpu.iterations == cpu.iterations and gpu.refinements == cpu.refinements
+    assert gpu.exact_error == cpu.exact_error
+    for a, b in zip(gpu.summaries, cpu.summaries):
+        assert a.values() == b.values(), (a, b)
+
+
 # ------------------------------------------------------------------ entry points
 
 def _prepare(eng, tmp_path, solver, prec, nx=70, ny=45, halo=2):
@@ -146,6 +162,10 @@ PPCG_CALLS = {
     "ppcg_calc_rrn": lambda e: e.ppcg_calc_rrn(),
     "ppcg_calc_2norm": lambda e: e.ppcg_calc_2norm(),
     "ppcg_calc_p": lambda e: e.ppcg_calc_p(0.45),
+    "fused_ppcg_begin_p": lambda e: e.ppcg_begin(True, 0.45),
+    "fused_ppcg_begin": lambda e: e.ppcg_begin(False, 0.0),
+    "fused_ppcg_ur": lambda e: e.ppcg_ur(0.37, 1.7),
+    "fused_ppcg_end": lambda e: e.ppcg_end(),
 }
 CHEBY_CALLS = {
     "cheby_init": lambda e: e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3),

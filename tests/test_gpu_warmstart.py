@@ -201,3 +201,38 @@ def test_warm_start_deck_reaches_the_same_target(tmp_path):
         assert r.warm_start == [0, 1, 1, 1]
         assert all(0 < e < 1e-10 for e in r.exact_error), r.exact_error
         assert "Warm start" in (tmp_path / f"ws_{prec}.out").read_text()
+
+
+def test_make_samples_matches_numpy():
+    """Training crops: features, the target of the guess with coefficients
+    (c0, c1), and the face coefficients; cells outside the mesh are zero."""
+    import ctypes
+
+    print("[gpu warmstart] make_samples", flush=True)
+    lib = wsm.library()
+    rng = np.random.default_rng(11)
+    nx, ny, crop = 96, 80, 32
+    f = {k: rng.uniform(0.5, 2.0, (ny, nx)) for k in ("rho", "p2", "p1", "u0", "un", "kx", "ky")}
+    dev = {k: torch.from_numpy(v).cuda() for k, v in f.items()}
+    origins = np.array([[0, 0], [40, 30], [80, 60]], dtype=np.int32)
+    od = torch.from_numpy(origins.ravel()).cuda()
+    out = torch.zeros((3, wsm.NPLANES, crop, crop), dtype=torch.float32, device="cuda")
+    scales = (ctypes.c_double * 2)()
+    c0, c1 = 0.9, 0.2
+    rc = lib.tea_ws_make_samples(*(ctypes.c_void_p(dev[k].data_ptr()) for k in ("rho", "p2", "p1", "u0", "un",
+                                                                                  "kx", "ky")),
+                                 nx, ny, c0, c1, ctypes.c_void_p(od.data_ptr()), 3, crop,
+                                 ctypes.c_void_p(out.data_ptr()), scales)
+    assert rc == 0
+    d = f["u0"] - f["p1"]
+    s, su = np.sqrt(np.mean(d * d)), np.sqrt(np.mean(f["u0"] ** 2))
+    assert scales[0] == pytest.approx(s, rel=1e-12) and scales[1] == pytest.approx(su, rel=1e-12)
+    planes = [(np.log10(f["rho"]) - 0.5) / 1.5, d / s, f["u0"] / su,
+              ((f["un"] - f["u0"]) - c0 * d - c1 * (f["p1"] - f["p2"])) / s, f["kx"], f["ky"]]
+    got = out.cpu().numpy()
+    for k, (ox, oy) in enumerate(origins):
+        for q, pl in enumerate(planes):
+            ref = np.zeros((crop, crop))
+            w, h = min(crop, nx - ox), min(crop, ny - oy)
+            ref[:h, :w] = pl[oy:oy + h, ox:ox + w]
+            np.testing.assert_allclose(got[k, q], ref.astype(np.float32), rtol=2e-7, atol=1e-7, err_msg=f"{k} {q}")

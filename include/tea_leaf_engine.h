@@ -226,6 +226,10 @@ void tea_engine_finalize(void);
 void tea_engine_ppcg_begin(int with_p, double beta, double *pw_rro);
 void tea_engine_ppcg_ur(double alpha, double theta, double *rrn);
 void tea_engine_ppcg_end(double *rrn);
+/* Inner steps `step` and `step + 1` (1-based) in one pass over HBM: the same
+ * results as tea_leaf_ppcg_inner_kernel_cuda_ called for both steps with
+ * bounds_extra 0 and all four faces external (one chunk). */
+void tea_engine_ppcg_inner2(int step);
 
 /* ---- mixed precision: FP32 inner solver under FP64 refinement ------------ *
  * tea_engine_set_precision(32) switches the CG and PPCG entry points

@@ -864,6 +864,14 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
     tea_leaf_ppcg_calc_rrn_kernel_cuda_(rrn);
 }
 
+EXPORT void tea_engine_ppcg_inner2(int step)
+{
+    int neighbours[4] = {EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE};
+    int be = 0, s1 = step, s2 = step + 1;
+    tea_leaf_ppcg_inner_kernel_cuda_(&s1, &be, neighbours);
+    tea_leaf_ppcg_inner_kernel_cuda_(&s2, &be, neighbours);
+}
+
 EXPORT int tea_engine_set_precision(int bits)
 {
     if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return -1; }

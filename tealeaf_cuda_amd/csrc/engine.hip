@@ -46,6 +46,7 @@ enum FieldId {
     X_U0, X_W, X_Z, X_MI, X_KX, X_KY, X_RTEMP, X_UTEMP, X_RSTORE, X_CP, X_BFP, X_VOLUME, X_COUNT,
     X_SD2 = X_COUNT,    // second sd buffer of the fused inner step (internal)
     X_P2,               // second p buffer of the fused outer iteration (internal)
+    X_RT2,              // second rtemp buffer of the two-step inner kernel (internal)
     N_SLOTS
 };
 enum { SOLVER_JACOBI = 0, SOLVER_CG = 1, SOLVER_CHEBY = 2, SOLVER_PPCG = 3 };
@@ -147,6 +148,10 @@ struct Chunk {
     // across before the next fused step.
     bool sd_halo_dirty = false;
     bool sd32_halo_dirty = false;  // the same for the FP32 shadow (set_field32)
+    // The same rule for the spare p and rtemp buffers of the fused outer
+    // iteration and the two-step inner kernel (index 0: FP64, 1: FP32).
+    bool p_halo_dirty[2] = {true, true};
+    bool rt_halo_dirty[2] = {true, true};
 
     // learned warm start: network, inference buffers, u0 of the two previous
     // steps (field layout: the same flat index as the fields)
@@ -958,6 +963,7 @@ EXPORT void update_halo_kernel_cuda_(const int *chunk_neighbours, const int *fie
     const int d = *depth;
     if (d < 1 || d > g.h) { die("update_halo: invalid depth %d (halo depth %d)", d, g.h); return; }
     if (fields[TEA_FIELD_SD - 1] == 1) g.sd_halo_dirty = true;
+    if (fields[TEA_FIELD_P - 1] == 1) g.p_halo_dirty[0] = true;
     for (int f = 1; f <= TEA_NUM_FIELDS; f++) {
         if (fields[f - 1] != 1) continue;
         double *a = F(f);
@@ -1003,6 +1009,7 @@ void pack_unpack(int *fields, int *offsets, int d, int face, int pack, double *h
     if (n == 0) return;
     if ((size_t)end > g.halo_buf_elems) { die("pack: buffer of %ld elements exceeds staging", end); return; }
     if (!pack && fields[TEA_FIELD_SD - 1]) g.sd_halo_dirty = true;
+    if (!pack && fields[TEA_FIELD_P - 1]) g.p_halo_dirty[0] = true;
     if (!pack) HIP_OK(hipMemcpyAsync(g.d_halo_buf, host_buffer, end * sizeof(double), hipMemcpyHostToDevice, g.stream));
     const int len = lr ? g.ny + 2 * d : g.nx + 2 * d;
     for (int f = 0; f < TEA_NUM_FIELDS; f++) {
@@ -1201,6 +1208,69 @@ template <typename T> void ppcg_inner_impl(const Ext &e, double alpha, double be
                   beta);
 }
 
+// Two inner steps in one pass (k_ppcg_inner2) on the interior.  Waves per
+// workgroup (tile height): measured fastest at 4096^2, 16 for FP32 (99 vs
+// 117 us per step with 10) and 10 for FP64 (190 vs 200 us with 16).
+template <typename T> constexpr int inner2_waves() { return std::is_same<T, float>::value ? 16 : 10; }
+
+template <typename T> void ppcg_inner2_impl(double a1, double b1, double a2, double b2)
+{
+    constexpr int INNER2_WAVES = inner2_waves<T>();
+    const long pt = pitch_of<T>();
+    const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+    const unsigned ring_blocks = (unsigned)((ring + 255) / 256);
+    bool &dirty = std::is_same<T, double>::value ? g.sd_halo_dirty : g.sd32_halo_dirty;
+    bool &rt_dirty = g.rt_halo_dirty[std::is_same<T, double>::value ? 0 : 1];
+    if (dirty || rt_dirty) {
+        Timed t("inner2_halo_rings");
+        if (dirty) {
+            hipLaunchKernelGGL(k_copy_ring<T>, dim3(ring_blocks), dim3(256), 0, g.stream, FT<T>(X_SD2),
+                               (const T *)FT<T>(F_SD), pt, g.h, g.nx, g.ny);
+            dirty = false;
+        }
+        if (rt_dirty) {
+            hipLaunchKernelGGL(k_copy_ring<T>, dim3(ring_blocks), dim3(256), 0, g.stream, FT<T>(X_RT2),
+                               (const T *)FT<T>(X_RTEMP), pt, g.h, g.nx, g.ny);
+            rt_dirty = false;
+        }
+    }
+    const Ext e = interior();
+    // FP32: two columns per lane (k_ppcg_inner2_x2), FP64: one
+    constexpr bool X2 = std::is_same<T, float>::value;
+    constexpr int OUT_X = X2 ? 2 * WAVE - 4 : WAVE - 2, OUT_Y = (INNER2_WAVES - 2) * STRIP;
+    const long ntiles = (long)((e.x1 - e.x0 + OUT_X) / OUT_X) * ((e.y1 - e.y0 + OUT_Y) / OUT_Y);
+    const unsigned grid = (unsigned)std::min<long>(ntiles, grid_max());
+    {
+        Timed t(nm<T>("k_ppcg_inner2"));
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(INNER2_WAVES * WAVE), 0, g.stream, e, pt,
+                               (const T *)FT<T>(F_SD), FT<T>(X_SD2), (const T *)FT<T>(X_RTEMP), FT<T>(X_RT2),
+                               FT<T>(X_UTEMP), (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX),
+                               (const T *)FT<T>(X_KY), a1, b1, a2, b2);
+        };
+        if constexpr (X2) {
+            if (g.prec == PREC_BLOCK) go(k_ppcg_inner2_x2<PREC_BLOCK, INNER2_WAVES>);
+            else if (g.prec == PREC_DIAG) go(k_ppcg_inner2_x2<PREC_DIAG, INNER2_WAVES>);
+            else go(k_ppcg_inner2_x2<PREC_NONE, INNER2_WAVES>);
+        } else {
+            if (g.prec == PREC_BLOCK) go(k_ppcg_inner2<T, PREC_BLOCK, INNER2_WAVES>);
+            else if (g.prec == PREC_DIAG) go(k_ppcg_inner2<T, PREC_DIAG, INNER2_WAVES>);
+            else go(k_ppcg_inner2<T, PREC_NONE, INNER2_WAVES>);
+        }
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) { die("launch of k_ppcg_inner2 failed: %s", hipGetErrorString(err)); return; }
+    if (std::is_same<T, double>::value) {
+        std::swap(g.f[F_SD], g.f[X_SD2]);
+        std::swap(g.fs[F_SD], g.fs[X_SD2]);
+        std::swap(g.f[X_RTEMP], g.f[X_RT2]);
+        std::swap(g.fs[X_RTEMP], g.fs[X_RT2]);
+    } else {
+        std::swap(g.fs32[F_SD], g.fs32[X_SD2]);
+        std::swap(g.fs32[X_RTEMP], g.fs32[X_RT2]);
+    }
+}
+
 template <typename T> void ppcg_store_r_impl()
 {
     launch(nm<T>("ppcg_store_r"), k_copy<T>, interior(), pitch_of<T>(), FT<T>(X_RSTORE), (const T *)FT<T>(F_R));
@@ -1237,10 +1307,12 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
 {
     const long pt = pitch_of<T>();
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
-    {
+    bool &dirty = g.p_halo_dirty[std::is_same<T, double>::value ? 0 : 1];
+    if (dirty) {  // the spare's halo ring takes p's (calc_p leaves the halo as it is)
         Timed t("p_halo_ring");
         hipLaunchKernelGGL(k_copy_ring<T>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream,
                            FT<T>(X_P2), (const T *)FT<T>(F_P), pt, g.h, g.nx, g.ny);
+        dirty = false;
     }
     if (!launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
                 FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
@@ -1480,6 +1552,21 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
     BY_PRECISION(ppcg_end_impl, rrn);
 }
 
+// Inner steps `step` and `step + 1` (1-based) over the interior in one pass:
+// tea_leaf_ppcg_inner_kernel_cuda_ twice with bounds_extra 0 and every face
+// external (one chunk).
+EXPORT void tea_engine_ppcg_inner2(int step)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    const int k = step - 1;
+    if (k < 0 || k + 1 >= (int)g.ch_a.size()) {
+        die("ppcg_inner2: steps %d, %d outside the %d uploaded coefficients", step, step + 1, (int)g.ch_a.size());
+        return;
+    }
+    BY_PRECISION(ppcg_inner2_impl, g.ch_a[k], g.ch_b[k], g.ch_a[k + 1], g.ch_b[k + 1]);
+}
+
 EXPORT void tea_engine_set_deck_path(const char *path) { g_deck_path = path ? path : "tea.in"; }
 EXPORT void tea_engine_set_error_mode(int mode) { g_error_mode = mode; }
 EXPORT int tea_engine_last_error(char *buf, int buflen)
@@ -1511,6 +1598,8 @@ EXPORT int tea_engine_set_field(int id, const double *host)
 {
     if (!g.initialised || id < 1 || id >= X_COUNT) return -1;
     if (id == F_SD) g.sd_halo_dirty = true;
+    if (id == F_P) g.p_halo_dirty[0] = true;
+    if (id == X_RTEMP) g.rt_halo_dirty[0] = true;
     const size_t w = (size_t)g.nx + 2 * g.h;
     HIP_OK_R(hipMemcpy2DAsync(g.f[id] + (g.lead - g.h), g.pitch * sizeof(double), host, w * sizeof(double),
                               w * sizeof(double), (size_t)g.rows, hipMemcpyHostToDevice, g.stream), -1);
@@ -1622,6 +1711,8 @@ EXPORT int tea_engine_set_field32(int id, const float *host)
     HIP_OK_R(hipMemcpy2DAsync(g.fs32[id], g.pitch32 * sizeof(float), host, w * sizeof(float), w * sizeof(float),
                               (size_t)g.rows, hipMemcpyHostToDevice, g.stream), -1);
     if (id == F_SD) g.sd32_halo_dirty = true;
+    if (id == F_P) g.p_halo_dirty[1] = true;
+    if (id == X_RTEMP) g.rt_halo_dirty[1] = true;
     HIP_OK_R(hipStreamSynchronize(g.stream), -1);
     return 0;
 }

@@ -381,6 +381,339 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_inner_fused(Ext e, long pitch, c
     STRIP_END
 }
 
+// Two PPCG inner steps in one pass (no reference counterpart: the composition
+// of two k_ppcg_inner_fused launches with coefficients (a1, b1) then (a2, b2),
+// bit-identical to them).  A workgroup of NW waves owns an output tile of
+// WAVE-2 columns x (NW-2) strips; wave j takes strip j-1 of the tile (waves 0
+// and NW-1 the strips below and above it), lane l column l-1 (lanes 0 and
+// WAVE-1 the columns left and right of it).  Step 1 runs on the whole
+// (WAVE) x (NW strips) region, so every sd value step 2 needs at the tile's
+// cells is at hand: the strip's own rows in registers, the rows above and below
+// through LDS, the columns left and right through lane shuffles.  Step 2 runs
+// on the tile only.  rtemp is read from rt_in and written to rt_out because
+// step 1 also reads it on the neighbouring tiles' cells; utemp is read and
+// written on the tile's own cells only.  sd outside the extent keeps its value.
+template <typename T, int PREC, int NW>
+__global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner2(Ext e, long pitch, const T *__restrict__ sd_in,
+                                                          T *__restrict__ sd_out, const T *__restrict__ rt_in,
+                                                          T *__restrict__ rt_out, T *__restrict__ utemp,
+                                                          const T *__restrict__ Mi, const T *__restrict__ Kx,
+                                                          const T *__restrict__ Ky, double a1_in, double b1_in,
+                                                          double a2_in, double b2_in)
+{
+    constexpr int OUT_X = WAVE - 2, OUT_Y = (NW - 2) * STRIP;
+    __shared__ T lds_lo[NW][WAVE], lds_hi[NW][WAVE];  // each strip's first and last row of sd after step 1
+    const T a1 = (T)a1_in, b1 = (T)b1_in, a2 = (T)a2_in, b2 = (T)b2_in;
+    const int lane = threadIdx.x & (WAVE - 1), j = threadIdx.x / WAVE;
+    const int ntx = (e.x1 - e.x0 + OUT_X) / OUT_X, nty = (e.y1 - e.y0 + OUT_Y) / OUT_Y;
+    const int ntiles = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : ntx * nty;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int ty = t / ntx, tx = t - ty * ntx;
+        const int c = e.x0 + tx * OUT_X - 1 + lane;
+        const int r0 = e.y0 + ty * OUT_Y + (j - 1) * STRIP;
+        const long i0 = (long)c + (long)r0 * pitch;
+        const bool col_in = c >= e.x0 && c <= e.x1;
+        const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;   // the strip's rows inside the extent
+        len = len < 0 ? 0 : (len > STRIP ? STRIP : len);
+        const int act = col_in ? len : 0;
+        const bool out = j >= 1 && j <= NW - 2 && lane >= 1 && lane <= WAVE - 2;
+
+        // ---- step 1 on the whole region
+        T sd1[STRIP], rt1[STRIP];
+        {
+            T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const long i = i0 + k * pitch;
+                const int rr = r0 + k;
+                sd1[k] = T(0.0);
+                if (col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) sd1[k] = sd_in[i];
+                if (k < act) {
+                    const T result = apply_A(sd_in, Kx, Ky, i, pitch);
+                    rv[k] = rt_in[i] - result;
+                }
+            }
+            if (PREC == PREC_BLOCK && act > 0) precond_strip<T, PREC_BLOCK>(rv, zv, act, i0, pitch, Mi, Kx, Ky);
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                rt1[k] = rv[k];
+                if (k < act) {
+                    const long i = i0 + k * pitch;
+                    if (PREC == PREC_BLOCK) sd1[k] = a1 * sd1[k] + b1 * zv[k];
+                    else if (PREC == PREC_DIAG) sd1[k] = a1 * sd1[k] + b1 * rv[k] * Mi[i];
+                    else sd1[k] = a1 * sd1[k] + b1 * rv[k];
+                }
+            }
+        }
+        lds_lo[j][lane] = sd1[0];
+        lds_hi[j][lane] = sd1[STRIP - 1];
+        __syncthreads();
+
+        // ---- step 2 on the tile
+        {
+            T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+            const int act2 = out ? act : 0;
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const T left = __shfl_up(sd1[k], 1, WAVE);
+                const T right = __shfl_down(sd1[k], 1, WAVE);
+                if (k < act2) {
+                    const long i = i0 + k * pitch;
+                    const T below = k == 0 ? lds_hi[j - 1][lane] : sd1[k - 1];
+                    const T above = k == STRIP - 1 ? lds_lo[j + 1][lane] : sd1[k + 1];
+                    const T result = (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * sd1[k]
+                                     - (Ky[i + pitch] * above + Ky[i] * below)
+                                     - (Kx[i + 1] * right + Kx[i] * left);
+                    rv[k] = rt1[k] - result;
+                }
+            }
+            if (PREC == PREC_BLOCK && act2 > 0) precond_strip<T, PREC_BLOCK>(rv, zv, act2, i0, pitch, Mi, Kx, Ky);
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                if (k < act2) {
+                    const long i = i0 + k * pitch;
+                    T sd2;
+                    if (PREC == PREC_BLOCK) sd2 = a2 * sd1[k] + b2 * zv[k];
+                    else if (PREC == PREC_DIAG) sd2 = a2 * sd1[k] + b2 * rv[k] * Mi[i];
+                    else sd2 = a2 * sd1[k] + b2 * rv[k];
+                    const T ut1 = utemp[i] + sd1[k];
+                    sd_out[i] = sd2;
+                    rt_out[i] = rv[k];
+                    utemp[i] = ut1 + sd2;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Block-Jacobi strip solve of one column from its own diagonal and Ky values
+// (diag[k] = diag_A at row k, ky[k] = Ky at row k, k = 0..len): the
+// expressions of block_solve_strip.
+__device__ __forceinline__ void strip_solve_col(const float (&rhs)[STRIP], float (&z)[STRIP], int len,
+                                                const float (&diag)[STRIP], const float (&ky)[STRIP + 1])
+{
+    float cp[STRIP], bfp[STRIP], dp[STRIP];
+    cp[0] = (-ky[1]) / diag[0];
+    dp[0] = rhs[0] / diag[0];
+#pragma unroll
+    for (int k = 1; k < STRIP; k++) {
+        if (k < len) {
+            bfp[k] = 1.0f / (diag[k] - (-ky[k]) * cp[k - 1]);
+            cp[k] = (-ky[k + 1]) * bfp[k];
+            dp[k] = (rhs[k] - (-ky[k]) * dp[k - 1]) * bfp[k];
+        }
+    }
+#pragma unroll
+    for (int k = STRIP - 1; k >= 0; k--) {
+        if (k == len - 1) z[k] = dp[k];
+        else if (k < len - 1) z[k] = dp[k] - cp[k] * z[k + 1];
+    }
+}
+
+// k_ppcg_inner2 for FP32 fields with two columns per lane (8-byte accesses:
+// an FP32 row segment of a wave is then as long as an FP64 one).  Lane l
+// holds columns X0-2+2l and X0-1+2l of an output tile of 2*WAVE-4 columns
+// starting at X0 (lane 0's and lane WAVE-1's pairs are the halo); otherwise
+// the same scheme and arithmetic as k_ppcg_inner2, bit-identical to two
+// k_ppcg_inner_fused<float> launches.
+template <int PREC, int NW>
+__global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner2_x2(Ext e, long pitch, const float *__restrict__ sd_in,
+                                                             float *__restrict__ sd_out,
+                                                             const float *__restrict__ rt_in,
+                                                             float *__restrict__ rt_out, float *__restrict__ utemp,
+                                                             const float *__restrict__ Mi,
+                                                             const float *__restrict__ Kx,
+                                                             const float *__restrict__ Ky, double a1_in,
+                                                             double b1_in, double a2_in, double b2_in)
+{
+    constexpr int OUT_X = 2 * WAVE - 4, OUT_Y = (NW - 2) * STRIP;
+    __shared__ float2 lds_lo[NW][WAVE], lds_hi[NW][WAVE];
+    const float a1 = (float)a1_in, b1 = (float)b1_in, a2 = (float)a2_in, b2 = (float)b2_in;
+    const int lane = threadIdx.x & (WAVE - 1), j = threadIdx.x / WAVE;
+    const int ntx = (e.x1 - e.x0 + OUT_X) / OUT_X, nty = (e.y1 - e.y0 + OUT_Y) / OUT_Y;
+    const int ntiles = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : ntx * nty;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int ty = t / ntx, tx = t - ty * ntx;
+        const int c = e.x0 + tx * OUT_X - 2 + 2 * lane;   // the lane's first column
+        const int r0 = e.y0 + ty * OUT_Y + (j - 1) * STRIP;
+        const long i0 = (long)c + (long)r0 * pitch;
+        const bool pair_mem = c >= e.x0 - 2 && c <= e.x1 + 1;
+        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
+        len = len < 0 ? 0 : (len > STRIP ? STRIP : len);
+        int act[2];
+        act[0] = (c >= e.x0 && c <= e.x1) ? len : 0;
+        act[1] = (c + 1 >= e.x0 && c + 1 <= e.x1) ? len : 0;
+        const bool out = j >= 1 && j <= NW - 2 && lane >= 1 && lane <= WAVE - 2;
+        const bool right_lane = lane < WAVE - 1;
+
+        // ---- loads: sd rows r0-1 .. r0+STRIP, rtemp, Kx, Ky (rows r0 .. r0+STRIP), utemp
+        float sv[2][STRIP + 2], kxv[2][STRIP], kyv[2][STRIP + 1], rtv[2][STRIP], utv[2][STRIP], miv[2][STRIP];
+#pragma unroll
+        for (int q = 0; q < STRIP + 2; q++) {
+            const int rr = r0 - 1 + q;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (pair_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) v = *(const float2 *)(sd_in + i0 + (long)(q - 1) * pitch);
+            sv[0][q] = v.x;
+            sv[1][q] = v.y;
+        }
+        const int lanelen = act[0] > act[1] ? act[0] : act[1];
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) {
+            float2 v = make_float2(0.0f, 0.0f);
+            if (k <= lanelen && lanelen > 0) v = *(const float2 *)(Ky + i0 + (long)k * pitch);
+            kyv[0][k] = v.x;
+            kyv[1][k] = v.y;
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            float2 kx = make_float2(0.0f, 0.0f), rt = make_float2(0.0f, 0.0f), ut = make_float2(0.0f, 0.0f),
+                   mi = make_float2(0.0f, 0.0f);
+            if (k < lanelen) {
+                const long i = i0 + (long)k * pitch;
+                kx = *(const float2 *)(Kx + i);
+                rt = *(const float2 *)(rt_in + i);
+                if (out) ut = *(const float2 *)(utemp + i);
+                if (PREC == PREC_DIAG) mi = *(const float2 *)(Mi + i);
+            }
+            kxv[0][k] = kx.x; kxv[1][k] = kx.y;
+            rtv[0][k] = rt.x; rtv[1][k] = rt.y;
+            utv[0][k] = ut.x; utv[1][k] = ut.y;
+            miv[0][k] = mi.x; miv[1][k] = mi.y;
+        }
+        // Kx on the right face of the second column (the next lane's first Kx)
+        float kxr[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const float sh = __shfl_down(kxv[0][k], 1, WAVE);
+            kxr[k] = 0.0f;
+            if (k < act[1]) kxr[k] = (right_lane && c + 2 <= e.x1) ? sh : Kx[i0 + (long)k * pitch + 2];
+        }
+        float diag[2][STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            diag[0][k] = 1.0f + (kyv[0][k + 1] + kyv[0][k]) + (kxv[1][k] + kxv[0][k]);
+            diag[1][k] = 1.0f + (kyv[1][k + 1] + kyv[1][k]) + (kxr[k] + kxv[1][k]);
+        }
+
+        // ---- step 1 on the whole region
+        float sd1[2][STRIP], rt1[2][STRIP];
+        {
+            float rv[2][STRIP], zv[2][STRIP];
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const float from_left = __shfl_up(sv[1][k + 1], 1, WAVE);
+                const float from_right = __shfl_down(sv[0][k + 1], 1, WAVE);
+                const long i = i0 + (long)k * pitch;
+                rv[0][k] = rv[1][k] = zv[0][k] = zv[1][k] = 0.0f;
+                sd1[0][k] = sv[0][k + 1];
+                sd1[1][k] = sv[1][k + 1];
+                if (k < act[0]) {
+                    const float left = lane > 0 ? from_left : sd_in[i - 1];
+                    const float res = diag[0][k] * sv[0][k + 1] - (kyv[0][k + 1] * sv[0][k + 2] + kyv[0][k] * sv[0][k])
+                                      - (kxv[1][k] * sv[1][k + 1] + kxv[0][k] * left);
+                    rv[0][k] = rtv[0][k] - res;
+                }
+                if (k < act[1]) {
+                    const float right = right_lane ? from_right : sd_in[i + 2];
+                    const float res = diag[1][k] * sv[1][k + 1] - (kyv[1][k + 1] * sv[1][k + 2] + kyv[1][k] * sv[1][k])
+                                      - (kxr[k] * right + kxv[1][k] * sv[0][k + 1]);
+                    rv[1][k] = rtv[1][k] - res;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                if (PREC == PREC_BLOCK && act[q] > 0) {
+                    float dq[STRIP], kq[STRIP + 1];
+#pragma unroll
+                    for (int k = 0; k < STRIP; k++) dq[k] = diag[q][k];
+#pragma unroll
+                    for (int k = 0; k < STRIP + 1; k++) kq[k] = kyv[q][k];
+                    strip_solve_col(rv[q], zv[q], act[q], dq, kq);
+                }
+#pragma unroll
+                for (int k = 0; k < STRIP; k++) {
+                    rt1[q][k] = rv[q][k];
+                    if (k < act[q]) {
+                        if (PREC == PREC_BLOCK) sd1[q][k] = a1 * sd1[q][k] + b1 * zv[q][k];
+                        else if (PREC == PREC_DIAG) sd1[q][k] = a1 * sd1[q][k] + b1 * rv[q][k] * miv[q][k];
+                        else sd1[q][k] = a1 * sd1[q][k] + b1 * rv[q][k];
+                    }
+                }
+            }
+        }
+        lds_lo[j][lane] = make_float2(sd1[0][0], sd1[1][0]);
+        lds_hi[j][lane] = make_float2(sd1[0][STRIP - 1], sd1[1][STRIP - 1]);
+        __syncthreads();
+
+        // ---- step 2 on the tile
+        {
+            const int act2[2] = {out ? act[0] : 0, out ? act[1] : 0};
+            float rv[2][STRIP], zv[2][STRIP];
+            const float2 lo_below = lds_hi[j > 0 ? j - 1 : 0][lane];
+            const float2 hi_above = lds_lo[j < NW - 1 ? j + 1 : NW - 1][lane];
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const float from_left = __shfl_up(sd1[1][k], 1, WAVE);
+                const float from_right = __shfl_down(sd1[0][k], 1, WAVE);
+                rv[0][k] = rv[1][k] = zv[0][k] = zv[1][k] = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    if (k < act2[q]) {
+                        const float below = k == 0 ? (q ? lo_below.y : lo_below.x) : sd1[q][k - 1];
+                        const float above = k == STRIP - 1 ? (q ? hi_above.y : hi_above.x) : sd1[q][k + 1];
+                        const float left = q ? sd1[0][k] : from_left;
+                        const float right = q ? from_right : sd1[1][k];
+                        const float kr = q ? kxr[k] : kxv[1][k];
+                        const float res = diag[q][k] * sd1[q][k] - (kyv[q][k + 1] * above + kyv[q][k] * below)
+                                          - (kr * right + kxv[q][k] * left);
+                        rv[q][k] = rt1[q][k] - res;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                if (PREC == PREC_BLOCK && act2[q] > 0) {
+                    float dq[STRIP], kq[STRIP + 1];
+#pragma unroll
+                    for (int k = 0; k < STRIP; k++) dq[k] = diag[q][k];
+#pragma unroll
+                    for (int k = 0; k < STRIP + 1; k++) kq[k] = kyv[q][k];
+                    strip_solve_col(rv[q], zv[q], act2[q], dq, kq);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                float sdo[2], uto[2];
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    if (PREC == PREC_BLOCK) sdo[q] = a2 * sd1[q][k] + b2 * zv[q][k];
+                    else if (PREC == PREC_DIAG) sdo[q] = a2 * sd1[q][k] + b2 * rv[q][k] * miv[q][k];
+                    else sdo[q] = a2 * sd1[q][k] + b2 * rv[q][k];
+                    const float ut1 = utv[q][k] + sd1[q][k];
+                    uto[q] = ut1 + sdo[q];
+                }
+                const long i = i0 + (long)k * pitch;
+                if (k < act2[0] && k < act2[1]) {
+                    *(float2 *)(sd_out + i) = make_float2(sdo[0], sdo[1]);
+                    *(float2 *)(rt_out + i) = make_float2(rv[0][k], rv[1][k]);
+                    *(float2 *)(utemp + i) = make_float2(uto[0], uto[1]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 2; q++)
+                        if (k < act2[q]) {
+                            sd_out[i + q] = sdo[q];
+                            rt_out[i + q] = rv[q][k];
+                            utemp[i + q] = uto[q];
+                        }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // tea_leaf_ppcg.cuknl:279-297: sum (r - r_store)*z
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const T *__restrict__ rstore,
@@ -406,7 +739,9 @@ __global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const T *
 // ppcg_store_r: rstore = r, ppcg_calc_2norm: r.z.  p' goes to the spare
 // buffer pn (the caller swaps it in): p is read at neighbours that other
 // lanes rewrite.  Cells outside the interior keep their p, as calc_p only
-// updates the interior.
+// updates the interior.  Each lane forms p' down its own column (the strip
+// and the rows above and below it) and takes its x neighbours from the
+// adjacent lanes; the wave's edge lanes form those themselves.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int with_p, double beta_in,
                                                       const T *__restrict__ p, T *__restrict__ pn,
@@ -415,21 +750,32 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int wit
                                                       const T *__restrict__ r, T *__restrict__ rstore, Red red)
 {
     const T beta = (T)beta_in;
+    const int lane = threadIdx.x & (WAVE - 1);
     DD acc[2] = {{0, 0}, {0, 0}};
     auto pv = [&](long j, int c, int rr) -> T {
         const bool in = c >= e.x0 && c <= e.x1 && rr >= e.y0 && rr <= e.y1;
         return (with_p && in) ? z[j] + beta * p[j] : p[j];
     };
     STRIP_BEGIN
+    T col[STRIP + 2];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; q++) {
+        col[q] = T(0.0);
+        if (s.len > 0 && q <= s.len + 1) col[q] = pv(i0 + (long)(q - 1) * pitch, s.c, s.r0 - 1 + q);
+    }
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
+        const T from_left = __shfl_up(col[k + 1], 1, WAVE);
+        const T from_right = __shfl_down(col[k + 1], 1, WAVE);
         if (k < s.len) {
             const long i = i0 + k * pitch;
             const int c = s.c, rr = s.r0 + k;
-            const T pc = pv(i, c, rr);
+            const T left = lane > 0 ? from_left : pv(i - 1, c - 1, rr);
+            const T right = (lane < WAVE - 1 && c + 1 <= e.x1) ? from_right : pv(i + 1, c + 1, rr);
+            const T pc = col[k + 1];
             const T wv = (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * pc
-                         - (Ky[i + pitch] * pv(i + pitch, c, rr + 1) + Ky[i] * pv(i - pitch, c, rr - 1))
-                         - (Kx[i + 1] * pv(i + 1, c + 1, rr) + Kx[i] * pv(i - 1, c - 1, rr));
+                         - (Ky[i + pitch] * col[k + 2] + Ky[i] * col[k])
+                         - (Kx[i + 1] * right + Kx[i] * left);
             pn[i] = pc;
             w[i] = wv;
             const T rv = r[i];

@@ -91,6 +91,7 @@ struct Backend {
     void (*ppcg_begin)(int, double, double *) = nullptr;
     void (*ppcg_ur)(double, double, double *) = nullptr;
     void (*ppcg_end)(double *) = nullptr;
+    void (*ppcg_inner2)(int) = nullptr;
 
     void check() const
     {
@@ -126,6 +127,7 @@ Backend load_backend(const char *path)
     b.ppcg_begin = reinterpret_cast<void (*)(int, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_begin"));
     b.ppcg_ur = reinterpret_cast<void (*)(double, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_ur"));
     b.ppcg_end = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_ppcg_end"));
+    b.ppcg_inner2 = reinterpret_cast<void (*)(int)>(dlsym(b.handle, "tea_engine_ppcg_inner2"));
     return b;
 }
 
@@ -719,6 +721,22 @@ void Run::run_ppcg_inner_steps(int *ppcg_inner_iters, bool fused)
     if (!fused) {
         b.tea_leaf_ppcg_init_sd_new_kernel_cuda_(&th);
         b.check();
+    }
+    const char *env = std::getenv("TEA_INNER2");
+    if (fused && b.ppcg_inner2 && !(env && env[0] == '0')) {
+        // one chunk, no reflective update: the steps run on the interior in
+        // order, two per pass
+        int step = 1;
+        for (; step + 1 <= d.tl_ppcg_inner_steps; step += 2) {
+            b.ppcg_inner2(step);
+            b.check();
+        }
+        if (step == d.tl_ppcg_inner_steps) {
+            int be = 0, st = step;
+            b.tea_leaf_ppcg_inner_kernel_cuda_(&st, &be, chunk_neighbours);
+            b.check();
+        }
+        return;
     }
     const int h = d.halo_exchange_depth;
     for (int cur = 1; cur <= d.tl_ppcg_inner_steps; cur += h) {

@@ -106,6 +106,7 @@ _EXT = {
     "tea_engine_ppcg_begin": ([ctypes.c_int, ctypes.c_double, _D], None),
     "tea_engine_ppcg_ur": ([ctypes.c_double, ctypes.c_double, _D], None),
     "tea_engine_ppcg_end": ([_D], None),
+    "tea_engine_ppcg_inner2": ([ctypes.c_int], None),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
     "tea_engine_mp_begin": ([_D], None),
@@ -333,6 +334,9 @@ class Engine:
         out = ctypes.c_double()
         self._call("tea_engine_ppcg_ur", float(alpha), float(theta), ctypes.byref(out))
         return out.value
+
+    def ppcg_inner2(self, step: int):
+        self._call("tea_engine_ppcg_inner2", int(step))
 
     def ppcg_end(self) -> float:
         out = ctypes.c_double()

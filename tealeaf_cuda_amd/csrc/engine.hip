@@ -306,6 +306,25 @@ bool launch(const char *name, K kernel, const Ext &e, A... args)
     return true;
 }
 
+// The same for kernels on tiles of 2*TILE_X x TILE_Y cells (FP32 kernels whose
+// lanes hold two columns).
+template <typename K, typename... A>
+bool launch_x2(const char *name, K kernel, const Ext &e, A... args)
+{
+    const int nbx = (e.x1 - e.x0 + 2 * TILE_X) / (2 * TILE_X), nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
+    int nb = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : nbx * nby;
+    nb = nb < grid_max() ? nb : grid_max();
+    g.last_grid = nb;
+    if (nb == 0) return true;
+    {
+        Timed t(name);
+        hipLaunchKernelGGL(kernel, dim3(nb), dim3(BLOCK), 0, g.stream, e, args...);
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
+    return true;
+}
+
 // Combines the partials of the last reduction launch on the device, waits,
 // and returns the n finished values.
 bool fetch(double *dst, int n)
@@ -1314,10 +1333,16 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
                            FT<T>(X_P2), (const T *)FT<T>(F_P), pt, g.h, g.nx, g.ny);
         dirty = false;
     }
-    if (!launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
-                FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
-                (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx()))
-        return;
+    bool ok;
+    if constexpr (std::is_same<T, float>::value)
+        ok = launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
+                       FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX),
+                       (const T *)FT<T>(X_KY), (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
+    else
+        ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
+                    FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
+                    (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
+    if (!ok) return;
     if (std::is_same<T, double>::value) {
         std::swap(g.f[F_P], g.f[X_P2]);
         std::swap(g.fs[F_P], g.fs[X_P2]);
@@ -1329,11 +1354,24 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
 
 template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
 {
-    if (!PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, theta, FT<T>(F_U),
-                       (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z), (const T *)FT<T>(X_MI),
-                       (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD), FT<T>(X_RTEMP),
-                       FT<T>(X_UTEMP), red_ctx()))
-        return;
+    bool ok;
+    if constexpr (std::is_same<T, float>::value) {
+        auto go = [&](auto kernel) {
+            return launch_x2("k_ppcg_ur_sd_f32", kernel, interior(), pitch_of<T>(), alpha, theta, FT<T>(F_U),
+                             (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
+                             (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD),
+                             FT<T>(X_RTEMP), FT<T>(X_UTEMP), red_ctx());
+        };
+        ok = g.prec == PREC_BLOCK ? go(k_ppcg_ur_sd_x2<PREC_BLOCK>)
+             : g.prec == PREC_DIAG ? go(k_ppcg_ur_sd_x2<PREC_DIAG>)
+                                   : go(k_ppcg_ur_sd_x2<PREC_NONE>);
+    } else {
+        ok = PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, theta,
+                           FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
+                           (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD),
+                           FT<T>(X_RTEMP), FT<T>(X_UTEMP), red_ctx());
+    }
+    if (!ok) return;
     fetch(rrn, 1);
 }
 

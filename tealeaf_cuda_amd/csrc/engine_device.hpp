@@ -142,13 +142,16 @@ struct Strip {
 // XCD b % 8; the tiles (row-major, 64 x 16 cells) are split into 8 contiguous
 // bands, one per XCD, and the workgroups of an XCD take the tiles of its band
 // round-robin, so vertically adjacent tiles share an L2.
-struct TileLoop {
+// TX: tile width in cells (TILE_X, or 2*TILE_X for the FP32 kernels whose
+// lanes hold two columns).
+template <int TX = TILE_X>
+struct TileLoopT {
     int nbx, count, start, k, nk;
     int t;
     Ext e;
-    __device__ __forceinline__ explicit TileLoop(const Ext ext) : e(ext)
+    __device__ __forceinline__ explicit TileLoopT(const Ext ext) : e(ext)
     {
-        nbx = (e.x1 - e.x0 + TILE_X) / TILE_X;
+        nbx = (e.x1 - e.x0 + TX) / TX;
         const int nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
         const int ntiles = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : nbx * nby;
         const int b = (int)blockIdx.x, g = (int)gridDim.x;
@@ -170,7 +173,7 @@ struct TileLoop {
         const int lane = threadIdx.x & (WAVE - 1);
         const int wave = threadIdx.x / WAVE;
         Strip s;
-        s.c = e.x0 + tx * TILE_X + lane;
+        s.c = e.x0 + tx * TX + (TX / TILE_X) * lane;
         s.r0 = e.y0 + ty * TILE_Y + wave * STRIP;
         int len = e.y1 - s.r0 + 1;
         len = len < STRIP ? len : STRIP;
@@ -179,6 +182,8 @@ struct TileLoop {
         return s;
     }
 };
+using TileLoop = TileLoopT<TILE_X>;
+
 
 // 5-point operator (I - dt div K grad) x at flat index i, in the expression
 // order of tea_leaf_cg.cuknl:53-57, evaluated in the field type T (FP64, or

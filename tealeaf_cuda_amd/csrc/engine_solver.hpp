@@ -788,6 +788,185 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int wit
     block_reduce<2>(acc, red);
 }
 
+// k_ppcg_begin for FP32 fields with two columns per lane (8-byte accesses);
+// tiles of 2*TILE_X x TILE_Y cells.  The same arithmetic cell by cell.
+__global__ __launch_bounds__(BLOCK) void k_ppcg_begin_x2(Ext e, long pitch, int with_p, double beta_in,
+                                                         const float *__restrict__ p, float *__restrict__ pn,
+                                                         const float *__restrict__ z, float *__restrict__ w,
+                                                         const float *__restrict__ Kx, const float *__restrict__ Ky,
+                                                         const float *__restrict__ r, float *__restrict__ rstore,
+                                                         Red red)
+{
+    const float beta = (float)beta_in;
+    const int lane = threadIdx.x & (WAVE - 1);
+    DD acc[2] = {{0, 0}, {0, 0}};
+    auto inside = [&](int c, int rr) { return c >= e.x0 && c <= e.x1 && rr >= e.y0 && rr <= e.y1; };
+    auto pv = [&](long j, int c, int rr) -> float { return (with_p && inside(c, rr)) ? z[j] + beta * p[j] : p[j]; };
+    for (TileLoopT<2 * TILE_X> tl_(e); tl_.more(); tl_.advance()) {
+        const Strip s = tl_.strip();
+        const int c = s.c;
+        const long i0 = (long)c + (long)s.r0 * pitch;
+        const int act0 = s.len, act1 = (c + 1 <= e.x1) ? s.len : 0;
+        // p' down both columns, rows r0-1 .. r0+len; z on the strip's rows
+        float col[2][STRIP + 2], zs[2][STRIP];
+#pragma unroll
+        for (int q = 0; q < STRIP + 2; q++) {
+            col[0][q] = col[1][q] = 0.0f;
+            if (act0 > 0 && q <= act0 + 1) {
+                const long j = i0 + (long)(q - 1) * pitch;
+                const int rr = s.r0 - 1 + q;
+                const float2 pp = *(const float2 *)(p + j);
+                const float2 zz = *(const float2 *)(z + j);
+                col[0][q] = (with_p && inside(c, rr)) ? zz.x + beta * pp.x : pp.x;
+                col[1][q] = (with_p && inside(c + 1, rr)) ? zz.y + beta * pp.y : pp.y;
+                if (q >= 1 && q <= STRIP) {
+                    zs[0][q - 1] = zz.x;
+                    zs[1][q - 1] = zz.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const float from_left = __shfl_up(col[1][k + 1], 1, WAVE);
+            const float from_right = __shfl_down(col[0][k + 1], 1, WAVE);
+            if (k < act0) {
+                const long i = i0 + (long)k * pitch;
+                const int rr = s.r0 + k;
+                const float2 kx = *(const float2 *)(Kx + i);
+                const float2 ky0 = *(const float2 *)(Ky + i);
+                const float2 ky1 = *(const float2 *)(Ky + i + pitch);
+                const float2 rv = *(const float2 *)(r + i);
+                const bool rlane = lane < WAVE - 1 && c + 2 <= e.x1;
+                const float left = lane > 0 ? from_left : pv(i - 1, c - 1, rr);
+                const float w0 = (1.0f + (ky1.x + ky0.x) + (kx.y + kx.x)) * col[0][k + 1]
+                                 - (ky1.x * col[0][k + 2] + ky0.x * col[0][k])
+                                 - (kx.y * col[1][k + 1] + kx.x * left);
+                dd_add_prod(acc[0], (double)col[0][k + 1], (double)w0);
+                dd_add_prod(acc[1], (double)rv.x, (double)zs[0][k]);
+                if (k < act1) {
+                    const float right = rlane ? from_right : pv(i + 2, c + 2, rr);
+                    const float kxr = Kx[i + 2];
+                    const float w1 = (1.0f + (ky1.y + ky0.y) + (kxr + kx.y)) * col[1][k + 1]
+                                     - (ky1.y * col[1][k + 2] + ky0.y * col[1][k])
+                                     - (kxr * right + kx.y * col[0][k + 1]);
+                    *(float2 *)(pn + i) = make_float2(col[0][k + 1], col[1][k + 1]);
+                    *(float2 *)(w + i) = make_float2(w0, w1);
+                    *(float2 *)(rstore + i) = rv;
+                    dd_add_prod(acc[0], (double)col[1][k + 1], (double)w1);
+                    dd_add_prod(acc[1], (double)rv.y, (double)zs[1][k]);
+                } else {
+                    pn[i] = col[0][k + 1];
+                    w[i] = w0;
+                    rstore[i] = rv.x;
+                }
+            }
+        }
+    }
+    block_reduce<2>(acc, red);
+}
+
+// k_ppcg_ur_sd for FP32 fields with two columns per lane.
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2(Ext e, long pitch, double alpha_in, double theta_in,
+                                                         float *__restrict__ u, const float *__restrict__ p,
+                                                         float *__restrict__ r, const float *__restrict__ w,
+                                                         float *__restrict__ z, const float *__restrict__ Mi,
+                                                         const float *__restrict__ Kx, const float *__restrict__ Ky,
+                                                         float *__restrict__ sd, float *__restrict__ rtemp,
+                                                         float *__restrict__ utemp, Red red)
+{
+    const float alpha = (float)alpha_in;
+    const float theta_r = 1.0f / (float)theta_in;
+    DD acc[1] = {{0, 0}};
+    for (TileLoopT<2 * TILE_X> tl_(e); tl_.more(); tl_.advance()) {
+        const Strip s = tl_.strip();
+        const int c = s.c;
+        const long i0 = (long)c + (long)s.r0 * pitch;
+        int act[2] = {s.len, (c + 1 <= e.x1) ? s.len : 0};
+        float rv[2][STRIP], zv[2][STRIP], mi[2][STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            rv[0][k] = rv[1][k] = zv[0][k] = zv[1][k] = mi[0][k] = mi[1][k] = 0.0f;
+            if (k < act[0]) {
+                const long i = i0 + (long)k * pitch;
+                const float2 pp = *(const float2 *)(p + i), ww = *(const float2 *)(w + i);
+                float2 uu = *(const float2 *)(u + i), rr = *(const float2 *)(r + i);
+                uu.x += alpha * pp.x;
+                uu.y += alpha * pp.y;
+                rv[0][k] = rr.x - alpha * ww.x;
+                rv[1][k] = rr.y - alpha * ww.y;
+                if (PREC == PREC_DIAG) {
+                    const float2 m = *(const float2 *)(Mi + i);
+                    mi[0][k] = m.x;
+                    mi[1][k] = m.y;
+                }
+                if (k < act[1]) {
+                    *(float2 *)(u + i) = uu;
+                    *(float2 *)(r + i) = make_float2(rv[0][k], rv[1][k]);
+                } else {
+                    u[i] = uu.x;
+                    r[i] = rv[0][k];
+                }
+            }
+        }
+        if (PREC == PREC_BLOCK && act[0] > 0) {
+            float diag[2][STRIP], ky[2][STRIP + 1];
+#pragma unroll
+            for (int k = 0; k < STRIP + 1; k++) {
+                ky[0][k] = ky[1][k] = 0.0f;
+                if (k <= act[0]) {
+                    const float2 v = *(const float2 *)(Ky + i0 + (long)k * pitch);
+                    ky[0][k] = v.x;
+                    ky[1][k] = v.y;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                diag[0][k] = diag[1][k] = 1.0f;
+                if (k < act[0]) {
+                    const long i = i0 + (long)k * pitch;
+                    const float2 kx = *(const float2 *)(Kx + i);
+                    const float kxr = Kx[i + 2];
+                    diag[0][k] = 1.0f + (ky[0][k + 1] + ky[0][k]) + (kx.y + kx.x);
+                    diag[1][k] = 1.0f + (ky[1][k + 1] + ky[1][k]) + (kxr + kx.y);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+                if (act[q] > 0) strip_solve_col(rv[q], zv[q], act[q], diag[q], ky[q]);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            if (k < act[0]) {
+                const long i = i0 + (long)k * pitch;
+                float src[2], sdv[2];
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    if (PREC == PREC_BLOCK) src[q] = zv[q][k];
+                    else if (PREC == PREC_DIAG) src[q] = rv[q][k] * mi[q][k];
+                    else src[q] = rv[q][k];
+                    sdv[q] = src[q] * theta_r;
+                }
+                const int nq = k < act[1] ? 2 : 1;
+                for (int q = 0; q < nq; q++)
+                    dd_add_prod(acc[0], (double)rv[q][k], (double)(PREC != PREC_NONE ? src[q] : rv[q][k]));
+                if (nq == 2) {
+                    if (PREC != PREC_NONE) *(float2 *)(z + i) = make_float2(src[0], src[1]);
+                    *(float2 *)(sd + i) = make_float2(sdv[0], sdv[1]);
+                    *(float2 *)(rtemp + i) = make_float2(rv[0][k], rv[1][k]);
+                    *(float2 *)(utemp + i) = make_float2(sdv[0], sdv[1]);
+                } else {
+                    if (PREC != PREC_NONE) z[i] = src[0];
+                    sd[i] = sdv[0];
+                    rtemp[i] = rv[0][k];
+                    utemp[i] = sdv[0];
+                }
+            }
+        }
+    }
+    block_reduce<1>(acc, red);
+}
+
 // cg_calc_ur (u += a p; r -= a w; z = M^-1 r; r.z or r.r) then
 // ppcg_init_sd_new (sd = (z or r) / theta; rtemp = r; utemp = sd).
 template <typename T, int PREC>

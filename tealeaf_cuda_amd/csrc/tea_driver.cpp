@@ -723,7 +723,7 @@ void Run::run_ppcg_inner_steps(int *ppcg_inner_iters, bool fused)
         b.check();
     }
     const char *env = std::getenv("TEA_INNER2");
-    if (fused && b.ppcg_inner2 && !(env && env[0] == '0')) {
+    if (fused_outer() && b.ppcg_inner2 && !(env && env[0] == '0')) {
         // one chunk, no reflective update: the steps run on the interior in
         // order, two per pass
         int step = 1;
@@ -734,6 +734,10 @@ void Run::run_ppcg_inner_steps(int *ppcg_inner_iters, bool fused)
         if (step == d.tl_ppcg_inner_steps) {
             int be = 0, st = step;
             b.tea_leaf_ppcg_inner_kernel_cuda_(&st, &be, chunk_neighbours);
+            b.check();
+        }
+        if (!fused) {
+            b.tea_leaf_ppcg_update_z_kernel_cuda_();
             b.check();
         }
         return;

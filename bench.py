@@ -77,9 +77,12 @@ def weights_for(n):
     return REPO / "tealeaf_cuda_amd" / "weights" / f"warmstart_{n}.tlws"
 
 
+WEIGHTS = {}
+
+
 def variants(n):
     return {
-        "headline": dict(inner_precision="fp32", warm_start=str(weights_for(n))),
+        "headline": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n)))),
         "fp32_extrapolate": dict(inner_precision="fp32", warm_start="extrapolate"),
         "fp32": dict(inner_precision="fp32"),
         "fp64_true": dict(inner_precision="fp64"),
@@ -107,6 +110,8 @@ def run_gpu(n, warmup, steps, tmp, variant):
 
     def on_step(step):
         stream = torch.cuda.ExternalStream(eng.lib.tea_engine_stream())
+        if step > warmup and (step - warmup) % 20 == 0 and step < warmup + steps:
+            print(f"[bench] {variant}: {step - warmup} of {steps} timed steps done", file=sys.stderr, flush=True)
         if step == warmup:
             eng.synchronize()
             eng.profile_reset()
@@ -217,6 +222,8 @@ def main():
     ap.add_argument("--steps", type=int, default=19)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--weights", default=None,
+                    help="warm-start weight file (default: the one trained at --size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", default=None,
                     help="comma-separated variants to run (profiling); the first one then gives value")
@@ -226,8 +233,9 @@ def main():
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     n = args.size
-    if not weights_for(n).exists():
-        print(f"bench.py: no warm-start weights for {n}^2 ({weights_for(n)})", file=sys.stderr)
+    WEIGHTS[n] = Path(args.weights) if args.weights else weights_for(n)
+    if not WEIGHTS[n].exists():
+        print(f"bench.py: no warm-start weights for {n}^2 ({WEIGHTS[n]})", file=sys.stderr)
         sys.exit(2)
     chosen = [v for v in variants(n) if args.only is None or v in args.only.split(",")]
     if not chosen:
@@ -270,7 +278,7 @@ def main():
         "dtype": "f32 fields, f64 accumulators and residual",
         "data": "synthetic",
         "config": {
-            "workload": f"PPCG, jac_block, 10 inner steps, learned warm start, FP32 fields with FP64 residual "
+            "workload": f"PPCG, jac_block, 10 inner steps, learned warm start ({WEIGHTS[n].name}), FP32 fields with FP64 residual "
                         f"replacement, {n}x{n} cells, random-rectangle layout seed {LAYOUT_SEED}, "
                         "stop at FP64 true relative residual 1e-10, dt 0.004, halo_depth 1",
             "cells": n * n,

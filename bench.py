@@ -54,9 +54,9 @@ ALGO_BYTES = {
     # one whole inner step (the reference's update_r + calc_sd_new launches, jac_block)
     "k_ppcg_inner_fused": 112,
     "k_ppcg_inner_fused_f32": 56,     # the same arrays in FP32
-    # two whole inner steps per launch (temporally blocked): twice the reference's bytes
-    "k_ppcg_inner2": 224,
-    "k_ppcg_inner2_f32": 112,
+    # n whole inner steps per launch (temporally blocked): n times the reference's bytes
+    **{f"k_ppcg_inner{n}": 112 * n for n in range(1, 6)},
+    **{f"k_ppcg_inner{n}_f32": 56 * n for n in range(1, 6)},
     "ppcg_update_r": 40,              # rtemp, sd, Kx, Ky read; rtemp written
     "k_ppcg_calc_sd": 72,             # jac_block: rtemp, sd, utemp, cp, bfp, Kx, Ky read; sd, utemp written
     "cg_calc_w": 32,                  # p, Kx, Ky read; w written

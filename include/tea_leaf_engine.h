@@ -226,10 +226,17 @@ void tea_engine_finalize(void);
 void tea_engine_ppcg_begin(int with_p, double beta, double *pw_rro);
 void tea_engine_ppcg_ur(double alpha, double theta, double *rrn);
 void tea_engine_ppcg_end(double *rrn);
-/* Inner steps `step` and `step + 1` (1-based) in one pass over HBM: the same
- * results as tea_leaf_ppcg_inner_kernel_cuda_ called for both steps with
- * bounds_extra 0 and all four faces external (one chunk). */
+/* Inner steps `step` .. `step + nsteps - 1` (1-based; nsteps 1..5) in one
+ * pass over HBM (temporal blocking): the same results as
+ * tea_leaf_ppcg_inner_kernel_cuda_ called for each of the steps with
+ * bounds_extra 0 and all four faces external (one chunk).
+ * tea_engine_ppcg_inner2(step) is tea_engine_ppcg_inner_n(step, 2).
+ * tea_engine_ppcg_inner_plan(remaining) returns how many of `remaining`
+ * consecutive steps the next call should take: the first pass of the
+ * cheapest grouping for the current precision mode (e.g. 10 steps: 4, 3, 3). */
+void tea_engine_ppcg_inner_n(int step, int nsteps);
 void tea_engine_ppcg_inner2(int step);
+int tea_engine_ppcg_inner_plan(int remaining);
 
 /* ---- mixed precision: FP32 inner solver under FP64 refinement ------------ *
  * tea_engine_set_precision(32) switches the CG and PPCG entry points

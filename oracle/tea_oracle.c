@@ -864,13 +864,19 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
     tea_leaf_ppcg_calc_rrn_kernel_cuda_(rrn);
 }
 
-EXPORT void tea_engine_ppcg_inner2(int step)
+EXPORT void tea_engine_ppcg_inner_n(int step, int nsteps)
 {
     int neighbours[4] = {EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE};
-    int be = 0, s1 = step, s2 = step + 1;
-    tea_leaf_ppcg_inner_kernel_cuda_(&s1, &be, neighbours);
-    tea_leaf_ppcg_inner_kernel_cuda_(&s2, &be, neighbours);
+    for (int k = 0; k < nsteps; k++) {
+        int be = 0, s = step + k;
+        tea_leaf_ppcg_inner_kernel_cuda_(&s, &be, neighbours);
+    }
 }
+
+EXPORT void tea_engine_ppcg_inner2(int step) { tea_engine_ppcg_inner_n(step, 2); }
+
+/* Any grouping of the steps gives the same results; one step per call here. */
+EXPORT int tea_engine_ppcg_inner_plan(int remaining) { return remaining > 0 ? 1 : 0; }
 
 EXPORT int tea_engine_set_precision(int bits)
 {

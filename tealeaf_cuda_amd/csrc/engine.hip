@@ -1227,21 +1227,70 @@ template <typename T> void ppcg_inner_impl(const Ext &e, double alpha, double be
                   beta);
 }
 
-// Two inner steps in one pass (k_ppcg_inner2) on the interior.  Waves per
-// workgroup (tile height): measured fastest at 4096^2, 16 for FP32 (99 vs
-// 117 us per step with 10) and 10 for FP64 (190 vs 200 us with 16).
-template <typename T> constexpr int inner2_waves() { return std::is_same<T, float>::value ? 16 : 10; }
+// NS inner steps in one pass (k_ppcg_inner_n_x2 for FP32, k_ppcg_inner_n for
+// FP64) on the interior.  16 waves per workgroup measured fastest on MI355X
+// for every NS and both field types (against 8, 10 and 12).
+constexpr int INNER_WAVES = 16;
 
-template <typename T> void ppcg_inner2_impl(double a1, double b1, double a2, double b2)
+// Microseconds per pass of NS = 1..MAX_PASS_STEPS inner steps at 4096^2 on
+// MI355X (block Jacobi; the ratios hold at 8192^2): the plan of passes for a
+// run of inner steps minimises their sum.
+constexpr double PASS_US_F32[MAX_PASS_STEPS] = {145, 179, 217, 278, 362};
+constexpr double PASS_US_F64[MAX_PASS_STEPS] = {250, 321, 386, 494, 766};
+
+int plan_first_pass(int remaining, const double *cost)
 {
-    constexpr int INNER2_WAVES = inner2_waves<T>();
+    if (remaining <= 0) return 0;
+    std::vector<double> best((size_t)remaining + 1, 0.0);
+    std::vector<int> first((size_t)remaining + 1, 0);
+    for (int r = 1; r <= remaining; r++) {
+        best[r] = 1e300;
+        for (int n = std::min(r, MAX_PASS_STEPS); n >= 1; n--) {   // ties: the longer pass first
+            const double v = cost[n - 1] + best[r - n];
+            if (v < best[r]) {
+                best[r] = v;
+                first[r] = n;
+            }
+        }
+    }
+    return first[remaining];
+}
+
+template <typename T, int NS> void launch_inner_n(const Ext &e, long pt, const StepCoefs &cf)
+{
+    constexpr bool X2 = std::is_same<T, float>::value;
+    using Tile = InnerTile<X2, INNER_WAVES, NS>;
+    static const char *names[MAX_PASS_STEPS] = {"k_ppcg_inner1", "k_ppcg_inner2", "k_ppcg_inner3", "k_ppcg_inner4",
+                                                "k_ppcg_inner5"};
+    const long ntiles = (long)((e.x1 - e.x0 + Tile::OUT_X) / Tile::OUT_X) * ((e.y1 - e.y0 + Tile::OUT_Y) / Tile::OUT_Y);
+    const unsigned grid = (unsigned)std::min<long>(ntiles, grid_max());
+    Timed t(nm<T>(names[NS - 1]));
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(INNER_WAVES * WAVE), 0, g.stream, e, pt, (const T *)FT<T>(F_SD),
+                           FT<T>(X_SD2), (const T *)FT<T>(X_RTEMP), FT<T>(X_RT2), FT<T>(X_UTEMP),
+                           (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
+    };
+    if constexpr (X2) {
+        if (g.prec == PREC_BLOCK) go(k_ppcg_inner_n_x2<PREC_BLOCK, INNER_WAVES, NS>);
+        else if (g.prec == PREC_DIAG) go(k_ppcg_inner_n_x2<PREC_DIAG, INNER_WAVES, NS>);
+        else go(k_ppcg_inner_n_x2<PREC_NONE, INNER_WAVES, NS>);
+    } else {
+        if (g.prec == PREC_BLOCK) go(k_ppcg_inner_n<T, PREC_BLOCK, INNER_WAVES, NS>);
+        else if (g.prec == PREC_DIAG) go(k_ppcg_inner_n<T, PREC_DIAG, INNER_WAVES, NS>);
+        else go(k_ppcg_inner_n<T, PREC_NONE, INNER_WAVES, NS>);
+    }
+}
+
+// Steps k0+1 .. k0+ns (0-based coefficient index k0) in one pass.
+template <typename T> void ppcg_inner_n_impl(int k0, int ns)
+{
     const long pt = pitch_of<T>();
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
     const unsigned ring_blocks = (unsigned)((ring + 255) / 256);
     bool &dirty = std::is_same<T, double>::value ? g.sd_halo_dirty : g.sd32_halo_dirty;
     bool &rt_dirty = g.rt_halo_dirty[std::is_same<T, double>::value ? 0 : 1];
     if (dirty || rt_dirty) {
-        Timed t("inner2_halo_rings");
+        Timed t("inner_halo_rings");
         if (dirty) {
             hipLaunchKernelGGL(k_copy_ring<T>, dim3(ring_blocks), dim3(256), 0, g.stream, FT<T>(X_SD2),
                                (const T *)FT<T>(F_SD), pt, g.h, g.nx, g.ny);
@@ -1253,32 +1302,21 @@ template <typename T> void ppcg_inner2_impl(double a1, double b1, double a2, dou
             rt_dirty = false;
         }
     }
+    StepCoefs cf{};
+    for (int k = 0; k < ns; k++) {
+        cf.a[k] = g.ch_a[k0 + k];
+        cf.b[k] = g.ch_b[k0 + k];
+    }
     const Ext e = interior();
-    // FP32: two columns per lane (k_ppcg_inner2_x2), FP64: one
-    constexpr bool X2 = std::is_same<T, float>::value;
-    constexpr int OUT_X = X2 ? 2 * WAVE - 4 : WAVE - 2, OUT_Y = (INNER2_WAVES - 2) * STRIP;
-    const long ntiles = (long)((e.x1 - e.x0 + OUT_X) / OUT_X) * ((e.y1 - e.y0 + OUT_Y) / OUT_Y);
-    const unsigned grid = (unsigned)std::min<long>(ntiles, grid_max());
-    {
-        Timed t(nm<T>("k_ppcg_inner2"));
-        auto go = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, dim3(grid), dim3(INNER2_WAVES * WAVE), 0, g.stream, e, pt,
-                               (const T *)FT<T>(F_SD), FT<T>(X_SD2), (const T *)FT<T>(X_RTEMP), FT<T>(X_RT2),
-                               FT<T>(X_UTEMP), (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX),
-                               (const T *)FT<T>(X_KY), a1, b1, a2, b2);
-        };
-        if constexpr (X2) {
-            if (g.prec == PREC_BLOCK) go(k_ppcg_inner2_x2<PREC_BLOCK, INNER2_WAVES>);
-            else if (g.prec == PREC_DIAG) go(k_ppcg_inner2_x2<PREC_DIAG, INNER2_WAVES>);
-            else go(k_ppcg_inner2_x2<PREC_NONE, INNER2_WAVES>);
-        } else {
-            if (g.prec == PREC_BLOCK) go(k_ppcg_inner2<T, PREC_BLOCK, INNER2_WAVES>);
-            else if (g.prec == PREC_DIAG) go(k_ppcg_inner2<T, PREC_DIAG, INNER2_WAVES>);
-            else go(k_ppcg_inner2<T, PREC_NONE, INNER2_WAVES>);
-        }
+    switch (ns) {
+    case 1: launch_inner_n<T, 1>(e, pt, cf); break;
+    case 2: launch_inner_n<T, 2>(e, pt, cf); break;
+    case 3: launch_inner_n<T, 3>(e, pt, cf); break;
+    case 4: launch_inner_n<T, 4>(e, pt, cf); break;
+    default: launch_inner_n<T, 5>(e, pt, cf); break;
     }
     hipError_t err = hipGetLastError();
-    if (err != hipSuccess) { die("launch of k_ppcg_inner2 failed: %s", hipGetErrorString(err)); return; }
+    if (err != hipSuccess) { die("launch of k_ppcg_inner%d failed: %s", ns, hipGetErrorString(err)); return; }
     if (std::is_same<T, double>::value) {
         std::swap(g.f[F_SD], g.f[X_SD2]);
         std::swap(g.fs[F_SD], g.fs[X_SD2]);
@@ -1590,19 +1628,33 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
     BY_PRECISION(ppcg_end_impl, rrn);
 }
 
-// Inner steps `step` and `step + 1` (1-based) over the interior in one pass:
-// tea_leaf_ppcg_inner_kernel_cuda_ twice with bounds_extra 0 and every face
-// external (one chunk).
-EXPORT void tea_engine_ppcg_inner2(int step)
+// Inner steps `step` .. `step + nsteps - 1` (1-based, nsteps 1..5) over the
+// interior in one pass: tea_leaf_ppcg_inner_kernel_cuda_ once per step with
+// bounds_extra 0 and every face external (one chunk).
+EXPORT void tea_engine_ppcg_inner_n(int step, int nsteps)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     const int k = step - 1;
-    if (k < 0 || k + 1 >= (int)g.ch_a.size()) {
-        die("ppcg_inner2: steps %d, %d outside the %d uploaded coefficients", step, step + 1, (int)g.ch_a.size());
+    if (nsteps < 1 || nsteps > MAX_PASS_STEPS) {
+        die("ppcg_inner_n: %d steps in one pass (1..%d)", nsteps, MAX_PASS_STEPS);
         return;
     }
-    BY_PRECISION(ppcg_inner2_impl, g.ch_a[k], g.ch_b[k], g.ch_a[k + 1], g.ch_b[k + 1]);
+    if (k < 0 || k + nsteps > (int)g.ch_a.size()) {
+        die("ppcg_inner_n: steps %d..%d outside the %d uploaded coefficients", step, step + nsteps - 1,
+            (int)g.ch_a.size());
+        return;
+    }
+    BY_PRECISION(ppcg_inner_n_impl, k, nsteps);
+}
+
+EXPORT void tea_engine_ppcg_inner2(int step) { tea_engine_ppcg_inner_n(step, 2); }
+
+// Steps the next pass should take out of `remaining` consecutive inner steps
+// (the first pass of the cheapest plan for the current precision mode).
+EXPORT int tea_engine_ppcg_inner_plan(int remaining)
+{
+    return plan_first_pass(remaining, g.fp32 ? PASS_US_F32 : PASS_US_F64);
 }
 
 EXPORT void tea_engine_set_deck_path(const char *path) { g_deck_path = path ? path : "tea.in"; }
@@ -1854,8 +1906,7 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
                     return;
                 {
                     Timed t("ws_network");
-                    ws::im2col(g.ws_acts, g.ws_planes, g.stream);
-                    ws::forward(g.ws_net, g.ws_acts, g.stream);
+                    ws::forward(g.ws_net, g.ws_acts, g.ws_planes, ws::NFEAT, g.stream);
                 }
                 HIP_OK(hipGetLastError());
                 // the network predicts (du - (u0 - up1)) / s: direction s * net

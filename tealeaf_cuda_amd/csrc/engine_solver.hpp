@@ -381,126 +381,20 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_inner_fused(Ext e, long pitch, c
     STRIP_END
 }
 
-// Two PPCG inner steps in one pass (no reference counterpart: the composition
-// of two k_ppcg_inner_fused launches with coefficients (a1, b1) then (a2, b2),
-// bit-identical to them).  A workgroup of NW waves owns an output tile of
-// WAVE-2 columns x (NW-2) strips; wave j takes strip j-1 of the tile (waves 0
-// and NW-1 the strips below and above it), lane l column l-1 (lanes 0 and
-// WAVE-1 the columns left and right of it).  Step 1 runs on the whole
-// (WAVE) x (NW strips) region, so every sd value step 2 needs at the tile's
-// cells is at hand: the strip's own rows in registers, the rows above and below
-// through LDS, the columns left and right through lane shuffles.  Step 2 runs
-// on the tile only.  rtemp is read from rt_in and written to rt_out because
-// step 1 also reads it on the neighbouring tiles' cells; utemp is read and
-// written on the tile's own cells only.  sd outside the extent keeps its value.
-template <typename T, int PREC, int NW>
-__global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner2(Ext e, long pitch, const T *__restrict__ sd_in,
-                                                          T *__restrict__ sd_out, const T *__restrict__ rt_in,
-                                                          T *__restrict__ rt_out, T *__restrict__ utemp,
-                                                          const T *__restrict__ Mi, const T *__restrict__ Kx,
-                                                          const T *__restrict__ Ky, double a1_in, double b1_in,
-                                                          double a2_in, double b2_in)
-{
-    constexpr int OUT_X = WAVE - 2, OUT_Y = (NW - 2) * STRIP;
-    __shared__ T lds_lo[NW][WAVE], lds_hi[NW][WAVE];  // each strip's first and last row of sd after step 1
-    const T a1 = (T)a1_in, b1 = (T)b1_in, a2 = (T)a2_in, b2 = (T)b2_in;
-    const int lane = threadIdx.x & (WAVE - 1), j = threadIdx.x / WAVE;
-    const int ntx = (e.x1 - e.x0 + OUT_X) / OUT_X, nty = (e.y1 - e.y0 + OUT_Y) / OUT_Y;
-    const int ntiles = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : ntx * nty;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int ty = t / ntx, tx = t - ty * ntx;
-        const int c = e.x0 + tx * OUT_X - 1 + lane;
-        const int r0 = e.y0 + ty * OUT_Y + (j - 1) * STRIP;
-        const long i0 = (long)c + (long)r0 * pitch;
-        const bool col_in = c >= e.x0 && c <= e.x1;
-        const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
-        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;   // the strip's rows inside the extent
-        len = len < 0 ? 0 : (len > STRIP ? STRIP : len);
-        const int act = col_in ? len : 0;
-        const bool out = j >= 1 && j <= NW - 2 && lane >= 1 && lane <= WAVE - 2;
-
-        // ---- step 1 on the whole region
-        T sd1[STRIP], rt1[STRIP];
-        {
-            T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-#pragma unroll
-            for (int k = 0; k < STRIP; k++) {
-                const long i = i0 + k * pitch;
-                const int rr = r0 + k;
-                sd1[k] = T(0.0);
-                if (col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) sd1[k] = sd_in[i];
-                if (k < act) {
-                    const T result = apply_A(sd_in, Kx, Ky, i, pitch);
-                    rv[k] = rt_in[i] - result;
-                }
-            }
-            if (PREC == PREC_BLOCK && act > 0) precond_strip<T, PREC_BLOCK>(rv, zv, act, i0, pitch, Mi, Kx, Ky);
-#pragma unroll
-            for (int k = 0; k < STRIP; k++) {
-                rt1[k] = rv[k];
-                if (k < act) {
-                    const long i = i0 + k * pitch;
-                    if (PREC == PREC_BLOCK) sd1[k] = a1 * sd1[k] + b1 * zv[k];
-                    else if (PREC == PREC_DIAG) sd1[k] = a1 * sd1[k] + b1 * rv[k] * Mi[i];
-                    else sd1[k] = a1 * sd1[k] + b1 * rv[k];
-                }
-            }
-        }
-        lds_lo[j][lane] = sd1[0];
-        lds_hi[j][lane] = sd1[STRIP - 1];
-        __syncthreads();
-
-        // ---- step 2 on the tile
-        {
-            T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
-            const int act2 = out ? act : 0;
-#pragma unroll
-            for (int k = 0; k < STRIP; k++) {
-                const T left = __shfl_up(sd1[k], 1, WAVE);
-                const T right = __shfl_down(sd1[k], 1, WAVE);
-                if (k < act2) {
-                    const long i = i0 + k * pitch;
-                    const T below = k == 0 ? lds_hi[j - 1][lane] : sd1[k - 1];
-                    const T above = k == STRIP - 1 ? lds_lo[j + 1][lane] : sd1[k + 1];
-                    const T result = (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * sd1[k]
-                                     - (Ky[i + pitch] * above + Ky[i] * below)
-                                     - (Kx[i + 1] * right + Kx[i] * left);
-                    rv[k] = rt1[k] - result;
-                }
-            }
-            if (PREC == PREC_BLOCK && act2 > 0) precond_strip<T, PREC_BLOCK>(rv, zv, act2, i0, pitch, Mi, Kx, Ky);
-#pragma unroll
-            for (int k = 0; k < STRIP; k++) {
-                if (k < act2) {
-                    const long i = i0 + k * pitch;
-                    T sd2;
-                    if (PREC == PREC_BLOCK) sd2 = a2 * sd1[k] + b2 * zv[k];
-                    else if (PREC == PREC_DIAG) sd2 = a2 * sd1[k] + b2 * rv[k] * Mi[i];
-                    else sd2 = a2 * sd1[k] + b2 * rv[k];
-                    const T ut1 = utemp[i] + sd1[k];
-                    sd_out[i] = sd2;
-                    rt_out[i] = rv[k];
-                    utemp[i] = ut1 + sd2;
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // Block-Jacobi strip solve of one column from its own diagonal and Ky values
 // (diag[k] = diag_A at row k, ky[k] = Ky at row k, k = 0..len): the
 // expressions of block_solve_strip.
-__device__ __forceinline__ void strip_solve_col(const float (&rhs)[STRIP], float (&z)[STRIP], int len,
-                                                const float (&diag)[STRIP], const float (&ky)[STRIP + 1])
+template <typename T>
+__device__ __forceinline__ void strip_solve_col(const T (&rhs)[STRIP], T (&z)[STRIP], int len,
+                                                const T (&diag)[STRIP], const T (&ky)[STRIP + 1])
 {
-    float cp[STRIP], bfp[STRIP], dp[STRIP];
+    T cp[STRIP], bfp[STRIP], dp[STRIP];
     cp[0] = (-ky[1]) / diag[0];
     dp[0] = rhs[0] / diag[0];
 #pragma unroll
     for (int k = 1; k < STRIP; k++) {
         if (k < len) {
-            bfp[k] = 1.0f / (diag[k] - (-ky[k]) * cp[k - 1]);
+            bfp[k] = T(1.0) / (diag[k] - (-ky[k]) * cp[k - 1]);
             cp[k] = (-ky[k + 1]) * bfp[k];
             dp[k] = (rhs[k] - (-ky[k]) * dp[k - 1]) * bfp[k];
         }
@@ -512,32 +406,61 @@ __device__ __forceinline__ void strip_solve_col(const float (&rhs)[STRIP], float
     }
 }
 
-// k_ppcg_inner2 for FP32 fields with two columns per lane (8-byte accesses:
-// an FP32 row segment of a wave is then as long as an FP64 one).  Lane l
-// holds columns X0-2+2l and X0-1+2l of an output tile of 2*WAVE-4 columns
-// starting at X0 (lane 0's and lane WAVE-1's pairs are the halo); otherwise
-// the same scheme and arithmetic as k_ppcg_inner2, bit-identical to two
-// k_ppcg_inner_fused<float> launches.
-template <int PREC, int NW>
-__global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner2_x2(Ext e, long pitch, const float *__restrict__ sd_in,
-                                                             float *__restrict__ sd_out,
-                                                             const float *__restrict__ rt_in,
-                                                             float *__restrict__ rt_out, float *__restrict__ utemp,
-                                                             const float *__restrict__ Mi,
-                                                             const float *__restrict__ Kx,
-                                                             const float *__restrict__ Ky, double a1_in,
-                                                             double b1_in, double a2_in, double b2_in)
+// Coefficients (alpha_k, beta_k) of consecutive inner steps, passed by value.
+constexpr int MAX_PASS_STEPS = 5;
+struct StepCoefs {
+    double a[MAX_PASS_STEPS], b[MAX_PASS_STEPS];
+};
+
+// Tile of a pass of NS inner steps with NW waves per workgroup: HL halo
+// lanes at each side (X2: two columns per lane), NS-1 halo strips at the top
+// and bottom; OUT_X x OUT_Y cells are written.
+template <bool X2, int NW, int NS>
+struct InnerTile {
+    static constexpr int HL = X2 ? NS / 2 : NS - 1;
+    static constexpr int HW = NS - 1;
+    static constexpr int OUT_X = X2 ? 2 * (WAVE - 2 * HL) : WAVE - 2 * HL;
+    static constexpr int OUT_Y = (NW - 2 * HW) * STRIP;
+};
+
+// NS = 1..5 PPCG inner steps in one pass (temporal blocking; no reference
+// counterpart), FP32 fields, two columns per lane (8-byte accesses): the
+// composition of NS k_ppcg_inner_fused<float> launches with coefficients
+// (cf.a[k], cf.b[k]), bit-identical to them.  A workgroup of NW waves loads
+// a region of 2*WAVE columns x NW strips (wave j the strip j, lane l the
+// columns 2l and 2l+1; sd one row beyond the strip at each end, so the
+// block-Jacobi strips are whole); step k (k = 1..NS) runs on that region minus k-1
+// strips at the top and bottom and k-1 columns at each side, so the tile
+// written at the end is NW - 2(NS-1) strips high and 2*WAVE - 4*(NS/2)
+// columns wide.  Between steps each wave passes its strip's first and last
+// rows of sd to the waves below and above through LDS (two buffers by step
+// parity, one barrier per step); columns come from lane shuffles (memory at
+// the wave's edge lanes in step 1, stale values there afterwards, which only
+// reach cells outside the tile).  sd and rtemp are read from the _in and
+// written to the _out buffers (the neighbouring tiles read them), utemp only
+// on the tile's own cells.  Cells outside the extent keep their sd, as in the
+// reference, where inner steps update the interior only.
+template <int PREC, int NW, int NS>
+__global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n_x2(Ext e, long pitch, const float *__restrict__ sd_in,
+                                                              float *__restrict__ sd_out,
+                                                              const float *__restrict__ rt_in,
+                                                              float *__restrict__ rt_out, float *__restrict__ utemp,
+                                                              const float *__restrict__ Mi,
+                                                              const float *__restrict__ Kx,
+                                                              const float *__restrict__ Ky, StepCoefs cf)
 {
-    constexpr int OUT_X = 2 * WAVE - 4, OUT_Y = (NW - 2) * STRIP;
-    __shared__ float2 lds_lo[NW][WAVE], lds_hi[NW][WAVE];
-    const float a1 = (float)a1_in, b1 = (float)b1_in, a2 = (float)a2_in, b2 = (float)b2_in;
+    static_assert(NS >= 1 && NS <= MAX_PASS_STEPS && NW - 2 * (NS - 1) >= 1, "tile too small for NS steps");
+    using Tile = InnerTile<true, NW, NS>;
+    constexpr int HL = Tile::HL, HW = Tile::HW, OUT_X = Tile::OUT_X, OUT_Y = Tile::OUT_Y;
+    __shared__ float2 lds[2][2][NW][WAVE];   // [step parity][first row, last row][wave][lane]
     const int lane = threadIdx.x & (WAVE - 1), j = threadIdx.x / WAVE;
     const int ntx = (e.x1 - e.x0 + OUT_X) / OUT_X, nty = (e.y1 - e.y0 + OUT_Y) / OUT_Y;
     const int ntiles = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : ntx * nty;
+    const int jb = j > 0 ? j - 1 : 0, ja = j < NW - 1 ? j + 1 : NW - 1;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int ty = t / ntx, tx = t - ty * ntx;
-        const int c = e.x0 + tx * OUT_X - 2 + 2 * lane;   // the lane's first column
-        const int r0 = e.y0 + ty * OUT_Y + (j - 1) * STRIP;
+        const int c = e.x0 + tx * OUT_X - 2 * HL + 2 * lane;   // the lane's first column
+        const int r0 = e.y0 + ty * OUT_Y + (j - HW) * STRIP;
         const long i0 = (long)c + (long)r0 * pitch;
         const bool pair_mem = c >= e.x0 - 2 && c <= e.x1 + 1;
         int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
@@ -545,11 +468,11 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner2_x2(Ext e, long pitch,
         int act[2];
         act[0] = (c >= e.x0 && c <= e.x1) ? len : 0;
         act[1] = (c + 1 >= e.x0 && c + 1 <= e.x1) ? len : 0;
-        const bool out = j >= 1 && j <= NW - 2 && lane >= 1 && lane <= WAVE - 2;
+        const bool out = j >= HW && j <= NW - 1 - HW && lane >= HL && lane <= WAVE - 1 - HL;
         const bool right_lane = lane < WAVE - 1;
 
         // ---- loads: sd rows r0-1 .. r0+STRIP, rtemp, Kx, Ky (rows r0 .. r0+STRIP), utemp
-        float sv[2][STRIP + 2], kxv[2][STRIP], kyv[2][STRIP + 1], rtv[2][STRIP], utv[2][STRIP], miv[2][STRIP];
+        float sv[2][STRIP + 2], kxv[2][STRIP], kyv[2][STRIP + 1], rtc[2][STRIP], utv[2][STRIP], miv[2][STRIP];
 #pragma unroll
         for (int q = 0; q < STRIP + 2; q++) {
             const int rr = r0 - 1 + q;
@@ -578,7 +501,7 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner2_x2(Ext e, long pitch,
                 if (PREC == PREC_DIAG) mi = *(const float2 *)(Mi + i);
             }
             kxv[0][k] = kx.x; kxv[1][k] = kx.y;
-            rtv[0][k] = rt.x; rtv[1][k] = rt.y;
+            rtc[0][k] = rt.x; rtc[1][k] = rt.y;
             utv[0][k] = ut.x; utv[1][k] = ut.y;
             miv[0][k] = mi.x; miv[1][k] = mi.y;
         }
@@ -596,119 +519,235 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner2_x2(Ext e, long pitch,
             diag[0][k] = 1.0f + (kyv[0][k + 1] + kyv[0][k]) + (kxv[1][k] + kxv[0][k]);
             diag[1][k] = 1.0f + (kyv[1][k + 1] + kyv[1][k]) + (kxr[k] + kxv[1][k]);
         }
+        float sdc[2][STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            sdc[0][k] = sv[0][k + 1];
+            sdc[1][k] = sv[1][k + 1];
+        }
 
-        // ---- step 1 on the whole region
-        float sd1[2][STRIP], rt1[2][STRIP];
-        {
+#pragma unroll
+        for (int st = 0; st < NS; st++) {
+            const float a = (float)cf.a[st], b = (float)cf.b[st];
+            const bool last = st == NS - 1;
+            const int ak[2] = {(!last || out) ? act[0] : 0, (!last || out) ? act[1] : 0};
+            float lo[2], hi[2];   // sd of the row below and above the strip
+            if (st == 0) {
+                lo[0] = sv[0][0]; lo[1] = sv[1][0];
+                hi[0] = sv[0][STRIP + 1]; hi[1] = sv[1][STRIP + 1];
+            } else {
+                const float2 vb = lds[(st - 1) & 1][1][jb][lane], va = lds[(st - 1) & 1][0][ja][lane];
+                lo[0] = vb.x; lo[1] = vb.y;
+                hi[0] = va.x; hi[1] = va.y;
+            }
             float rv[2][STRIP], zv[2][STRIP];
 #pragma unroll
             for (int k = 0; k < STRIP; k++) {
-                const float from_left = __shfl_up(sv[1][k + 1], 1, WAVE);
-                const float from_right = __shfl_down(sv[0][k + 1], 1, WAVE);
+                const float from_left = __shfl_up(sdc[1][k], 1, WAVE);
+                const float from_right = __shfl_down(sdc[0][k], 1, WAVE);
                 const long i = i0 + (long)k * pitch;
                 rv[0][k] = rv[1][k] = zv[0][k] = zv[1][k] = 0.0f;
-                sd1[0][k] = sv[0][k + 1];
-                sd1[1][k] = sv[1][k + 1];
-                if (k < act[0]) {
-                    const float left = lane > 0 ? from_left : sd_in[i - 1];
-                    const float res = diag[0][k] * sv[0][k + 1] - (kyv[0][k + 1] * sv[0][k + 2] + kyv[0][k] * sv[0][k])
-                                      - (kxv[1][k] * sv[1][k + 1] + kxv[0][k] * left);
-                    rv[0][k] = rtv[0][k] - res;
-                }
-                if (k < act[1]) {
-                    const float right = right_lane ? from_right : sd_in[i + 2];
-                    const float res = diag[1][k] * sv[1][k + 1] - (kyv[1][k + 1] * sv[1][k + 2] + kyv[1][k] * sv[1][k])
-                                      - (kxr[k] * right + kxv[1][k] * sv[0][k + 1]);
-                    rv[1][k] = rtv[1][k] - res;
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    if (k < ak[q]) {
+                        const float below = k == 0 ? lo[q] : sdc[q][k - 1];
+                        const float above = k == STRIP - 1 ? hi[q] : sdc[q][k + 1];
+                        float left, right;
+                        if (q == 0) {
+                            left = (st > 0 || lane > 0) ? from_left : sd_in[i - 1];
+                            right = sdc[1][k];
+                        } else {
+                            left = sdc[0][k];
+                            right = (st > 0 || right_lane) ? from_right : sd_in[i + 2];
+                        }
+                        const float kr = q ? kxr[k] : kxv[1][k];
+                        const float res = diag[q][k] * sdc[q][k] - (kyv[q][k + 1] * above + kyv[q][k] * below)
+                                          - (kr * right + kxv[q][k] * left);
+                        rv[q][k] = rtc[q][k] - res;
+                    }
                 }
             }
 #pragma unroll
             for (int q = 0; q < 2; q++) {
-                if (PREC == PREC_BLOCK && act[q] > 0) {
+                if (PREC == PREC_BLOCK && ak[q] > 0) {
                     float dq[STRIP], kq[STRIP + 1];
 #pragma unroll
                     for (int k = 0; k < STRIP; k++) dq[k] = diag[q][k];
 #pragma unroll
                     for (int k = 0; k < STRIP + 1; k++) kq[k] = kyv[q][k];
-                    strip_solve_col(rv[q], zv[q], act[q], dq, kq);
+                    strip_solve_col(rv[q], zv[q], ak[q], dq, kq);
                 }
 #pragma unroll
                 for (int k = 0; k < STRIP; k++) {
-                    rt1[q][k] = rv[q][k];
-                    if (k < act[q]) {
-                        if (PREC == PREC_BLOCK) sd1[q][k] = a1 * sd1[q][k] + b1 * zv[q][k];
-                        else if (PREC == PREC_DIAG) sd1[q][k] = a1 * sd1[q][k] + b1 * rv[q][k] * miv[q][k];
-                        else sd1[q][k] = a1 * sd1[q][k] + b1 * rv[q][k];
+                    if (k < ak[q]) {
+                        float sdn;
+                        if (PREC == PREC_BLOCK) sdn = a * sdc[q][k] + b * zv[q][k];
+                        else if (PREC == PREC_DIAG) sdn = a * sdc[q][k] + b * rv[q][k] * miv[q][k];
+                        else sdn = a * sdc[q][k] + b * rv[q][k];
+                        sdc[q][k] = sdn;
+                        rtc[q][k] = rv[q][k];
+                        utv[q][k] = utv[q][k] + sdn;
                     }
                 }
+            }
+            if (!last) {
+                lds[st & 1][0][j][lane] = make_float2(sdc[0][0], sdc[1][0]);
+                lds[st & 1][1][j][lane] = make_float2(sdc[0][STRIP - 1], sdc[1][STRIP - 1]);
+                __syncthreads();
             }
         }
-        lds_lo[j][lane] = make_float2(sd1[0][0], sd1[1][0]);
-        lds_hi[j][lane] = make_float2(sd1[0][STRIP - 1], sd1[1][STRIP - 1]);
-        __syncthreads();
 
-        // ---- step 2 on the tile
-        {
-            const int act2[2] = {out ? act[0] : 0, out ? act[1] : 0};
-            float rv[2][STRIP], zv[2][STRIP];
-            const float2 lo_below = lds_hi[j > 0 ? j - 1 : 0][lane];
-            const float2 hi_above = lds_lo[j < NW - 1 ? j + 1 : NW - 1][lane];
+        // ---- the tile's cells
+        if (out) {
 #pragma unroll
             for (int k = 0; k < STRIP; k++) {
-                const float from_left = __shfl_up(sd1[1][k], 1, WAVE);
-                const float from_right = __shfl_down(sd1[0][k], 1, WAVE);
-                rv[0][k] = rv[1][k] = zv[0][k] = zv[1][k] = 0.0f;
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    if (k < act2[q]) {
-                        const float below = k == 0 ? (q ? lo_below.y : lo_below.x) : sd1[q][k - 1];
-                        const float above = k == STRIP - 1 ? (q ? hi_above.y : hi_above.x) : sd1[q][k + 1];
-                        const float left = q ? sd1[0][k] : from_left;
-                        const float right = q ? from_right : sd1[1][k];
-                        const float kr = q ? kxr[k] : kxv[1][k];
-                        const float res = diag[q][k] * sd1[q][k] - (kyv[q][k + 1] * above + kyv[q][k] * below)
-                                          - (kr * right + kxv[q][k] * left);
-                        rv[q][k] = rt1[q][k] - res;
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                if (PREC == PREC_BLOCK && act2[q] > 0) {
-                    float dq[STRIP], kq[STRIP + 1];
-#pragma unroll
-                    for (int k = 0; k < STRIP; k++) dq[k] = diag[q][k];
-#pragma unroll
-                    for (int k = 0; k < STRIP + 1; k++) kq[k] = kyv[q][k];
-                    strip_solve_col(rv[q], zv[q], act2[q], dq, kq);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < STRIP; k++) {
-                float sdo[2], uto[2];
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    if (PREC == PREC_BLOCK) sdo[q] = a2 * sd1[q][k] + b2 * zv[q][k];
-                    else if (PREC == PREC_DIAG) sdo[q] = a2 * sd1[q][k] + b2 * rv[q][k] * miv[q][k];
-                    else sdo[q] = a2 * sd1[q][k] + b2 * rv[q][k];
-                    const float ut1 = utv[q][k] + sd1[q][k];
-                    uto[q] = ut1 + sdo[q];
-                }
                 const long i = i0 + (long)k * pitch;
-                if (k < act2[0] && k < act2[1]) {
-                    *(float2 *)(sd_out + i) = make_float2(sdo[0], sdo[1]);
-                    *(float2 *)(rt_out + i) = make_float2(rv[0][k], rv[1][k]);
-                    *(float2 *)(utemp + i) = make_float2(uto[0], uto[1]);
+                if (k < act[0] && k < act[1]) {
+                    *(float2 *)(sd_out + i) = make_float2(sdc[0][k], sdc[1][k]);
+                    *(float2 *)(rt_out + i) = make_float2(rtc[0][k], rtc[1][k]);
+                    *(float2 *)(utemp + i) = make_float2(utv[0][k], utv[1][k]);
                 } else {
 #pragma unroll
                     for (int q = 0; q < 2; q++)
-                        if (k < act2[q]) {
-                            sd_out[i + q] = sdo[q];
-                            rt_out[i + q] = rv[q][k];
-                            utemp[i + q] = uto[q];
+                        if (k < act[q]) {
+                            sd_out[i + q] = sdc[q][k];
+                            rt_out[i + q] = rtc[q][k];
+                            utemp[i + q] = utv[q][k];
                         }
                 }
             }
+        }
+        __syncthreads();
+    }
+}
+
+// NS = 1..5 PPCG inner steps in one pass, one column per lane (any field
+// type; used for FP64): the composition of NS k_ppcg_inner_fused<T>
+// launches, bit-identical to them.  The scheme of k_ppcg_inner_n_x2: the
+// tile written at the end is NW - 2(NS-1) strips high and WAVE - 2(NS-1)
+// columns wide.
+template <typename T, int PREC, int NW, int NS>
+__global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n(Ext e, long pitch, const T *__restrict__ sd_in,
+                                                           T *__restrict__ sd_out, const T *__restrict__ rt_in,
+                                                           T *__restrict__ rt_out, T *__restrict__ utemp,
+                                                           const T *__restrict__ Mi, const T *__restrict__ Kx,
+                                                           const T *__restrict__ Ky, StepCoefs cf)
+{
+    static_assert(NS >= 1 && NS <= MAX_PASS_STEPS && NW - 2 * (NS - 1) >= 1, "tile too small for NS steps");
+    using Tile = InnerTile<false, NW, NS>;
+    constexpr int HL = Tile::HL, HW = Tile::HW, OUT_X = Tile::OUT_X, OUT_Y = Tile::OUT_Y;
+    __shared__ T lds[2][2][NW][WAVE];   // [step parity][first row, last row][wave][lane]
+    const int lane = threadIdx.x & (WAVE - 1), j = threadIdx.x / WAVE;
+    const int ntx = (e.x1 - e.x0 + OUT_X) / OUT_X, nty = (e.y1 - e.y0 + OUT_Y) / OUT_Y;
+    const int ntiles = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : ntx * nty;
+    const int jb = j > 0 ? j - 1 : 0, ja = j < NW - 1 ? j + 1 : NW - 1;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int ty = t / ntx, tx = t - ty * ntx;
+        const int c = e.x0 + tx * OUT_X - HL + lane;
+        const int r0 = e.y0 + ty * OUT_Y + (j - HW) * STRIP;
+        const long i0 = (long)c + (long)r0 * pitch;
+        const bool col_in = c >= e.x0 && c <= e.x1;
+        const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
+        len = len < 0 ? 0 : (len > STRIP ? STRIP : len);
+        const int act = col_in ? len : 0;
+        const bool out = j >= HW && j <= NW - 1 - HW && lane >= HL && lane <= WAVE - 1 - HL;
+        const bool right_lane = lane < WAVE - 1;
+
+        // ---- loads: sd rows r0-1 .. r0+STRIP, rtemp, Kx, Ky (rows r0 .. r0+STRIP), utemp
+        T sv[STRIP + 2], kxv[STRIP], kyv[STRIP + 1], rtc[STRIP], utv[STRIP], miv[STRIP];
+#pragma unroll
+        for (int q = 0; q < STRIP + 2; q++) {
+            const int rr = r0 - 1 + q;
+            sv[q] = T(0.0);
+            if (col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) sv[q] = sd_in[i0 + (long)(q - 1) * pitch];
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) {
+            kyv[k] = T(0.0);
+            if (k <= act && act > 0) kyv[k] = Ky[i0 + (long)k * pitch];
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            kxv[k] = rtc[k] = utv[k] = miv[k] = T(0.0);
+            if (k < act) {
+                const long i = i0 + (long)k * pitch;
+                kxv[k] = Kx[i];
+                rtc[k] = rt_in[i];
+                if (out) utv[k] = utemp[i];
+                if (PREC == PREC_DIAG) miv[k] = Mi[i];
+            }
+        }
+        // Kx on the cell's right face (the next lane's Kx)
+        T kxr[STRIP], diag[STRIP], sdc[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const T sh = __shfl_down(kxv[k], 1, WAVE);
+            kxr[k] = T(0.0);
+            if (k < act) kxr[k] = (right_lane && c + 1 <= e.x1) ? sh : Kx[i0 + (long)k * pitch + 1];
+            diag[k] = T(1.0) + (kyv[k + 1] + kyv[k]) + (kxr[k] + kxv[k]);
+            sdc[k] = sv[k + 1];
+        }
+
+#pragma unroll
+        for (int st = 0; st < NS; st++) {
+            const T a = (T)cf.a[st], b = (T)cf.b[st];
+            const bool last = st == NS - 1;
+            const int ak = (!last || out) ? act : 0;
+            T lo, hi;   // sd of the row below and above the strip
+            if (st == 0) {
+                lo = sv[0];
+                hi = sv[STRIP + 1];
+            } else {
+                lo = lds[(st - 1) & 1][1][jb][lane];
+                hi = lds[(st - 1) & 1][0][ja][lane];
+            }
+            T rv[STRIP], zv[STRIP];
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const T from_left = __shfl_up(sdc[k], 1, WAVE);
+                const T from_right = __shfl_down(sdc[k], 1, WAVE);
+                const long i = i0 + (long)k * pitch;
+                rv[k] = zv[k] = T(0.0);
+                if (k < ak) {
+                    const T below = k == 0 ? lo : sdc[k - 1];
+                    const T above = k == STRIP - 1 ? hi : sdc[k + 1];
+                    const T left = (st > 0 || lane > 0) ? from_left : sd_in[i - 1];
+                    const T right = (st > 0 || right_lane) ? from_right : sd_in[i + 1];
+                    const T res = diag[k] * sdc[k] - (kyv[k + 1] * above + kyv[k] * below)
+                                  - (kxr[k] * right + kxv[k] * left);
+                    rv[k] = rtc[k] - res;
+                }
+            }
+            if (PREC == PREC_BLOCK && ak > 0) strip_solve_col<T>(rv, zv, ak, diag, kyv);
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                if (k < ak) {
+                    T sdn;
+                    if (PREC == PREC_BLOCK) sdn = a * sdc[k] + b * zv[k];
+                    else if (PREC == PREC_DIAG) sdn = a * sdc[k] + b * rv[k] * miv[k];
+                    else sdn = a * sdc[k] + b * rv[k];
+                    sdc[k] = sdn;
+                    rtc[k] = rv[k];
+                    utv[k] = utv[k] + sdn;
+                }
+            }
+            if (!last) {
+                lds[st & 1][0][j][lane] = sdc[0];
+                lds[st & 1][1][j][lane] = sdc[STRIP - 1];
+                __syncthreads();
+            }
+        }
+
+        if (out) {
+#pragma unroll
+            for (int k = 0; k < STRIP; k++)
+                if (k < act) {
+                    const long i = i0 + (long)k * pitch;
+                    sd_out[i] = sdc[k];
+                    rt_out[i] = rtc[k];
+                    utemp[i] = utv[k];
+                }
         }
         __syncthreads();
     }

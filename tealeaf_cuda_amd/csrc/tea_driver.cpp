@@ -91,7 +91,8 @@ struct Backend {
     void (*ppcg_begin)(int, double, double *) = nullptr;
     void (*ppcg_ur)(double, double, double *) = nullptr;
     void (*ppcg_end)(double *) = nullptr;
-    void (*ppcg_inner2)(int) = nullptr;
+    void (*ppcg_inner_n)(int, int) = nullptr;
+    int (*ppcg_inner_plan)(int) = nullptr;
 
     void check() const
     {
@@ -127,7 +128,8 @@ Backend load_backend(const char *path)
     b.ppcg_begin = reinterpret_cast<void (*)(int, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_begin"));
     b.ppcg_ur = reinterpret_cast<void (*)(double, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_ur"));
     b.ppcg_end = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_ppcg_end"));
-    b.ppcg_inner2 = reinterpret_cast<void (*)(int)>(dlsym(b.handle, "tea_engine_ppcg_inner2"));
+    b.ppcg_inner_n = reinterpret_cast<void (*)(int, int)>(dlsym(b.handle, "tea_engine_ppcg_inner_n"));
+    b.ppcg_inner_plan = reinterpret_cast<int (*)(int)>(dlsym(b.handle, "tea_engine_ppcg_inner_plan"));
     return b;
 }
 
@@ -722,19 +724,16 @@ void Run::run_ppcg_inner_steps(int *ppcg_inner_iters, bool fused)
         b.tea_leaf_ppcg_init_sd_new_kernel_cuda_(&th);
         b.check();
     }
-    const char *env = std::getenv("TEA_INNER2");
-    if (fused_outer() && b.ppcg_inner2 && !(env && env[0] == '0')) {
+    const char *env = std::getenv("TEA_INNER_PASSES");
+    if (fused_outer() && b.ppcg_inner_n && b.ppcg_inner_plan && !(env && env[0] == '0')) {
         // one chunk, no reflective update: the steps run on the interior in
-        // order, two per pass
-        int step = 1;
-        for (; step + 1 <= d.tl_ppcg_inner_steps; step += 2) {
-            b.ppcg_inner2(step);
+        // order, several per pass as the library plans them
+        for (int step = 1; step <= d.tl_ppcg_inner_steps;) {
+            const int n = b.ppcg_inner_plan(d.tl_ppcg_inner_steps - step + 1);
+            if (n < 1) fail("tea_engine_ppcg_inner_plan returned no steps");
+            b.ppcg_inner_n(step, n);
             b.check();
-        }
-        if (step == d.tl_ppcg_inner_steps) {
-            int be = 0, st = step;
-            b.tea_leaf_ppcg_inner_kernel_cuda_(&st, &be, chunk_neighbours);
-            b.check();
+            step += n;
         }
         if (!fused) {
             b.tea_leaf_ppcg_update_z_kernel_cuda_();

@@ -1,8 +1,8 @@
 // warmstart.hip -- the warm-start network's kernels (gfx950) and its trainer.
 //
 // Convolutions with 32 input channels run on the matrix cores as implicit
-// GEMMs with v_mfma_f32_16x16x32_bf16: M = 16 pixels of one image row,
-// N = 16 output channels, K = the 32 input channels of one tap, accumulated
+// GEMMs with v_mfma_f32_16x16x32_bf16: M = 16 output channels, N = 16
+// pixels of one image row, K = the 32 input channels of one tap, accumulated
 // over the 9 taps in FP32.  The backward pass of such a layer with respect to
 // its input is the same kernel with the taps flipped and the channel roles
 // swapped; its weight gradient is a GEMM with K = pixels (M = input channels,
@@ -29,6 +29,7 @@
 namespace ws {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 constexpr int BLOCK = 256;
@@ -56,7 +57,10 @@ __global__ void k_pack(const float *__restrict__ p, __bf16 *__restrict__ packed,
     if (e < TAPS * CH) w6[e] = bf16_round(p[OFF_W6 + e]);
     if (e >= PK_TOTAL) return;
     float v;
-    if (e < PK_HID) {
+    if (e >= PK_OUT) {   // layer 6: W6 in row 0 (lanes 0, 16, 32, 48), zeros elsewhere
+        const int r = e - PK_OUT, t = r / FRAG, lane = (r / 8) % 64, i = r % 8;
+        v = lane % 16 == 0 ? p[OFF_W6 + t * CH + (lane / 16) * 8 + i] : 0.0f;
+    } else if (e < PK_HID) {
         const int nb = e / FRAG, lane = (e / 8) % 64, i = e % 8;
         const int k = (lane / 16) * 8 + i, co = nb * 16 + lane % 16;
         v = k < K1 ? p[OFF_W1 + k * CH + co] : 0.0f;
@@ -101,69 +105,168 @@ __global__ void k_im2col(const float *__restrict__ planes, int npl, __bf16 *__re
     for (int q = 0; q < 4; q++) dst[q] = v[q];
 }
 
-// One wave: 64 pixels of row y (4 M-blocks) x 32 output channels (2 N-blocks).
-// Workgroup: 4 waves on rows 4*blockIdx.y .. +3; grid (W/64, H/4, B).
-// Forward:  out = bf16(relu(sum + bias)).
-// Backward (BWD): out = bf16(sum * (mask > 0)), mask = the layer's forward input.
-template <int NTAP, bool BWD>
-__global__ __launch_bounds__(BLOCK) void k_conv32(const __bf16 *__restrict__ in, __bf16 *__restrict__ out,
-                                                  const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
-                                                  const __bf16 *__restrict__ mask, int H, int W)
+// Convolution kernels: a workgroup of CONV_ROWS waves computes rows
+// y0 .. y0+CONV_ROWS-1, 64 pixels from x0, one row per wave (4 blocks of 16
+// pixels) and all 32 output channels (2 blocks of 16).  It first stages its
+// input window (rows y0-1 .. y0+CONV_ROWS, pixels x0-1 .. x0+64) in LDS with
+// 16-byte loads; the taps then read the window from LDS.  The MFMA's A operand
+// is the weight fragment (rows = output channels), B the activations
+// (columns = pixels), so each lane's result is 4 consecutive channels of one
+// pixel, stored as one 8-byte vector.  Grid (W/64, H/ROWS, B); ROWS = 8
+// where H allows it (a smaller share of re-read halo rows), else 4.
+constexpr int WIN_X = 64 + 2;
+constexpr int LDS_PIX = CH + 8;   // bf16 per staged pixel: an 80-byte stride spreads 16-byte reads over the banks
+inline int conv_rows(int H) { return H % 8 == 0 ? 8 : 4; }
+
+// Stages rows y0-1 .. y0+ROWS of a padded NHWC32 image into win.
+template <int ROWS>
+__device__ __forceinline__ void stage_nhwc32(const __bf16 *__restrict__ in, __bf16 *win, int b, int y0, int x0,
+                                             int H, int W)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m = lane & 15, kg = lane >> 4;
-    const int y = blockIdx.y * 4 + wave, x0 = blockIdx.x * 64, b = blockIdx.z;
-    floatx4 acc[4][2];
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) acc[mb][0] = acc[mb][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int t = 0; t < NTAP; t++) {
-        const int dy = NTAP == 1 ? 0 : t / 3 - 1, dx = NTAP == 1 ? 0 : t % 3 - 1;
-        const bf16x8 w0 = frag[(t * 2 + 0) * 64 + lane];
-        const bf16x8 w1 = frag[(t * 2 + 1) * 64 + lane];
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++) {
-            const long pix = padded(b, y + dy, x0 + mb * 16 + m + dx, H, W);
-            const bf16x8 a = *(const bf16x8 *)(in + pix * CH + kg * 8);
-            acc[mb][0] = mfma(a, w0, acc[mb][0]);
-            acc[mb][1] = mfma(a, w1, acc[mb][1]);
-        }
+    constexpr int CHUNKS = (ROWS + 2) * WIN_X * 4;   // 16-byte pieces, 4 per pixel
+    for (int c = threadIdx.x; c < CHUNKS; c += ROWS * 64) {
+        const int r = c / (WIN_X * 4), rem = c - r * (WIN_X * 4), px = rem >> 2, q = rem & 3;
+        const bf16x8 v = *(const bf16x8 *)(in + padded(b, y0 - 1 + r, x0 - 1 + px, H, W) * CH + q * 8);
+        *(bf16x8 *)(win + (r * WIN_X + px) * LDS_PIX + q * 8) = v;
     }
+    __syncthreads();
+}
+
+// acc (4 pixel blocks x 2 channel blocks) -> out at row y, pixels x0..x0+63.
+// Forward: bf16(relu(acc + bias)); backward (BWD): bf16(acc * (mask > 0)).
+template <bool BWD>
+__device__ __forceinline__ void store_nhwc32(const floatx4 (&acc)[4][2], __bf16 *__restrict__ out,
+                                             const float *__restrict__ bias, const __bf16 *__restrict__ mask, int b,
+                                             int y, int x0, int H, int W)
+{
+    const int lane = threadIdx.x & 63, m = lane & 15, kg = lane >> 4;
 #pragma unroll
     for (int mb = 0; mb < 4; mb++)
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) {
-            const int co = nb * 16 + m;
-            const float bv = BWD ? 0.0f : bias[co];
+            const int co = nb * 16 + kg * 4;
+            const long idx = padded(b, y, x0 + mb * 16 + m, H, W) * CH + co;
+            bf16x4 o;
+            bf16x4 mk;
+            if (BWD) mk = *(const bf16x4 *)(mask + idx);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const long idx = padded(b, y, x0 + mb * 16 + kg * 4 + i, H, W) * CH + co;
                 float v = acc[mb][nb][i];
-                if (BWD) v = (float)mask[idx] > 0.0f ? v : 0.0f;
-                else v = fmaxf(v + bv, 0.0f);
-                out[idx] = (__bf16)v;
+                if (BWD) v = (float)mk[i] > 0.0f ? v : 0.0f;
+                else v = fmaxf(v + bias[co + i], 0.0f);
+                o[i] = (__bf16)v;
             }
+            *(bf16x4 *)(out + idx) = o;
         }
 }
 
-// Layer 6 (32 -> 1): out[b][y][x] = b6 + sum_{tap, ci} a5 * W6.
-__global__ void k_out_fwd(const __bf16 *__restrict__ a5, const float *__restrict__ w6, const float *__restrict__ b6,
-                          float *__restrict__ out, int B, int H, int W)
+// 32 -> 32 channels, 3x3 taps.
+// Forward:  out = bf16(relu(conv(in) + bias)).
+// Backward (BWD): out = bf16(conv(in) * (mask > 0)), mask = the layer's forward input.
+template <int ROWS, bool BWD>
+__global__ __launch_bounds__(ROWS * 64) void k_conv32(const __bf16 *__restrict__ in, __bf16 *__restrict__ out,
+                                                      const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
+                                                      const __bf16 *__restrict__ mask, int H, int W)
 {
-    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (p >= (long)B * H * W) return;
-    const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((long)W * H));
-    float acc = b6[0];
+    __shared__ __attribute__((aligned(16))) __bf16 win[(ROWS + 2) * WIN_X * LDS_PIX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    const int y0 = blockIdx.y * ROWS, x0 = blockIdx.x * 64, b = blockIdx.z;
+    stage_nhwc32<ROWS>(in, win, b, y0, x0, H, W);
+    floatx4 acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) acc[mb][0] = acc[mb][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int t = 0; t < TAPS; t++) {
-        const bf16x8 *src = (const bf16x8 *)(a5 + padded(b, y + t / 3 - 1, x + t % 3 - 1, H, W) * CH);
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        const bf16x8 w0 = frag[(t * 2 + 0) * 64 + lane];
+        const bf16x8 w1 = frag[(t * 2 + 1) * 64 + lane];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const bf16x8 v = src[q];
-#pragma unroll
-            for (int i = 0; i < 8; i++) acc += (float)v[i] * w6[t * CH + q * 8 + i];
+        for (int mb = 0; mb < 4; mb++) {
+            const bf16x8 a = *(const bf16x8 *)(win + ((wave + 1 + dy) * WIN_X + 1 + mb * 16 + m + dx) * LDS_PIX + kg * 8);
+            acc[mb][0] = mfma(w0, a, acc[mb][0]);
+            acc[mb][1] = mfma(w1, a, acc[mb][1]);
         }
     }
-    out[p] = acc;
+    store_nhwc32<BWD>(acc, out, bias, mask, b, y0 + wave, x0, H, W);
+}
+
+// Layer 1 straight from the feature planes (planes[b][f][y][x], npl planes
+// per image, features f = 0..2): the im2col value k = tap*3 + f (zero for
+// k >= 27 and outside the image) of each pixel is formed from the staged
+// planes, then one MFMA K-block does the whole 3x3x3 -> 32 convolution.
+template <int ROWS>
+__global__ __launch_bounds__(ROWS * 64) void k_conv_first(const float *__restrict__ planes, int npl,
+                                                          __bf16 *__restrict__ out, const bf16x8 *__restrict__ frag,
+                                                          const float *__restrict__ bias, int H, int W)
+{
+    constexpr int WIN_Y = ROWS + 2;
+    __shared__ float feat[NFEAT][WIN_Y][WIN_X];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    const int y0 = blockIdx.y * ROWS, x0 = blockIdx.x * 64, b = blockIdx.z;
+    const float *pl = planes + (long)b * npl * H * W;
+    for (int c = threadIdx.x; c < NFEAT * WIN_Y * WIN_X; c += ROWS * 64) {
+        const int f = c / (WIN_Y * WIN_X), rem = c - f * (WIN_Y * WIN_X), r = rem / WIN_X, px = rem - r * WIN_X;
+        const int y = y0 - 1 + r, x = x0 - 1 + px;
+        feat[f][r][px] = (y >= 0 && y < H && x >= 0 && x < W) ? pl[(long)f * H * W + (long)y * W + x] : 0.0f;
+    }
+    __syncthreads();
+    // this lane's 8 K-values: offsets into feat relative to the pixel
+    int off[8];
+    bool real[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = kg * 8 + i, t = k / 3, f = k - 3 * t;
+        real[i] = k < K1;
+        off[i] = real[i] ? (f * WIN_Y + (t / 3 - 1)) * WIN_X + (t % 3 - 1) : 0;
+    }
+    const float *base = &feat[0][wave + 1][1];
+    const bf16x8 w0 = frag[lane], w1 = frag[64 + lane];
+    floatx4 acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+        bf16x8 a;
+#pragma unroll
+        for (int i = 0; i < 8; i++) a[i] = (__bf16)(real[i] ? base[mb * 16 + m + off[i]] : 0.0f);
+        const floatx4 z = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        acc[mb][0] = mfma(w0, a, z);
+        acc[mb][1] = mfma(w1, a, z);
+    }
+    store_nhwc32<false>(acc, out, bias, (const __bf16 *)nullptr, b, y0 + wave, x0, H, W);
+}
+
+// Layer 6 (32 -> 1) on the matrix cores: the A fragment (packed at PK_OUT)
+// holds W6 in its first row (output channel 0) and zeros elsewhere;
+// out[b][y][x] = b6 + acc.
+template <int ROWS>
+__global__ __launch_bounds__(ROWS * 64) void k_conv_out(const __bf16 *__restrict__ a5, const bf16x8 *__restrict__ frag,
+                                                        const float *__restrict__ b6, float *__restrict__ out, int H,
+                                                        int W)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 win[(ROWS + 2) * WIN_X * LDS_PIX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    const int y0 = blockIdx.y * ROWS, x0 = blockIdx.x * 64, b = blockIdx.z;
+    stage_nhwc32<ROWS>(a5, win, b, y0, x0, H, W);
+    floatx4 acc[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) acc[mb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < TAPS; t++) {
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        const bf16x8 w = frag[t * 64 + lane];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            const bf16x8 a = *(const bf16x8 *)(win + ((wave + 1 + dy) * WIN_X + 1 + mb * 16 + m + dx) * LDS_PIX + kg * 8);
+            acc[mb] = mfma(w, a, acc[mb]);
+        }
+    }
+    if (kg == 0) {
+        const float bv = b6[0];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) out[((long)b * H + y0 + wave) * W + x0 + mb * 16 + m] = acc[mb][0] + bv;
+    }
 }
 
 // ---------------------------------------------------------------- loss and backward
@@ -459,8 +562,10 @@ hipError_t acts_alloc(Acts &a, int B, int H, int W, bool training)
     const size_t bytes = act_bytes(B, H, W);
     const int distinct = training ? 5 : 2;
     hipError_t e;
-    if ((e = hipMalloc(&a.x1, bytes)) != hipSuccess) return e;
-    if ((e = hipMemset(a.x1, 0, bytes)) != hipSuccess) return e;
+    if (training) {   // the im2col image is only needed for layer 1's weight gradient
+        if ((e = hipMalloc(&a.x1, bytes)) != hipSuccess) return e;
+        if ((e = hipMemset(a.x1, 0, bytes)) != hipSuccess) return e;
+    }
     for (int l = 1; l <= distinct; l++) {
         if ((e = hipMalloc(&a.a[l], bytes)) != hipSuccess) return e;
         if ((e = hipMemset(a.a[l], 0, bytes)) != hipSuccess) return e;
@@ -487,19 +592,23 @@ void im2col(const Acts &a, const float *planes, int npl, hipStream_t stream)
                        a.B, a.H, a.W);
 }
 
-void im2col(const Acts &a, const float *planes, hipStream_t stream) { im2col(a, planes, NFEAT, stream); }
-
-void forward(const Net &net, const Acts &a, hipStream_t stream)
+template <int ROWS> void forward_rows(const Net &net, const Acts &a, const float *planes, int npl, hipStream_t stream)
 {
-    const dim3 grid(a.W / 64, a.H / 4, a.B);
+    const dim3 grid(a.W / 64, a.H / ROWS, a.B), block(ROWS * 64);
     const bf16x8 *pk = (const bf16x8 *)net.packed;
-    hipLaunchKernelGGL((k_conv32<1, false>), grid, dim3(BLOCK), 0, stream, a.x1, a.a[1], pk + PK_L1 / 8,
-                       net.params + OFF_B1, (const __bf16 *)nullptr, a.H, a.W);
+    hipLaunchKernelGGL(k_conv_first<ROWS>, grid, block, 0, stream, planes, npl, a.a[1], pk + PK_L1 / 8,
+                       net.params + OFF_B1, a.H, a.W);
     for (int l = 2; l <= 5; l++)
-        hipLaunchKernelGGL((k_conv32<TAPS, false>), grid, dim3(BLOCK), 0, stream, a.a[l - 1], a.a[l],
-                           pk + pk_fwd(l) / 8, net.params + off_b(l), (const __bf16 *)nullptr, a.H, a.W);
-    hipLaunchKernelGGL(k_out_fwd, dim3(nblocks((long)a.B * a.H * a.W)), dim3(BLOCK), 0, stream, a.a[5], net.w6,
-                       net.params + OFF_B6, a.out, a.B, a.H, a.W);
+        hipLaunchKernelGGL((k_conv32<ROWS, false>), grid, block, 0, stream, a.a[l - 1], a.a[l], pk + pk_fwd(l) / 8,
+                           net.params + off_b(l), (const __bf16 *)nullptr, a.H, a.W);
+    hipLaunchKernelGGL(k_conv_out<ROWS>, grid, block, 0, stream, a.a[5], pk + PK_OUT / 8, net.params + OFF_B6, a.out,
+                       a.H, a.W);
+}
+
+void forward(const Net &net, const Acts &a, const float *planes, int npl, hipStream_t stream)
+{
+    if (conv_rows(a.H) == 8) forward_rows<8>(net, a, planes, npl, stream);
+    else forward_rows<4>(net, a, planes, npl, stream);
 }
 
 }  // namespace ws
@@ -558,7 +667,7 @@ void destroy(Trainer *t)
 int forward_pass(Trainer *t, const float *samples)
 {
     ws::im2col(t->acts, samples, ws::NPLANES, t->stream);
-    ws::forward(t->net, t->acts, t->stream);
+    ws::forward(t->net, t->acts, samples, ws::NPLANES, t->stream);
     WS_OK(hipGetLastError());
     return 0;
 }
@@ -582,7 +691,8 @@ int backward_pass(Trainer *t, const float *samples)
     hipLaunchKernelGGL(k_out_bwd, dim3(nblocks(npix)), dim3(BLOCK), 0, s, t->g6, t->net.w6, t->acts.a[5], t->g[0],
                        B, H, W);
     // layers 5..2: weight and bias gradients from g(l), then g(l-1)
-    const dim3 grid(W / 64, H / 4, B);
+    const int rows = conv_rows(H);
+    const dim3 grid(W / 64, H / rows, B), block(rows * 64);
     const bf16x8 *pk = (const bf16x8 *)t->net.packed;
     int cur = 0;
     for (int l = 5; l >= 2; l--) {
@@ -593,8 +703,12 @@ int backward_pass(Trainer *t, const float *samples)
         hipLaunchKernelGGL(k_chansum, dim3(WG_CHUNKS), dim3(BLOCK), 0, s, t->g[cur], t->part, B, H, W, WG_CHUNKS);
         hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(BLOCK), 0, s, t->part, (long)CH, WG_CHUNKS, CH,
                            t->grad + off_b(l));
-        hipLaunchKernelGGL((k_conv32<TAPS, true>), grid, dim3(BLOCK), 0, s, t->g[cur], t->g[1 - cur],
-                           pk + pk_bwd(l) / 8, (const float *)nullptr, t->acts.a[l - 1], H, W);
+        if (rows == 8)
+            hipLaunchKernelGGL((k_conv32<8, true>), grid, block, 0, s, t->g[cur], t->g[1 - cur], pk + pk_bwd(l) / 8,
+                               (const float *)nullptr, t->acts.a[l - 1], H, W);
+        else
+            hipLaunchKernelGGL((k_conv32<4, true>), grid, block, 0, s, t->g[cur], t->g[1 - cur], pk + pk_bwd(l) / 8,
+                               (const float *)nullptr, t->acts.a[l - 1], H, W);
         cur = 1 - cur;
     }
     // layer 1 (im2col input, no input gradient)

@@ -42,16 +42,21 @@ constexpr int OFF_W6 = off_w(2) + HIDDEN * HID_SIZE;
 constexpr int OFF_B6 = OFF_W6 + TAPS * CH;
 constexpr int NPARAM = OFF_B6 + 1;               // 38177
 
-// Packed BF16 MFMA B-operand fragments (8 values per lane, 64 lanes):
+// Packed BF16 MFMA weight fragments (8 values per lane, 64 lanes; lane l
+// holds output channel l % 16 of its block and K-values 8*(l/16) .. +7, the
+// same contents as an A operand over output channels or a B operand over
+// them):
 //   layer 1:            [2 column blocks][64][8]
 //   layer l forward:    [9 taps][2][64][8]
 //   layer l backward:   [9 taps][2][64][8]  (taps flipped, ci/co swapped)
+//   layer 6:            [9 taps][64][8]     (W6 as output channel 0 of 16)
 constexpr int FRAG = 64 * 8;
 constexpr int PK_L1 = 0;
 constexpr int PK_HID = 2 * FRAG;
 constexpr int pk_fwd(int l) { return PK_HID + (l - 2) * TAPS * 2 * FRAG; }
 constexpr int pk_bwd(int l) { return PK_HID + (HIDDEN + l - 2) * TAPS * 2 * FRAG; }
-constexpr int PK_TOTAL = PK_HID + 2 * HIDDEN * TAPS * 2 * FRAG;  // 74752
+constexpr int PK_OUT = PK_HID + 2 * HIDDEN * TAPS * 2 * FRAG;
+constexpr int PK_TOTAL = PK_OUT + TAPS * FRAG;  // 79360
 
 // Device copy of one parameter set ready for inference and training.
 struct Net {
@@ -63,7 +68,7 @@ struct Net {
 // Buffers of one batch shape: B images of H x W interior pixels.
 struct Acts {
     int B = 0, H = 0, W = 0;
-    __bf16 *x1 = nullptr;        // im2col input, padded NHWC32
+    __bf16 *x1 = nullptr;        // im2col input, padded NHWC32 (training only)
     __bf16 *a[6] = {};           // a[1..5]: layer outputs, padded NHWC32
     float *out = nullptr;        // B*H*W network output
 };
@@ -82,10 +87,11 @@ void net_pack(const Net &net, hipStream_t stream);
 hipError_t acts_alloc(Acts &a, int B, int H, int W, bool training);
 void acts_free(Acts &a);
 
-// Input features: planes[b][f][y][x] (FP32, f = 0..2) -> im2col image x1.
-void im2col(const Acts &a, const float *planes, hipStream_t stream);
-// Layers 1..6 from x1; writes a.a[1..5] and a.out.
-void forward(const Net &net, const Acts &a, hipStream_t stream);
+// Training: the im2col image x1 of the feature planes (planes[b][f][y][x],
+// npl FP32 planes per image, features f = 0..2), for layer 1's weight gradient.
+void im2col(const Acts &a, const float *planes, int npl, hipStream_t stream);
+// Layers 1..6 from the feature planes; writes a.a[1..5] and a.out.
+void forward(const Net &net, const Acts &a, const float *planes, int npl, hipStream_t stream);
 
 }  // namespace ws
 

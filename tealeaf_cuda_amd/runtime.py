@@ -107,6 +107,8 @@ _EXT = {
     "tea_engine_ppcg_ur": ([ctypes.c_double, ctypes.c_double, _D], None),
     "tea_engine_ppcg_end": ([_D], None),
     "tea_engine_ppcg_inner2": ([ctypes.c_int], None),
+    "tea_engine_ppcg_inner_n": ([ctypes.c_int, ctypes.c_int], None),
+    "tea_engine_ppcg_inner_plan": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
     "tea_engine_mp_begin": ([_D], None),
@@ -337,6 +339,14 @@ class Engine:
 
     def ppcg_inner2(self, step: int):
         self._call("tea_engine_ppcg_inner2", int(step))
+
+    def ppcg_inner_n(self, step: int, nsteps: int):
+        self._call("tea_engine_ppcg_inner_n", int(step), int(nsteps))
+
+    def ppcg_inner_plan(self, remaining: int) -> int:
+        n = self.lib.tea_engine_ppcg_inner_plan(int(remaining))
+        self._check()
+        return int(n)
 
     def ppcg_end(self) -> float:
         out = ctypes.c_double()

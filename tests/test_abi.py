@@ -62,3 +62,28 @@ def test_engine_exports_the_warm_start_interface():
         pytest.fail("libtea_engine.so has not been built (run __graft_entry__.build())")
     so = ctypes.CDLL(str(ENGINE_SO))
     assert not [n for n in names if not hasattr(so, n)]
+
+
+def test_inner_pass_plan_covers_the_steps():
+    """tea_engine_ppcg_inner_plan (no GPU state involved): every plan takes
+    1..5 steps per pass, covers the run exactly, and 10 FP64 steps go as
+    4, 3, 3 (the cheapest grouping by the measured pass times)."""
+    if not ENGINE_SO.exists():
+        pytest.fail("libtea_engine.so has not been built (run __graft_entry__.build())")
+    so = ctypes.CDLL(str(ENGINE_SO))
+    so.tea_engine_ppcg_inner_plan.argtypes = [ctypes.c_int]
+    so.tea_engine_ppcg_inner_plan.restype = ctypes.c_int
+
+    def plan(total):
+        out, left = [], total
+        while left > 0:
+            n = so.tea_engine_ppcg_inner_plan(left)
+            assert 1 <= n <= min(5, left)
+            out.append(n)
+            left -= n
+        return out
+
+    for total in range(1, 40):
+        assert sum(plan(total)) == total
+    assert plan(10) == [4, 3, 3]
+    assert so.tea_engine_ppcg_inner_plan(0) == 0

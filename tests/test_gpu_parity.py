@@ -92,11 +92,11 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
         assert a.values() == b.values(), (a, b)
 
 
-@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER2"])
+@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
     """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0), and the fused
-    outer iteration with one inner step per pass (TEA_INNER2=0), on the GPU
+    outer iteration with one inner step per pass (TEA_INNER_PASSES=0), on the GPU
     give the same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
@@ -169,6 +169,9 @@ PPCG_CALLS = {
     "fused_ppcg_ur": lambda e: e.ppcg_ur(0.37, 1.7),
     "fused_ppcg_end": lambda e: e.ppcg_end(),
     "fused_ppcg_inner2": lambda e: (e.ppcg_init_constants([0.3, 0.5, 0.7], [1.1, 0.9, 0.8]), e.ppcg_inner2(2)),
+    **{f"fused_ppcg_inner_n{n}": (lambda n: lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_inner_n(7 - n, n)))(n) for n in range(1, 6)},
 }
 CHEBY_CALLS = {
     "cheby_init": lambda e: e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3),

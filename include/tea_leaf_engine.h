@@ -196,8 +196,11 @@ int tea_engine_set_field(int field_id, const double *host);
 /* Kernel profiler (the reference's profiler_on, cuda_common.hpp:97-122):
  * enable/disable per-launch HIP event timing, reset counters, and read
  * back count and total milliseconds for one kernel name.  Returns 0 if the
- * name was seen.  tea_engine_profile_names writes newline-separated names. */
+ * name was seen.  tea_engine_profile_names writes newline-separated names.
+ * tea_engine_profile_only(prefix) times only launches whose name starts with
+ * prefix (NULL or "": all); a timed launch costs the run a few microseconds. */
 void tea_engine_profile_enable(int on);
+void tea_engine_profile_only(const char *prefix);
 void tea_engine_profile_reset(void);
 int tea_engine_profile_get(const char *kernel, long *count, double *total_ms);
 int tea_engine_profile_names(char *buf, int buflen);
@@ -222,10 +225,14 @@ void tea_engine_finalize(void);
  *   tea_engine_ppcg_begin: [ppcg_calc_p(beta) if with_p], cg_calc_w (pw_rro[0]),
  *                          ppcg_store_r, ppcg_calc_2norm (pw_rro[1])
  *   tea_engine_ppcg_ur:    cg_calc_ur(alpha) (rrn), ppcg_init_sd_new(theta)
- *   tea_engine_ppcg_end:   ppcg_update_z, ppcg_calc_rrn (rrn) */
+ *   tea_engine_ppcg_end:   ppcg_update_z, ppcg_calc_rrn (rrn)
+ *   tea_engine_ppcg_begin_ur: tea_engine_ppcg_begin, then tea_engine_ppcg_ur
+ *                          with alpha = pw_rro[1] / pw_rro[0] formed on the
+ *                          device (one host wait instead of two) */
 void tea_engine_ppcg_begin(int with_p, double beta, double *pw_rro);
 void tea_engine_ppcg_ur(double alpha, double theta, double *rrn);
 void tea_engine_ppcg_end(double *rrn);
+void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, double *pw_rro, double *rrn);
 /* Inner steps `step` .. `step + nsteps - 1` (1-based; nsteps 1..5) in one
  * pass over HBM (temporal blocking): the same results as
  * tea_leaf_ppcg_inner_kernel_cuda_ called for each of the steps with

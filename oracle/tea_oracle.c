@@ -834,6 +834,7 @@ EXPORT int tea_engine_set_field(int id, const double *host)
     return 0;
 }
 EXPORT void tea_engine_profile_enable(int on) { (void)on; }
+EXPORT void tea_engine_profile_only(const char *prefix) { (void)prefix; }
 EXPORT void tea_engine_profile_reset(void) {}
 EXPORT int tea_engine_profile_get(const char *k, long *count, double *ms) { (void)k; *count = 0; *ms = 0; return -1; }
 EXPORT int tea_engine_profile_names(char *buf, int n) { if (buf && n > 0) buf[0] = 0; return 0; }
@@ -856,6 +857,12 @@ EXPORT void tea_engine_ppcg_ur(double alpha, double theta, double *rrn)
 {
     tea_leaf_cg_calc_ur_kernel_cuda_(&alpha, rrn);
     tea_leaf_ppcg_init_sd_new_kernel_cuda_(&theta);
+}
+
+EXPORT void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, double *pw_rro, double *rrn)
+{
+    tea_engine_ppcg_begin(with_p, beta, pw_rro);
+    tea_engine_ppcg_ur(pw_rro[1] / pw_rro[0], theta, rrn);
 }
 
 EXPORT void tea_engine_ppcg_end(double *rrn)

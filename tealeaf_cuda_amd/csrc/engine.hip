@@ -13,6 +13,7 @@
 // doubles, so every wavefront row access is a set of whole cache lines.  All
 // fields live in one allocation; kernels index them with one flat index
 // i = c + r*pitch on pointers pre-shifted by (lead - h).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -54,6 +55,7 @@ constexpr int S_CG_FAMILY = (1 << SOLVER_CG) | (1 << SOLVER_CHEBY) | (1 << SOLVE
 
 struct Profiler {
     bool on = false;
+    std::string only;   // name prefix of the launches to time (empty: all)
     struct Pending {
         int name;
         hipEvent_t a, b;
@@ -76,8 +78,11 @@ struct Profiler {
         ms.push_back(0.0);
         return k;
     }
+    bool wants(const char *n) const { return on && (only.empty() || std::strncmp(n, only.c_str(), only.size()) == 0); }
+    void drain(bool wait);
     hipEvent_t get()
     {
+        if (pool.empty() && pending.size() > (1u << 16)) drain(false);   // bound a long profiled run
         if (!pool.empty()) {
             hipEvent_t e = pool.back();
             pool.pop_back();
@@ -87,24 +92,37 @@ struct Profiler {
         (void)hipEventCreate(&e);
         return e;
     }
-    // Folds every completed pair into the totals (called after synchronisation).
-    void drain(bool wait)
+    // Events made ahead, so that a profiled launch costs the host no event
+    // creation.
+    void reserve(size_t n)
     {
-        size_t k = 0;
-        for (; k < pending.size(); k++) {
-            Pending &p = pending[k];
-            if (!wait && hipEventQuery(p.b) != hipSuccess) break;
-            if (wait) (void)hipEventSynchronize(p.b);
-            float t = 0.0f;
-            (void)hipEventElapsedTime(&t, p.a, p.b);
-            counts[p.name]++;
-            ms[p.name] += t;
-            pool.push_back(p.a);
-            pool.push_back(p.b);
+        while (pool.size() < n) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) break;
+            pool.push_back(e);
         }
-        pending.erase(pending.begin(), pending.begin() + (long)k);
     }
 };
+
+// Folds every completed pair into the totals.  Reading events costs the host
+// microseconds per pair, so this runs when the profile is read or switched
+// off, not at the solver's synchronisation points.
+void Profiler::drain(bool wait)
+{
+    size_t k = 0;
+    for (; k < pending.size(); k++) {
+        Pending &p = pending[k];
+        if (!wait && hipEventQuery(p.b) != hipSuccess) break;
+        if (wait) (void)hipEventSynchronize(p.b);
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, p.a, p.b);
+        counts[p.name]++;
+        ms[p.name] += t;
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.erase(pending.begin(), pending.begin() + (long)k);
+}
 
 struct Chunk {
     bool initialised = false;
@@ -126,6 +144,7 @@ struct Chunk {
     DD *d_partials = nullptr;        // MAX_RED x max_blocks workgroup partials
     double *h_out = nullptr;         // pinned: the finished values
     double *d_out = nullptr;         // device alias of h_out
+    double *d_scal = nullptr;        // finished values kept in device memory for a following launch
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -278,7 +297,7 @@ struct Timed {
     hipEvent_t b;
     explicit Timed(const char *n)
     {
-        if (!g.prof.on) return;
+        if (!g.prof.wants(n)) return;
         name = g.prof.id(n);
         hipEvent_t a = g.prof.get();
         b = g.prof.get();
@@ -291,16 +310,30 @@ struct Timed {
     }
 };
 
+// Launches a kernel on the engine's stream.  With the profiler on, the launch
+// carries its own start and stop events (hipExtLaunchKernelGGL: the
+// timestamps of the dispatch itself), so no marker packets are queued between
+// kernels and the profiled run keeps the unprofiled pace.
+template <typename K, typename... A>
+void klaunch(const char *name, K kernel, dim3 grid, dim3 block, A... args)
+{
+    if (!g.prof.wants(name)) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, g.stream, args...);
+        return;
+    }
+    const int id = g.prof.id(name);
+    hipEvent_t a = g.prof.get(), b = g.prof.get();
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, g.stream, a, b, 0u, args...);
+    g.prof.pending.push_back({id, a, b});
+}
+
 template <typename K, typename... A>
 bool launch(const char *name, K kernel, const Ext &e, A... args)
 {
     const int nb = grid_for(e);
     g.last_grid = nb;
     if (nb == 0) return true;
-    {
-        Timed t(name);
-        hipLaunchKernelGGL(kernel, dim3(nb), dim3(BLOCK), 0, g.stream, e, args...);
-    }
+    klaunch(name, kernel, dim3(nb), dim3(BLOCK), e, args...);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
     return true;
@@ -316,32 +349,35 @@ bool launch_x2(const char *name, K kernel, const Ext &e, A... args)
     nb = nb < grid_max() ? nb : grid_max();
     g.last_grid = nb;
     if (nb == 0) return true;
-    {
-        Timed t(name);
-        hipLaunchKernelGGL(kernel, dim3(nb), dim3(BLOCK), 0, g.stream, e, args...);
-    }
+    klaunch(name, kernel, dim3(nb), dim3(BLOCK), e, args...);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
     return true;
 }
 
-// Combines the partials of the last reduction launch on the device, waits,
-// and returns the n finished values.
-bool fetch(double *dst, int n)
+// Combines the partials of the last reduction launch on the device into
+// h_out[slot .. slot+n-1] (and dev_out[0 .. n-1] if given), without waiting.
+bool finish(int slot, int n, double *dev_out)
 {
-    {
-        Timed t("finish");
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, g.stream, (const DD *)g.d_partials,
-                           g.max_blocks, g.last_grid, n, g.d_out);
-    }
+    klaunch("finish", k_finish, dim3(1), dim3(FINISH_THREADS), (const DD *)g.d_partials, g.max_blocks, g.last_grid,
+            n, g.d_out + slot, dev_out);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return die("launch of k_finish failed: %s", hipGetErrorString(err));
-    err = hipStreamSynchronize(g.stream);
-    if (err != hipSuccess) return die("hipStreamSynchronize failed: %s", hipGetErrorString(err));
-    for (int j = 0; j < n; j++) dst[j] = g.h_out[j];
-    if (g.prof.on) g.prof.drain(false);
     return true;
 }
+
+// Waits for the stream and copies h_out[slot .. slot+n-1] to dst.
+bool collect(double *dst, int slot, int n)
+{
+    hipError_t err = hipStreamSynchronize(g.stream);
+    if (err != hipSuccess) return die("hipStreamSynchronize failed: %s", hipGetErrorString(err));
+    for (int j = 0; j < n; j++) dst[j] = g.h_out[slot + j];
+    return true;
+}
+
+// Combines the partials of the last reduction launch on the device, waits,
+// and returns the n finished values.
+bool fetch(double *dst, int n) { return finish(0, n, nullptr) && collect(dst, 0, n); }
 
 }  // namespace
 
@@ -817,6 +853,7 @@ void release_all()
     if (g.cellx) (void)hipFree(g.cellx);
     if (g.d_partials) (void)hipFree(g.d_partials);
     if (g.h_out) (void)hipHostFree(g.h_out);
+    if (g.d_scal) (void)hipFree(g.d_scal);
     if (g.d_halo_buf) (void)hipFree(g.d_halo_buf);
     if (g.slab32) (void)hipFree(g.slab32);
     ws::net_free(g.ws_net);
@@ -896,6 +933,7 @@ EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
     HIP_OK(hipMalloc(&g.d_partials, (size_t)MAX_RED * g.max_blocks * sizeof(DD)));
     HIP_OK(hipHostMalloc(&g.h_out, MAX_RED * sizeof(double), hipHostMallocDefault));
     HIP_OK(hipHostGetDevicePointer((void **)&g.d_out, g.h_out, 0));
+    HIP_OK(hipMalloc(&g.d_scal, MAX_RED * sizeof(double)));
 
     g.halo_buf_elems = (size_t)TEA_NUM_FIELDS * (size_t)g.h * (size_t)((g.nx > g.ny ? g.nx : g.ny) + 2 * g.h);
     HIP_OK(hipMalloc(&g.d_halo_buf, g.halo_buf_elems * sizeof(double)));
@@ -1264,11 +1302,10 @@ template <typename T, int NS> void launch_inner_n(const Ext &e, long pt, const S
                                                 "k_ppcg_inner5"};
     const long ntiles = (long)((e.x1 - e.x0 + Tile::OUT_X) / Tile::OUT_X) * ((e.y1 - e.y0 + Tile::OUT_Y) / Tile::OUT_Y);
     const unsigned grid = (unsigned)std::min<long>(ntiles, grid_max());
-    Timed t(nm<T>(names[NS - 1]));
     auto go = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(INNER_WAVES * WAVE), 0, g.stream, e, pt, (const T *)FT<T>(F_SD),
-                           FT<T>(X_SD2), (const T *)FT<T>(X_RTEMP), FT<T>(X_RT2), FT<T>(X_UTEMP),
-                           (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
+        klaunch(nm<T>(names[NS - 1]), kernel, dim3(grid), dim3(INNER_WAVES * WAVE), e, pt, (const T *)FT<T>(F_SD),
+                FT<T>(X_SD2), (const T *)FT<T>(X_RTEMP), FT<T>(X_RT2), FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
+                (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
     };
     if constexpr (X2) {
         if (g.prec == PREC_BLOCK) go(k_ppcg_inner_n_x2<PREC_BLOCK, INNER_WAVES, NS>);
@@ -1360,7 +1397,7 @@ template <typename T> void ppcg_calc_p_impl(double beta)
 }
 
 // Fused outer iteration (see k_ppcg_begin / k_ppcg_ur_sd / k_ppcg_end).
-template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
+template <typename T> bool ppcg_begin_launch(int with_p, double beta)
 {
     const long pt = pitch_of<T>();
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
@@ -1380,22 +1417,28 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
         ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
                     FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
                     (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
-    if (!ok) return;
+    if (!ok) return false;
     if (std::is_same<T, double>::value) {
         std::swap(g.f[F_P], g.f[X_P2]);
         std::swap(g.fs[F_P], g.fs[X_P2]);
     } else {
         std::swap(g.fs32[F_P], g.fs32[X_P2]);
     }
-    fetch(out, 2);
+    return true;
 }
 
-template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
+template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
+{
+    if (ppcg_begin_launch<T>(with_p, beta)) fetch(out, 2);
+}
+
+// pw_rro: null (alpha given) or device memory holding p.w, r.z.
+template <typename T> bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta)
 {
     bool ok;
     if constexpr (std::is_same<T, float>::value) {
         auto go = [&](auto kernel) {
-            return launch_x2("k_ppcg_ur_sd_f32", kernel, interior(), pitch_of<T>(), alpha, theta, FT<T>(F_U),
+            return launch_x2("k_ppcg_ur_sd_f32", kernel, interior(), pitch_of<T>(), alpha, pw_rro, theta, FT<T>(F_U),
                              (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
                              (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD),
                              FT<T>(X_RTEMP), FT<T>(X_UTEMP), red_ctx());
@@ -1404,13 +1447,25 @@ template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
              : g.prec == PREC_DIAG ? go(k_ppcg_ur_sd_x2<PREC_DIAG>)
                                    : go(k_ppcg_ur_sd_x2<PREC_NONE>);
     } else {
-        ok = PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, theta,
+        ok = PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, pw_rro, theta,
                            FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
                            (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD),
                            FT<T>(X_RTEMP), FT<T>(X_UTEMP), red_ctx());
     }
-    if (!ok) return;
-    fetch(rrn, 1);
+    return ok;
+}
+
+template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
+{
+    if (ppcg_ur_launch<T>(alpha, nullptr, theta)) fetch(rrn, 1);
+}
+
+// begin, then ur with alpha = rro / pw formed on the device: one wait.
+template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double theta, double *pw_rro, double *rrn)
+{
+    if (!ppcg_begin_launch<T>(with_p, beta) || !finish(0, 2, g.d_scal)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta) || !finish(2, 1, nullptr)) return;
+    if (collect(pw_rro, 0, 2)) *rrn = g.h_out[2];
 }
 
 template <typename T> void ppcg_end_impl(double *rrn)
@@ -1621,6 +1676,13 @@ EXPORT void tea_engine_ppcg_ur(double alpha, double theta, double *rrn)
     BY_PRECISION(ppcg_ur_impl, alpha, theta, rrn);
 }
 
+EXPORT void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, double *pw_rro, double *rrn)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    BY_PRECISION(ppcg_begin_ur_impl, with_p, beta, theta, pw_rro, rrn);
+}
+
 EXPORT void tea_engine_ppcg_end(double *rrn)
 {
     CHECK_INIT();
@@ -1703,8 +1765,12 @@ EXPORT void tea_engine_profile_enable(int on)
         (void)hipStreamSynchronize(g.stream);
         g.prof.drain(true);
     }
+    if (on && !g.prof.on) g.prof.reserve(1 << 15);
     g.prof.on = on != 0;
 }
+// Times only the launches whose name starts with `prefix` (NULL or "": all);
+// each timed launch costs the run a few microseconds.
+EXPORT void tea_engine_profile_only(const char *prefix) { g.prof.only = prefix ? prefix : ""; }
 EXPORT void tea_engine_profile_reset(void)
 {
     if (g.stream) (void)hipStreamSynchronize(g.stream);

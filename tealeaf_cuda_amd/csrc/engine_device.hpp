@@ -112,7 +112,8 @@ __device__ __forceinline__ void block_reduce(DD (&v)[NV], const Red red)
 constexpr int FINISH_THREADS = 1024;
 
 __global__ __launch_bounds__(FINISH_THREADS) void k_finish(const DD *__restrict__ partials, int max_blocks,
-                                                          int nblocks, int nv, double *__restrict__ out)
+                                                          int nblocks, int nv, double *__restrict__ out,
+                                                          double *__restrict__ dev_out)
 {
     __shared__ DD lds[FINISH_THREADS / WAVE];
     const int lane = threadIdx.x & (WAVE - 1);
@@ -128,6 +129,7 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish(const DD *__restrict_
             DD t = lds[0];
             for (int w = 1; w < FINISH_THREADS / WAVE; w++) dd_merge(t, lds[w]);
             out[j] = t.s + t.c;
+            if (dev_out) dev_out[j] = t.s + t.c;   // for a following launch on the device
         }
         __syncthreads();
     }

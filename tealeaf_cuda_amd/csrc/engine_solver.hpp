@@ -906,7 +906,8 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin_x2(Ext e, long pitch, int 
 
 // k_ppcg_ur_sd for FP32 fields with two columns per lane.
 template <int PREC>
-__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2(Ext e, long pitch, double alpha_in, double theta_in,
+__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2(Ext e, long pitch, double alpha_in,
+                                                         const double *__restrict__ pw_rro, double theta_in,
                                                          float *__restrict__ u, const float *__restrict__ p,
                                                          float *__restrict__ r, const float *__restrict__ w,
                                                          float *__restrict__ z, const float *__restrict__ Mi,
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2(Ext e, long pitch, doub
                                                          float *__restrict__ sd, float *__restrict__ rtemp,
                                                          float *__restrict__ utemp, Red red)
 {
-    const float alpha = (float)alpha_in;
+    const float alpha = (float)(pw_rro ? pw_rro[1] / pw_rro[0] : alpha_in);
     const float theta_r = 1.0f / (float)theta_in;
     DD acc[1] = {{0, 0}};
     for (TileLoopT<2 * TILE_X> tl_(e); tl_.more(); tl_.advance()) {
@@ -1007,9 +1008,12 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2(Ext e, long pitch, doub
 }
 
 // cg_calc_ur (u += a p; r -= a w; z = M^-1 r; r.z or r.r) then
-// ppcg_init_sd_new (sd = (z or r) / theta; rtemp = r; utemp = sd).
+// ppcg_init_sd_new (sd = (z or r) / theta; rtemp = r; utemp = sd).  With
+// pw_rro (device memory: p.w and r.z of k_ppcg_begin) a = rro / pw is formed
+// here, as the host would form it, instead of taken from alpha_in.
 template <typename T, int PREC>
-__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd(Ext e, long pitch, double alpha_in, double theta_in,
+__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd(Ext e, long pitch, double alpha_in,
+                                                      const double *__restrict__ pw_rro, double theta_in,
                                                       T *__restrict__ u, const T *__restrict__ p,
                                                       T *__restrict__ r, const T *__restrict__ w,
                                                       T *__restrict__ z, const T *__restrict__ Mi,
@@ -1017,7 +1021,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd(Ext e, long pitch, double 
                                                       T *__restrict__ sd, T *__restrict__ rtemp,
                                                       T *__restrict__ utemp, Red red)
 {
-    const T alpha = (T)alpha_in;
+    const T alpha = (T)(pw_rro ? pw_rro[1] / pw_rro[0] : alpha_in);
     const T theta_r = T(1.0) / (T)theta_in;
     DD acc[1] = {{0, 0}};
     STRIP_BEGIN

@@ -92,6 +92,7 @@ struct Backend {
     void (*ppcg_ur)(double, double, double *) = nullptr;
     void (*ppcg_end)(double *) = nullptr;
     void (*ppcg_inner_n)(int, int) = nullptr;
+    void (*ppcg_begin_ur)(int, double, double, double *, double *) = nullptr;
     int (*ppcg_inner_plan)(int) = nullptr;
 
     void check() const
@@ -129,6 +130,8 @@ Backend load_backend(const char *path)
     b.ppcg_ur = reinterpret_cast<void (*)(double, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_ur"));
     b.ppcg_end = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_ppcg_end"));
     b.ppcg_inner_n = reinterpret_cast<void (*)(int, int)>(dlsym(b.handle, "tea_engine_ppcg_inner_n"));
+    b.ppcg_begin_ur =
+        reinterpret_cast<void (*)(int, double, double, double *, double *)>(dlsym(b.handle, "tea_engine_ppcg_begin_ur"));
     b.ppcg_inner_plan = reinterpret_cast<int (*)(int)>(dlsym(b.handle, "tea_engine_ppcg_inner_plan"));
     return b;
 }
@@ -990,14 +993,22 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                 // the same calls as below, fused; ppcg_calc_p of the previous
                 // iteration is folded into this iteration's first pass
                 double two[2];
-                b.ppcg_begin(p_pending ? 1 : 0, beta_pending, two);
-                b.check();
-                p_pending = false;
-                pw = two[0];
-                rro = two[1];
-                alpha = rro / pw;
-                b.ppcg_ur(alpha, theta, &rrn);
-                b.check();
+                if (b.ppcg_begin_ur) {   // alpha = rro / pw formed on the device
+                    b.ppcg_begin_ur(p_pending ? 1 : 0, beta_pending, theta, two, &rrn);
+                    b.check();
+                    p_pending = false;
+                    pw = two[0];
+                    rro = two[1];
+                } else {
+                    b.ppcg_begin(p_pending ? 1 : 0, beta_pending, two);
+                    b.check();
+                    p_pending = false;
+                    pw = two[0];
+                    rro = two[1];
+                    alpha = rro / pw;
+                    b.ppcg_ur(alpha, theta, &rrn);
+                    b.check();
+                }
                 bool replaced = false;
                 if (rel.on && reliable_step(rrn, true, &replaced)) {
                     converged = true;

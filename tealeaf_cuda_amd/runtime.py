@@ -97,6 +97,7 @@ _EXT = {
     "tea_engine_get_field": ([ctypes.c_int, _D], ctypes.c_int),
     "tea_engine_set_field": ([ctypes.c_int, _D], ctypes.c_int),
     "tea_engine_profile_enable": ([ctypes.c_int], None),
+    "tea_engine_profile_only": ([ctypes.c_char_p], None),
     "tea_engine_profile_reset": ([], None),
     "tea_engine_profile_get": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), _D], ctypes.c_int),
     "tea_engine_profile_names": ([ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
@@ -108,6 +109,8 @@ _EXT = {
     "tea_engine_ppcg_end": ([_D], None),
     "tea_engine_ppcg_inner2": ([ctypes.c_int], None),
     "tea_engine_ppcg_inner_n": ([ctypes.c_int, ctypes.c_int], None),
+    "tea_engine_ppcg_begin_ur": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_ppcg_inner_plan": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
@@ -312,6 +315,9 @@ class Engine:
     def profile_enable(self, on=True):
         self.lib.tea_engine_profile_enable(1 if on else 0)
 
+    def profile_only(self, prefix: str = ""):
+        self.lib.tea_engine_profile_only(prefix.encode())
+
     def profile_reset(self):
         self.lib.tea_engine_profile_reset()
 
@@ -331,6 +337,11 @@ class Engine:
         out = (ctypes.c_double * 2)()
         self._call("tea_engine_ppcg_begin", int(bool(with_p)), float(beta), out)
         return out[0], out[1]
+
+    def ppcg_begin_ur(self, with_p: bool, beta: float, theta: float):
+        two, rrn = (ctypes.c_double * 2)(), ctypes.c_double()
+        self._call("tea_engine_ppcg_begin_ur", int(bool(with_p)), float(beta), float(theta), two, ctypes.byref(rrn))
+        return two[0], two[1], rrn.value
 
     def ppcg_ur(self, alpha: float, theta: float) -> float:
         out = ctypes.c_double()

@@ -1271,10 +1271,58 @@ template <typename T> void ppcg_inner_impl(const Ext &e, double alpha, double be
 constexpr int INNER_WAVES = 16;
 
 // Microseconds per pass of NS = 1..MAX_PASS_STEPS inner steps at 4096^2 on
-// MI355X (block Jacobi; the ratios hold at 8192^2): the plan of passes for a
-// run of inner steps minimises their sum.
-constexpr double PASS_US_F32[MAX_PASS_STEPS] = {145, 179, 217, 278, 362};
-constexpr double PASS_US_F64[MAX_PASS_STEPS] = {250, 321, 386, 494, 766};
+// MI355X (block Jacobi; the ratios hold at 8192^2), with the kernel
+// use_march picks: the plan of passes for a run of inner steps minimises
+// their sum.
+constexpr double PASS_US_F32[MAX_PASS_STEPS] = {125, 140, 165, 185, 345};
+constexpr double PASS_US_F64[MAX_PASS_STEPS] = {223, 267, 326, 381, 766};
+
+// The streaming pass (k_ppcg_inner_march) is the faster kernel for every NS
+// in FP32 and for NS <= 4 in FP64 (five FP64 steps spill registers); it
+// addresses cells with 32-bit byte offsets, so fields must be under 4 GiB.
+// TEA_INNER_MARCH=0 selects the tile kernels (k_ppcg_inner_n[_x2]).
+template <typename T, int NS> bool use_march()
+{
+    const char *s = getenv("TEA_INNER_MARCH");
+    if (s && s[0] == '0') return false;
+    if (sizeof(T) == 8 && NS > 4) return false;
+    return (double)pitch_of<T>() * (double)g.rows * sizeof(T) < 4294967296.0;
+}
+
+int cu_count()
+{
+    static int v = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+            n = 256;
+        return n;
+    }();
+    return v;
+}
+
+// One wave per band of columns and chunk of strips: enough waves for the
+// resident waves per SIMD the kernel's registers allow (two up to four FP32
+// steps and for one FP64 step, else one), chunks as long as that leaves.
+template <typename T, int NS> void launch_inner_march(const Ext &e, long pt, const StepCoefs &cf, const char *name)
+{
+    constexpr int OX = WAVE - 2 * (NS - 1);
+    const int nbands = (e.x1 - e.x0 + OX) / OX, nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
+    if (nbands < 1 || nstrips < 1) return;
+    const int per_simd = sizeof(T) == 4 ? (NS <= 4 ? 2 : 1) : (NS == 1 ? 2 : 1);
+    int nchunks = cu_count() * 4 * per_simd / nbands;
+    nchunks = nchunks < 1 ? 1 : (nchunks > nstrips ? nstrips : nchunks);
+    const int chunk = (nstrips + nchunks - 1) / nchunks;
+    nchunks = (nstrips + chunk - 1) / chunk;
+    const unsigned grid = (unsigned)((nbands * nchunks + MARCH_WAVES - 1) / MARCH_WAVES);
+    auto go = [&](auto kernel) {
+        klaunch(name, kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk, (const T *)FT<T>(F_SD),
+                FT<T>(X_SD2), (const T *)FT<T>(X_RTEMP), FT<T>(X_RT2), FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
+                (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
+    };
+    if (g.prec == PREC_BLOCK) go(k_ppcg_inner_march<T, PREC_BLOCK, NS>);
+    else if (g.prec == PREC_DIAG) go(k_ppcg_inner_march<T, PREC_DIAG, NS>);
+    else go(k_ppcg_inner_march<T, PREC_NONE, NS>);
+}
 
 int plan_first_pass(int remaining, const double *cost)
 {
@@ -1300,6 +1348,10 @@ template <typename T, int NS> void launch_inner_n(const Ext &e, long pt, const S
     using Tile = InnerTile<X2, INNER_WAVES, NS>;
     static const char *names[MAX_PASS_STEPS] = {"k_ppcg_inner1", "k_ppcg_inner2", "k_ppcg_inner3", "k_ppcg_inner4",
                                                 "k_ppcg_inner5"};
+    if (use_march<T, NS>()) {
+        launch_inner_march<T, NS>(e, pt, cf, nm<T>(names[NS - 1]));
+        return;
+    }
     const long ntiles = (long)((e.x1 - e.x0 + Tile::OUT_X) / Tile::OUT_X) * ((e.y1 - e.y0 + Tile::OUT_Y) / Tile::OUT_Y);
     const unsigned grid = (unsigned)std::min<long>(ntiles, grid_max());
     auto go = [&](auto kernel) {

@@ -406,6 +406,43 @@ __device__ __forceinline__ void strip_solve_col(const T (&rhs)[STRIP], T (&z)[ST
     }
 }
 
+// The two halves of strip_solve_col: the factors cp, bfp depend on the
+// strip's diagonal and Ky only (the reference's block_init,
+// tea_block_jacobi.cuknl:80-111), so a pass of several inner steps forms them
+// once; strip_apply is the per-application part (block_solve_func,
+// tea_block_jacobi.cuknl:9-42).  Same expressions, so the same values.
+template <typename T>
+__device__ __forceinline__ void strip_factor(int len, const T (&diag)[STRIP], const T (&ky)[STRIP + 1],
+                                             T (&cp)[STRIP], T (&bfp)[STRIP])
+{
+    cp[0] = (-ky[1]) / diag[0];
+    bfp[0] = T(0.0);
+#pragma unroll
+    for (int k = 1; k < STRIP; k++) {
+        bfp[k] = cp[k] = T(0.0);
+        if (k < len) {
+            bfp[k] = T(1.0) / (diag[k] - (-ky[k]) * cp[k - 1]);
+            cp[k] = (-ky[k + 1]) * bfp[k];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void strip_apply(const T (&rhs)[STRIP], T (&z)[STRIP], int len, T diag0,
+                                            const T (&ky)[STRIP + 1], const T (&cp)[STRIP], const T (&bfp)[STRIP])
+{
+    T dp[STRIP];
+    dp[0] = rhs[0] / diag0;
+#pragma unroll
+    for (int k = 1; k < STRIP; k++)
+        if (k < len) dp[k] = (rhs[k] - (-ky[k]) * dp[k - 1]) * bfp[k];
+#pragma unroll
+    for (int k = STRIP - 1; k >= 0; k--) {
+        if (k == len - 1) z[k] = dp[k];
+        else if (k < len - 1) z[k] = dp[k] - cp[k] * z[k + 1];
+    }
+}
+
 // Coefficients (alpha_k, beta_k) of consecutive inner steps, passed by value.
 constexpr int MAX_PASS_STEPS = 5;
 struct StepCoefs {
@@ -525,6 +562,12 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n_x2(Ext e, long pitch
             sdc[0][k] = sv[0][k + 1];
             sdc[1][k] = sv[1][k + 1];
         }
+        float cpf[2][STRIP], bff[2][STRIP];   // block-Jacobi factors, formed once for all steps
+        if (PREC == PREC_BLOCK) {
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+                if (act[q] > 0) strip_factor(act[q], diag[q], kyv[q], cpf[q], bff[q]);
+        }
 
 #pragma unroll
         for (int st = 0; st < NS; st++) {
@@ -569,14 +612,7 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n_x2(Ext e, long pitch
             }
 #pragma unroll
             for (int q = 0; q < 2; q++) {
-                if (PREC == PREC_BLOCK && ak[q] > 0) {
-                    float dq[STRIP], kq[STRIP + 1];
-#pragma unroll
-                    for (int k = 0; k < STRIP; k++) dq[k] = diag[q][k];
-#pragma unroll
-                    for (int k = 0; k < STRIP + 1; k++) kq[k] = kyv[q][k];
-                    strip_solve_col(rv[q], zv[q], ak[q], dq, kq);
-                }
+                if (PREC == PREC_BLOCK && ak[q] > 0) strip_apply(rv[q], zv[q], ak[q], diag[q][0], kyv[q], cpf[q], bff[q]);
 #pragma unroll
                 for (int k = 0; k < STRIP; k++) {
                     if (k < ak[q]) {
@@ -688,6 +724,11 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n(Ext e, long pitch, c
             diag[k] = T(1.0) + (kyv[k + 1] + kyv[k]) + (kxr[k] + kxv[k]);
             sdc[k] = sv[k + 1];
         }
+        // block-Jacobi factors formed once for all steps, where the registers
+        // allow (more than three FP64 steps would spill)
+        constexpr bool HOIST = sizeof(T) == 4 || NS <= 3;
+        T cpf[STRIP], bff[STRIP];
+        if (HOIST && PREC == PREC_BLOCK && act > 0) strip_factor<T>(act, diag, kyv, cpf, bff);
 
 #pragma unroll
         for (int st = 0; st < NS; st++) {
@@ -719,7 +760,10 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n(Ext e, long pitch, c
                     rv[k] = rtc[k] - res;
                 }
             }
-            if (PREC == PREC_BLOCK && ak > 0) strip_solve_col<T>(rv, zv, ak, diag, kyv);
+            if (PREC == PREC_BLOCK && ak > 0) {
+                if (HOIST) strip_apply<T>(rv, zv, ak, diag[0], kyv, cpf, bff);
+                else strip_solve_col<T>(rv, zv, ak, diag, kyv);
+            }
 #pragma unroll
             for (int k = 0; k < STRIP; k++) {
                 if (k < ak) {
@@ -750,6 +794,237 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n(Ext e, long pitch, c
                 }
         }
         __syncthreads();
+    }
+}
+
+// ---- streaming pass of NS inner steps ------------------------------------------------
+
+// One block-Jacobi strip (4 rows of one column) as read from memory: the
+// fields, the coefficients, and for the wave's edge lanes the neighbouring
+// column's sd (lane 0: left, last lane: right) and the right-face Kx.
+template <typename T> struct MarchRaw {
+    T sd[STRIP], rt[STRIP], ut[STRIP], kx[STRIP], ky[STRIP + 1], mi[STRIP], kxe[STRIP], sde[STRIP];
+};
+// A strip inside the pipeline: what it carries from step to step and the
+// coefficients every step uses (the diagonal is re-formed per step).
+template <typename T> struct MarchStrip {
+    T sd[STRIP], rt[STRIP], ut[STRIP], kx[STRIP], kxr[STRIP], ky[STRIP + 1], cp[STRIP], bfp[STRIP], mi[STRIP];
+    int r0, act;
+};
+
+constexpr int MARCH_WAVES = 4;   // independent waves per workgroup
+
+// Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, VALU moves with
+// no LDS round trip): lane l receives v from lane l-1 (from_below) or l+1
+// (from_above); the end lane keeps its own value, as __shfl_up/__shfl_down.
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_from_below(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
+__device__ __forceinline__ float lane_from_above(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
+__device__ __forceinline__ double lane_from_below(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shr1((int)(b & 0xffffffffLL)), hi = dpp_shr1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double lane_from_above(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shl1((int)(b & 0xffffffffLL)), hi = dpp_shl1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// NS = 1..5 PPCG inner steps in one streaming pass (temporal blocking; no
+// reference counterpart): the composition of NS k_ppcg_inner_fused<T>
+// launches with coefficients (cf.a[k], cf.b[k]), bit-identical to them.
+// Each wave owns a band of WAVE columns (one per lane; NS-1 halo lanes at each
+// side, so WAVE - 2(NS-1) columns are written) and a chunk of `chunk` strips,
+// and walks down it one strip per iteration: at iteration i strip i enters
+// (read one iteration ahead) and strip i-1-m moves from step m to step m+1
+// for m = 0..NS-1, so strip i-NS leaves the pipeline finished.  Step m+1 of a
+// strip needs step m of the strip itself and of the rows just below and above
+// it (the previous strip's last row, kept from the iteration before, and the
+// next strip's first row, formed earlier in this iteration); columns come from
+// lane shifts (memory at the edge lanes in the first step, stale values there
+// afterwards, which only reach halo lanes).  The strips in the pipeline move
+// down one place per iteration (register moves: a ring of slots unrolled NS+1
+// times needs more registers than the 256 that keep two waves per SIMD).  The NS strips before the
+// chunk and the NS after it are read and advanced only as far as the chunk's
+// strips need.  No barriers: the waves are independent.  sd and rtemp are
+// read from the _in and written to the _out buffers, utemp in place; cells
+// outside the extent keep their sd.
+template <typename T, int PREC, int NS>
+__global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
+    Ext e, long pitch, int chunk, const T *__restrict__ sd_in, T *__restrict__ sd_out, const T *__restrict__ rt_in,
+    T *__restrict__ rt_out, T *__restrict__ utemp, const T *__restrict__ Mi, const T *__restrict__ Kx,
+    const T *__restrict__ Ky, StepCoefs cf)
+{
+    static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
+    constexpr int HL = NS - 1, OUT_X = WAVE - 2 * HL;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (e.x1 < e.x0 || e.y1 < e.y0) return;
+    const int nbands = (e.x1 - e.x0 + OUT_X) / OUT_X;
+    const int nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
+    const int nchunks = (nstrips + chunk - 1) / chunk;
+    const int w = blockIdx.x * MARCH_WAVES + (int)(threadIdx.x / WAVE);
+    if (w >= nbands * nchunks) return;
+    const int band = w % nbands, ck = w / nbands;
+    const int c = e.x0 + band * OUT_X - HL + lane;
+    const bool col_in = c >= e.x0 && c <= e.x1;
+    const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+    const bool out_lane = lane >= HL && lane <= WAVE - 1 - HL;
+    const bool right_lane = lane < WAVE - 1;
+    const bool edge_kx = !right_lane || c + 1 > e.x1;   // right-face Kx not in the next lane
+    // Byte offsets of cells from the field bases, in 32 bits (the host
+    // launches this kernel only for fields under 4 GiB; every offset read or
+    // written is that of a cell of the field, so it is in range).
+    const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
+    auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
+    auto ld = [](const T *base, unsigned b) { return *(const T *)((const char *)base + b); };
+    auto st_ = [](T *base, unsigned b, T v) { *(T *)((char *)base + b) = v; };
+    const int s_first = ck * chunk;
+    const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
+
+    auto strip_act = [&](int r0) {
+        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
+        len = len < 0 ? 0 : (len > STRIP ? STRIP : len);
+        return col_in ? len : 0;
+    };
+    // Issues the reads of strip s.
+    auto fetch = [&](int s, MarchRaw<T> &m) {
+        const int r0 = e.y0 + s * STRIP, act = strip_act(r0);
+        const unsigned b0 = off(r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const int rr = r0 + k;
+            m.sd[k] = (col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) ? ld(sd_in, b0 + k * row_b) : T(0.0);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = (k <= act && act > 0) ? ld(Ky, b0 + k * row_b) : T(0.0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            m.kx[k] = m.rt[k] = m.ut[k] = m.mi[k] = m.kxe[k] = m.sde[k] = T(0.0);
+            if (k < act) {
+                const unsigned bk = b0 + k * row_b;
+                m.kx[k] = ld(Kx, bk);
+                m.rt[k] = ld(rt_in, bk);
+                if (out_lane) m.ut[k] = ld(utemp, bk);
+                if (PREC == PREC_DIAG) m.mi[k] = ld(Mi, bk);
+                if (edge_kx) m.kxe[k] = ld(Kx, bk + (unsigned)sizeof(T));
+                if (lane == 0) m.sde[k] = ld(sd_in, bk - (unsigned)sizeof(T));
+                else if (!right_lane) m.sde[k] = ld(sd_in, bk + (unsigned)sizeof(T));
+            }
+        }
+    };
+    // Forms the pipeline strip of strip s from its reads.
+    auto prepare = [&](int s, const MarchRaw<T> &raw, MarchStrip<T> &m) {
+        m.r0 = e.y0 + s * STRIP;
+        m.act = strip_act(m.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            m.sd[k] = raw.sd[k];
+            m.rt[k] = raw.rt[k];
+            m.ut[k] = raw.ut[k];
+            m.kx[k] = raw.kx[k];
+            m.mi[k] = raw.mi[k];
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = raw.ky[k];
+        T diag[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const T sh = lane_from_above(m.kx[k]);
+            m.kxr[k] = T(0.0);
+            if (k < m.act) m.kxr[k] = edge_kx ? raw.kxe[k] : sh;
+            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (m.kxr[k] + m.kx[k]);
+        }
+        if (PREC == PREC_BLOCK && m.act > 0) strip_factor<T>(m.act, diag, m.ky, m.cp, m.bfp);
+    };
+
+    // Moves strip m from step st to st+1 (lo, hi: sd of the rows below and
+    // above it at step st; sde: the edge lanes' neighbour column at step 0).
+    auto advance = [&](MarchStrip<T> &m, T lo, T hi, const T (&sde)[STRIP], int st) {
+        const T a = (T)cf.a[st], b = (T)cf.b[st];
+        const int ak = (st < NS - 1 || out_lane) ? m.act : 0;
+        T rv[STRIP], zv[STRIP], diag[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const T from_left = lane_from_below(m.sd[k]);
+            const T from_right = lane_from_above(m.sd[k]);
+            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (m.kxr[k] + m.kx[k]);
+            rv[k] = zv[k] = T(0.0);
+            if (k < ak) {
+                const T below = k == 0 ? lo : m.sd[k - 1];
+                const T above = k == STRIP - 1 ? hi : m.sd[k + 1];
+                const T left = (st > 0 || lane > 0) ? from_left : sde[k];
+                const T right = (st > 0 || right_lane) ? from_right : sde[k];
+                const T res = diag[k] * m.sd[k] - (m.ky[k + 1] * above + m.ky[k] * below)
+                              - (m.kxr[k] * right + m.kx[k] * left);
+                rv[k] = m.rt[k] - res;
+            }
+        }
+        if (PREC == PREC_BLOCK && ak > 0) strip_apply<T>(rv, zv, ak, diag[0], m.ky, m.cp, m.bfp);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            if (k < ak) {
+                T sdn;
+                if (PREC == PREC_BLOCK) sdn = a * m.sd[k] + b * zv[k];
+                else if (PREC == PREC_DIAG) sdn = a * m.sd[k] + b * rv[k] * m.mi[k];
+                else sdn = a * m.sd[k] + b * rv[k];
+                m.sd[k] = sdn;
+                m.rt[k] = rv[k];
+                m.ut[k] = m.ut[k] + sdn;
+            }
+        }
+    };
+
+    MarchStrip<T> H[NS];   // H[m]: strip i-1-m at step m
+    T P[NS];               // P[m]: last row of strip i-2-m at step m
+    T sde0[STRIP];         // edge-lane neighbours of the strip in H[0]
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+        H[m] = MarchStrip<T>{};
+        H[m].act = 0;
+        H[m].r0 = e.y0;
+        P[m] = T(0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) sde0[k] = T(0.0);
+    MarchRaw<T> nxt;
+    fetch(s_first - NS, nxt);
+    for (int i = s_first - NS; i <= s_last + NS; i++) {
+        MarchStrip<T> cur;
+        T sde_cur[STRIP];
+        prepare(i, nxt, cur);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde_cur[k] = nxt.sde[k];
+        if (i < s_last + NS) fetch(i + 1, nxt);   // read one strip ahead
+        T hi = cur.sd[0];
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+            const T last = H[m].sd[STRIP - 1];
+            advance(H[m], P[m], hi, sde0, m);   // strip i-1-m: step m -> m+1
+            P[m] = last;
+            hi = H[m].sd[0];
+        }
+        const MarchStrip<T> &f = H[NS - 1];   // strip i-NS, finished
+        const int sf = i - NS;
+        if (out_lane && sf >= s_first && sf <= s_last) {
+            const unsigned b0 = off(f.r0);
+#pragma unroll
+            for (int k = 0; k < STRIP; k++)
+                if (k < f.act) {
+                    const unsigned bk = b0 + k * row_b;
+                    st_(sd_out, bk, f.sd[k]);
+                    st_(rt_out, bk, f.rt[k]);
+                    st_(utemp, bk, f.ut[k]);
+                }
+        }
+#pragma unroll
+        for (int m = NS - 1; m > 0; m--) H[m] = H[m - 1];
+        H[0] = cur;
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde0[k] = sde_cur[k];
     }
 }
 

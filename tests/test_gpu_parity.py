@@ -14,6 +14,8 @@ libraries, comparing every field afterwards.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -92,12 +94,13 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
         assert a.values() == b.values(), (a, b)
 
 
-@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES"])
+@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
-    """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0), and the fused
-    outer iteration with one inner step per pass (TEA_INNER_PASSES=0), on the GPU
-    give the same results as the oracle."""
+    """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0), the fused
+    outer iteration with one inner step per pass (TEA_INNER_PASSES=0), and the
+    inner passes on the tile kernels (TEA_INNER_MARCH=0), on the GPU give the
+    same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -152,6 +155,18 @@ CALLS = {
     "cg_calc_p": lambda e: e.cg_calc_p(0.61),
     "field_summary": lambda e: e.field_summary(),
 }
+def _tile_kernels(fn):
+    """Runs fn with the engine's inner-step passes on the tile kernels
+    (TEA_INNER_MARCH=0; the oracle ignores the switch)."""
+    def call(e):
+        os.environ["TEA_INNER_MARCH"] = "0"
+        try:
+            return fn(e)
+        finally:
+            del os.environ["TEA_INNER_MARCH"]
+    return call
+
+
 PPCG_CALLS = {
     "ppcg_init_1": lambda e: e.ppcg_init(1),
     "ppcg_init_2": lambda e: e.ppcg_init(2),
@@ -174,6 +189,10 @@ PPCG_CALLS = {
     **{f"fused_ppcg_inner_n{n}": (lambda n: lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_inner_n(7 - n, n)))(n) for n in range(1, 6)},
+    # the same passes through the tile kernels instead of the streaming kernel
+    **{f"fused_ppcg_inner_n{n}_tile": (lambda n: _tile_kernels(lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_inner_n(7 - n, n))))(n) for n in range(1, 6)},
 }
 CHEBY_CALLS = {
     "cheby_init": lambda e: e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3),

@@ -1274,19 +1274,18 @@ constexpr int INNER_WAVES = 16;
 // MI355X (block Jacobi; the ratios hold at 8192^2), with the kernel
 // use_march picks: the plan of passes for a run of inner steps minimises
 // their sum.
-constexpr double PASS_US_F32[MAX_PASS_STEPS] = {125, 140, 165, 185, 345};
-constexpr double PASS_US_F64[MAX_PASS_STEPS] = {223, 267, 326, 381, 766};
+constexpr double PASS_US_F32[MAX_PASS_STEPS] = {125, 140, 158, 177, 310};
+constexpr double PASS_US_F64[MAX_PASS_STEPS] = {223, 266, 319, 361, 739};
 
 // The streaming pass (k_ppcg_inner_march) is the faster kernel for every NS
-// in FP32 and for NS <= 4 in FP64 (five FP64 steps spill registers); it
-// addresses cells with 32-bit byte offsets, so fields must be under 4 GiB.
-// TEA_INNER_MARCH=0 selects the tile kernels (k_ppcg_inner_n[_x2]).
+// in both precisions (measured against k_ppcg_inner_n[_x2]); it addresses
+// cells with 32-bit byte offsets, so fields must be under 4 GiB.
+// TEA_INNER_MARCH=0 selects the tile kernels.
 template <typename T, int NS> bool use_march()
 {
     const char *s = getenv("TEA_INNER_MARCH");
     if (s && s[0] == '0') return false;
-    if (sizeof(T) == 8 && NS > 4) return false;
-    return (double)pitch_of<T>() * (double)g.rows * sizeof(T) < 4294967296.0;
+    return (double)pitch_of<T>() * (double)g.rows * sizeof(T) < 4294967296.0 - 65536.0;
 }
 
 int cu_count()

@@ -799,6 +799,31 @@ __global__ __launch_bounds__(NW * WAVE) void k_ppcg_inner_n(Ext e, long pitch, c
 
 // ---- streaming pass of NS inner steps ------------------------------------------------
 
+// A field seen through a raw buffer resource of `bytes` bytes (gfx9
+// descriptor word 3: 0x00020000): an offset at or beyond `bytes` reads 0 and
+// writes nothing, with no memory access.
+constexpr unsigned OOB = 0xFFFFFFF0u;
+template <typename T> struct FieldBuf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ FieldBuf(const T *base, unsigned bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000))
+    {
+    }
+    __device__ __forceinline__ T load(unsigned off) const
+    {
+        if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+        else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    }
+    __device__ __forceinline__ void store(unsigned off, T v) const
+    {
+        if constexpr (sizeof(T) == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b64(
+                __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, (int)off, 0, 0);
+    }
+};
+
 // One block-Jacobi strip (4 rows of one column) as read from memory: the
 // fields, the coefficients, and for the wave's edge lanes the neighbouring
 // column's sd (lane 0: left, last lane: right) and the right-face Kx.
@@ -848,7 +873,10 @@ __device__ __forceinline__ double lane_from_above(double v)
 // lane shifts (memory at the edge lanes in the first step, stale values there
 // afterwards, which only reach halo lanes).  The strips in the pipeline move
 // down one place per iteration (register moves: a ring of slots unrolled NS+1
-// times needs more registers than the 256 that keep two waves per SIMD).  The NS strips before the
+// times needs more registers than the 256 that keep two waves per SIMD).
+// Reads and writes go through bounded buffer resources, so the guards of the
+// reference semantics (rows and columns outside the extent, halo lanes) are
+// offsets, not branches.  The NS strips before the
 // chunk and the NS after it are read and advanced only as far as the chunk's
 // strips need.  No barriers: the waves are independent.  sd and rtemp are
 // read from the _in and written to the _out buffers, utemp in place; cells
@@ -866,7 +894,9 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     const int nbands = (e.x1 - e.x0 + OUT_X) / OUT_X;
     const int nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
     const int nchunks = (nstrips + chunk - 1) / chunk;
-    const int w = blockIdx.x * MARCH_WAVES + (int)(threadIdx.x / WAVE);
+    // the wave's index, known to the compiler as uniform across the wave, so
+    // the band, chunk and strip bookkeeping and the loop stay scalar
+    const int w = blockIdx.x * MARCH_WAVES + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     if (w >= nbands * nchunks) return;
     const int band = w % nbands, ck = w / nbands;
     const int c = e.x0 + band * OUT_X - HL + lane;
@@ -875,13 +905,15 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     const bool out_lane = lane >= HL && lane <= WAVE - 1 - HL;
     const bool right_lane = lane < WAVE - 1;
     const bool edge_kx = !right_lane || c + 1 > e.x1;   // right-face Kx not in the next lane
-    // Byte offsets of cells from the field bases, in 32 bits (the host
-    // launches this kernel only for fields under 4 GiB; every offset read or
-    // written is that of a cell of the field, so it is in range).
+    // Fields are read and written through buffer resources over their rows
+    // 0 .. y1+1 (the host launches this kernel only for fields under 4 GiB):
+    // an access the reference semantics exclude gets the offset OOB, which
+    // reads 0 and writes nothing, so no access needs a branch.
     const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
+    const unsigned bytes = (unsigned)(e.y1 + 2) * row_b;
+    const FieldBuf<T> b_sd(sd_in, bytes), b_rt(rt_in, bytes), b_ut(utemp, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
+        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes);
     auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
-    auto ld = [](const T *base, unsigned b) { return *(const T *)((const char *)base + b); };
-    auto st_ = [](T *base, unsigned b, T v) { *(T *)((char *)base + b) = v; };
     const int s_first = ck * chunk;
     const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
 
@@ -894,26 +926,24 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     auto fetch = [&](int s, MarchRaw<T> &m) {
         const int r0 = e.y0 + s * STRIP, act = strip_act(r0);
         const unsigned b0 = off(r0);
+        constexpr unsigned W = sizeof(T);
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
             const int rr = r0 + k;
-            m.sd[k] = (col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) ? ld(sd_in, b0 + k * row_b) : T(0.0);
+            m.sd[k] = b_sd.load((col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) ? b0 + k * row_b : OOB);
         }
 #pragma unroll
-        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = (k <= act && act > 0) ? ld(Ky, b0 + k * row_b) : T(0.0);
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = b_ky.load((k <= act && act > 0) ? b0 + k * row_b : OOB);
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
-            m.kx[k] = m.rt[k] = m.ut[k] = m.mi[k] = m.kxe[k] = m.sde[k] = T(0.0);
-            if (k < act) {
-                const unsigned bk = b0 + k * row_b;
-                m.kx[k] = ld(Kx, bk);
-                m.rt[k] = ld(rt_in, bk);
-                if (out_lane) m.ut[k] = ld(utemp, bk);
-                if (PREC == PREC_DIAG) m.mi[k] = ld(Mi, bk);
-                if (edge_kx) m.kxe[k] = ld(Kx, bk + (unsigned)sizeof(T));
-                if (lane == 0) m.sde[k] = ld(sd_in, bk - (unsigned)sizeof(T));
-                else if (!right_lane) m.sde[k] = ld(sd_in, bk + (unsigned)sizeof(T));
-            }
+            const bool a = k < act;
+            const unsigned bk = b0 + k * row_b;
+            m.kx[k] = b_kx.load(a ? bk : OOB);
+            m.rt[k] = b_rt.load(a ? bk : OOB);
+            m.ut[k] = b_ut.load(a && out_lane ? bk : OOB);
+            m.mi[k] = PREC == PREC_DIAG ? b_mi.load(a ? bk : OOB) : T(0.0);
+            m.kxe[k] = b_kx.load(a && edge_kx ? bk + W : OOB);
+            m.sde[k] = b_sd.load(a && lane == 0 ? bk - W : (a && !right_lane ? bk + W : OOB));
         }
     };
     // Forms the pipeline strip of strip s from its reads.
@@ -1009,16 +1039,14 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
         }
         const MarchStrip<T> &f = H[NS - 1];   // strip i-NS, finished
         const int sf = i - NS;
-        if (out_lane && sf >= s_first && sf <= s_last) {
-            const unsigned b0 = off(f.r0);
+        const bool out = out_lane && sf >= s_first && sf <= s_last;
+        const unsigned b0 = off(f.r0);
 #pragma unroll
-            for (int k = 0; k < STRIP; k++)
-                if (k < f.act) {
-                    const unsigned bk = b0 + k * row_b;
-                    st_(sd_out, bk, f.sd[k]);
-                    st_(rt_out, bk, f.rt[k]);
-                    st_(utemp, bk, f.ut[k]);
-                }
+        for (int k = 0; k < STRIP; k++) {
+            const unsigned bk = out && k < f.act ? b0 + k * row_b : OOB;
+            b_sdo.store(bk, f.sd[k]);
+            b_rto.store(bk, f.rt[k]);
+            b_ut.store(bk, f.ut[k]);
         }
 #pragma unroll
         for (int m = NS - 1; m > 0; m--) H[m] = H[m - 1];

@@ -273,11 +273,21 @@ def main():
 
     def summary(variant):
         r, g_ms, w_ms, _, t, _ = runs[variant]
+        f = r.final
         return {"ms_per_step": round(g_ms / args.steps, 3), "wall_ms_per_step": round(w_ms / args.steps, 3),
                 "iterations": r.iterations[t], "ppcg_inner": r.ppcg_inner[t],
-                "residual_replacements": r.refinements[t], "exact_rel_residual": r.exact_error[t]}
+                "residual_replacements": r.refinements[t], "exact_rel_residual": r.exact_error[t],
+                "field_summary": {"step": f.step, "volume": f.volume, "mass": f.mass, "internal_energy": f.energy,
+                                  "temperature": f.u}}
 
     comparison = {v: summary(v) for v in runs}
+    # every variant's final field summary against plain FP64 PPCG with the same true-residual stop
+    if "fp64_true" in comparison:
+        ref = comparison["fp64_true"]["field_summary"]
+        for v in runs:
+            fs = comparison[v]["field_summary"]
+            comparison[v]["field_summary_rel_err_vs_fp64_true"] = max(
+                abs(fs[k] - ref[k]) / abs(ref[k]) for k in ("mass", "internal_energy", "temperature"))
     for v in [v for v in runs if v != main_variant]:
         comparison[f"speedup_vs_{v}"] = round(comparison[v]["ms_per_step"] / comparison[main_variant]["ms_per_step"],
                                               3)

@@ -89,16 +89,25 @@ def variants(n):
         "fp32": dict(inner_precision="fp32"),
         "fp64_true": dict(inner_precision="fp64"),
         "fp64_reference": dict(),
+        # the plain FP64 solvers on the reference's rectangle layout (run with --only)
+        "ref_layout_cg_diag": dict(solver="cg", prec="jac_diag", layout="reference"),
+        "ref_layout_chebyshev": dict(solver="chebyshev", prec="none", layout="reference", ch_cg_presteps=20),
+        "ref_layout_ppcg_block": dict(layout="reference"),
     }
 
 
+DEFAULT_VARIANTS = ("headline", "fp32_extrapolate", "fp32", "fp64_true", "fp64_reference")
+
+
 def make_deck(path, n, steps, solver="ppcg", prec="jac_block", eps=1e-10, max_iters=10000, inner=10,
-              epslim=None, **kw):
+              epslim=None, layout="random", **kw):
     from tealeaf_cuda_amd import Deck, random_layout
 
+    if layout == "random":
+        kw["states"] = random_layout(LAYOUT_SEED, cells=n)   # else the deck's default: the reference's rectangles
     d = Deck(x_cells=n, y_cells=n, end_step=steps, solver=solver, preconditioner=prec, eps=eps,
              max_iters=max_iters, halo_depth=1, ppcg_inner_steps=inner, summary_frequency=0,
-             check_result=True, ch_cg_epslim=epslim, states=random_layout(LAYOUT_SEED, cells=n), **kw)
+             check_result=True, ch_cg_epslim=epslim, **kw)
     return d.write(path)
 
 
@@ -254,7 +263,7 @@ def main():
     if not WEIGHTS[n].exists():
         print(f"bench.py: no warm-start weights for {n}^2 ({WEIGHTS[n]})", file=sys.stderr)
         sys.exit(2)
-    chosen = [v for v in variants(n) if args.only is None or v in args.only.split(",")]
+    chosen = [v for v in variants(n) if (v in DEFAULT_VARIANTS if args.only is None else v in args.only.split(","))]
     if not chosen:
         print(f"bench.py: --only names none of {list(variants(n))}", file=sys.stderr)
         sys.exit(2)

@@ -93,6 +93,9 @@ def variants(n):
         "ref_layout_cg_diag": dict(solver="cg", prec="jac_diag", layout="reference"),
         "ref_layout_chebyshev": dict(solver="chebyshev", prec="none", layout="reference", ch_cg_presteps=20),
         "ref_layout_ppcg_block": dict(layout="reference"),
+        # Chebyshev (block Jacobi) refined by restarts to the FP64 true residual, FP64 or FP32 inner solves
+        "cheby_block_fp64": dict(solver="chebyshev", ch_cg_presteps=20, inner_precision="fp64", inner_eps=1e-4),
+        "cheby_block_fp32": dict(solver="chebyshev", ch_cg_presteps=20, inner_precision="fp32", inner_eps=1e-4),
     }
 
 

@@ -574,13 +574,22 @@ EXPORT void tea_leaf_common_finalise_kernel_cuda_(void)
 EXPORT void tea_leaf_calc_2norm_kernel_cuda_(int *norm_array, double *norm)
 {
     CHECK_INIT();
-    const double *a, *b;
-    if (*norm_array == 0) { a = U0; b = U0; }
-    else if (*norm_array == 1) { a = R; b = R; }
-    else if (*norm_array == 2) { a = R; b = (g.prec != PREC_NONE) ? Z : R; }
-    else { die("Invalid value '%d' for norm_array passed, should be [0, 1, 2]", *norm_array); return; }
+    if (*norm_array < 0 || *norm_array > 2) {
+        die("Invalid value '%d' for norm_array passed, should be [0, 1, 2]", *norm_array);
+        return;
+    }
     double s = 0;
-    REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, { ta = a[i]; tb = b[i]; });
+    if (g.fp32) {   /* the FP32 solver's fields (its right-hand side, residual, z) */
+        const float *u0 = g.f32[X_U0], *r = g.f32[F_R], *z = (g.prec != PREC_NONE) ? g.f32[X_Z] : g.f32[F_R];
+        const float *a = *norm_array == 0 ? u0 : r, *b = *norm_array == 0 ? u0 : (*norm_array == 1 ? r : z);
+        REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, { ta = a[i]; tb = b[i]; });
+    } else {
+        const double *a, *b;
+        if (*norm_array == 0) { a = U0; b = U0; }
+        else if (*norm_array == 1) { a = R; b = R; }
+        else { a = R; b = (g.prec != PREC_NONE) ? Z : R; }
+        REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, { ta = a[i]; tb = b[i]; });
+    }
     *norm = s;
 }
 
@@ -633,55 +642,24 @@ static void upload_coefs(const double *a, const double *b, int n)
     if (n) { memcpy(g.ch_a, a, (size_t)n * sizeof(double)); memcpy(g.ch_b, b, (size_t)n * sizeof(double)); }
 }
 
-/* w = A u; r = u0 - w; then p from M^-1 r (cheby_solve_init_p / calc_p share this). */
-static void cheby_residual(void)
-{
-    FOR_EXTENT(IX0, IX1, IY0, IY1, {
-        W[i] = apply_A(U, i);
-        R[i] = U0[i] - W[i];
-    });
-}
-
 /* tea_leaf_kernel_cuda.cu:105-123; tea_leaf_cheby.cuknl:5-82 */
 EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const double *ch_betas,
                                              int *n_coefs, const double *theta)
 {
     CHECK_INIT();
-    if (g.fp32) { die("%s: not available in the FP32 solver mode", __func__); return; }
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
     upload_coefs(ch_alphas, ch_betas, *n_coefs);
-    const double th = *theta;
-    cheby_residual();
-    if (g.prec == PREC_BLOCK) {
-        block_solve_extent(R, g.zl, IX0, IX1, IY0, IY1);
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = g.zl[i] / th; });
-    } else if (g.prec == PREC_DIAG) {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = (MI[i] * R[i]) / th; });
-    } else {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = R[i] / th; });
-    }
-    FOR_EXTENT(IX0, IX1, IY0, IY1, { U[i] += P[i]; });
+    if (g.fp32) cheby_step_f32(1, *theta, 0.0, 0.0); else cheby_step_f64(1, *theta, 0.0, 0.0);
 }
 
 /* tea_leaf_kernel_cuda.cu:125-135; tea_leaf_cheby.cuknl:84-152 */
 EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
 {
     CHECK_INIT();
-    if (g.fp32) { die("%s: not available in the FP32 solver mode", __func__); return; }
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
     const int k = *cheby_calc_step - 1;
     if (k < 0 || k >= g.n_coefs) { die("cheby_iterate: step %d outside the %d uploaded coefficients", k + 1, g.n_coefs); return; }
-    const double al = g.ch_a[k], be = g.ch_b[k];
-    cheby_residual();
-    if (g.prec == PREC_BLOCK) {
-        block_solve_extent(R, g.zl, IX0, IX1, IY0, IY1);
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = al * P[i] + be * g.zl[i]; });
-    } else if (g.prec == PREC_DIAG) {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = al * P[i] + be * MI[i] * R[i]; });
-    } else {
-        FOR_EXTENT(IX0, IX1, IY0, IY1, { P[i] = al * P[i] + be * R[i]; });
-    }
-    FOR_EXTENT(IX0, IX1, IY0, IY1, { U[i] += P[i]; });
+    if (g.fp32) cheby_step_f32(0, 0.0, g.ch_a[k], g.ch_b[k]); else cheby_step_f64(0, 0.0, g.ch_a[k], g.ch_b[k]);
 }
 
 /* ---------------- solver: Jacobi ----------------------------------------------------- */
@@ -907,15 +885,17 @@ EXPORT void tea_engine_mp_setup(void)
     for (size_t i = 0; i < n2; i++) { g.f32[X_KX][i] = (float)KX[i]; g.f32[X_KY][i] = (float)KY[i]; }
 }
 
-/* r32 = (float) r, u32 = 0 (the correction), rr = r32.r32 */
+/* r32 = (float) r, the FP32 right-hand side u0_32 = r32 too (Chebyshev
+ * reads it), u32 = 0 (the correction), rr = r32.r32 */
 EXPORT void tea_engine_mp_begin(double *rr)
 {
     CHECK_INIT();
     if (!g.f32[1]) { die("tea_engine_mp_begin: call tea_engine_set_precision(32) first"); return; }
-    float *R32 = g.f32[F_R], *U32 = g.f32[F_U];
+    float *R32 = g.f32[F_R], *U32 = g.f32[F_U], *U032 = g.f32[X_U0];
     double s = 0;
     REDUCE_EXTENT(s, IX0, IX1, IY0, IY1, {
         R32[i] = (float)R[i];
+        U032[i] = R32[i];
         U32[i] = 0.0f;
         ta = R32[i]; tb = R32[i];
     });

@@ -751,33 +751,35 @@ __global__ __launch_bounds__(BLOCK) void k_ws_apply(Ext e, long pitch, double *_
 // ---- solver: Chebyshev --------------------------------------------------------------
 
 // tea_leaf_cheby.cuknl:5-68 (first = 1, p = M^-1 r / theta) and :84-152
-// (p = alpha p + beta M^-1 r); both start with w = A u, r = u0 - w.
-template <int PREC, int FIRST>
-__global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const double *__restrict__ u,
-                                                   const double *__restrict__ u0, double *__restrict__ p,
-                                                   double *__restrict__ r, double *__restrict__ w,
-                                                   const double *__restrict__ Mi, const double *__restrict__ Kx,
-                                                   const double *__restrict__ Ky, double theta, double alpha,
-                                                   double beta)
+// (p = alpha p + beta M^-1 r); both start with w = A u, r = u0 - w.  T =
+// float is the inner solver of the mixed-precision refinement (scalars
+// rounded to T once).
+template <typename T, int PREC, int FIRST>
+__global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const T *__restrict__ u,
+                                                   const T *__restrict__ u0, T *__restrict__ p, T *__restrict__ r,
+                                                   T *__restrict__ w, const T *__restrict__ Mi,
+                                                   const T *__restrict__ Kx, const T *__restrict__ Ky, double theta_in,
+                                                   double alpha_in, double beta_in)
 {
+    const T theta = (T)theta_in, alpha = (T)alpha_in, beta = (T)beta_in;
     STRIP_BEGIN
-    double rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+    T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
             const long i = i0 + k * pitch;
-            const double wv = apply_A(u, Kx, Ky, i, pitch);
+            const T wv = apply_A(u, Kx, Ky, i, pitch);
             w[i] = wv;
             rv[k] = u0[i] - wv;
             r[i] = rv[k];
         }
     }
-    if (PREC == PREC_BLOCK) precond_strip<double, PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, Kx, Ky);
+    if (PREC == PREC_BLOCK) precond_strip<T, PREC_BLOCK>(rv, zv, s.len, i0, pitch, Mi, Kx, Ky);
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
             const long i = i0 + k * pitch;
-            double pv;
+            T pv;
             if (FIRST) {
                 if (PREC == PREC_BLOCK) pv = zv[k] / theta;
                 else if (PREC == PREC_DIAG) pv = (Mi[i] * rv[k]) / theta;
@@ -794,8 +796,8 @@ __global__ __launch_bounds__(BLOCK) void k_cheby_p(Ext e, long pitch, const doub
 }
 
 // tea_leaf_cheby.cuknl:71-82: u += p
-__global__ __launch_bounds__(BLOCK) void k_add(Ext e, long pitch, double *__restrict__ u,
-                                               const double *__restrict__ p)
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_add(Ext e, long pitch, T *__restrict__ u, const T *__restrict__ p)
 {
     STRIP_BEGIN
 #pragma unroll
@@ -1136,13 +1138,20 @@ EXPORT void tea_leaf_common_finalise_kernel_cuda_(void)
 EXPORT void tea_leaf_calc_2norm_kernel_cuda_(int *norm_array, double *norm)
 {
     CHECK_INIT();
-    const double *a, *b;
-    if (*norm_array == 0) { a = F(X_U0); b = F(X_U0); }
-    else if (*norm_array == 1) { a = F(F_R); b = F(F_R); }
-    else if (*norm_array == 2) { a = F(F_R); b = (g.prec != PREC_NONE) ? F(X_Z) : F(F_R); }
-    else { die("Invalid value '%d' for norm_array passed, should be [0, 1, 2]", *norm_array); return; }
-    if (!launch("calc_2norm", k_dot<double>, interior(), g.pitch, a, b, red_ctx())) return;
-    fetch(norm, 1);
+    if (*norm_array < 0 || *norm_array > 2) {
+        die("Invalid value '%d' for norm_array passed, should be [0, 1, 2]", *norm_array);
+        return;
+    }
+    const int u0 = X_U0, r = F_R, z = g.prec != PREC_NONE ? X_Z : F_R;
+    const int ia = *norm_array == 0 ? u0 : r, ib = *norm_array == 0 ? u0 : (*norm_array == 1 ? r : z);
+    bool ok;
+    if (g.fp32)   // the FP32 solver's fields (its right-hand side, residual, z)
+        ok = launch("calc_2norm_f32", k_dot<float>, interior(), g.pitch32, (const float *)g.fs32[ia],
+                    (const float *)g.fs32[ib], red_ctx());
+    else
+        ok = launch("calc_2norm", k_dot<double>, interior(), g.pitch, (const double *)F(ia), (const double *)F(ib),
+                    red_ctx());
+    if (ok) fetch(norm, 1);
 }
 
 namespace {
@@ -1569,19 +1578,31 @@ void upload_coefs(const double *a, const double *b, int n)
     g.ch_b.assign(b, b + n);
 }
 
-template <int FIRST>
+template <typename T, int FIRST>
 bool cheby_p(double theta, double alpha, double beta)
 {
-    const char *name = FIRST ? "cheby_init_p" : "cheby_calc_p";
+    const char *name = nm<T>(FIRST ? "cheby_init_p" : "cheby_calc_p");
     const Ext in = interior();
     auto args = [&](auto kernel) {
-        return launch(name, kernel, in, g.pitch, (const double *)F(F_U), (const double *)F(X_U0), F(F_P), F(F_R),
-                      F(X_W), (const double *)F(X_MI), (const double *)F(X_KX), (const double *)F(X_KY), theta,
-                      alpha, beta);
+        return launch(name, kernel, in, pitch_of<T>(), (const T *)FT<T>(F_U), (const T *)FT<T>(X_U0), FT<T>(F_P),
+                      FT<T>(F_R), FT<T>(X_W), (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX),
+                      (const T *)FT<T>(X_KY), theta, alpha, beta);
     };
-    if (g.prec == PREC_BLOCK) return args(k_cheby_p<PREC_BLOCK, FIRST>);
-    if (g.prec == PREC_DIAG) return args(k_cheby_p<PREC_DIAG, FIRST>);
-    return args(k_cheby_p<PREC_NONE, FIRST>);
+    if (g.prec == PREC_BLOCK) return args(k_cheby_p<T, PREC_BLOCK, FIRST>);
+    if (g.prec == PREC_DIAG) return args(k_cheby_p<T, PREC_DIAG, FIRST>);
+    return args(k_cheby_p<T, PREC_NONE, FIRST>);
+}
+
+template <typename T> void cheby_init_impl(double theta)
+{
+    if (!cheby_p<T, 1>(theta, 0.0, 0.0)) return;
+    launch(nm<T>("cheby_calc_u"), k_add<T>, interior(), pitch_of<T>(), FT<T>(F_U), (const T *)FT<T>(F_P));
+}
+
+template <typename T> void cheby_iterate_impl(double alpha, double beta)
+{
+    if (!cheby_p<T, 0>(0.0, alpha, beta)) return;
+    launch(nm<T>("cheby_calc_u"), k_add<T>, interior(), pitch_of<T>(), FT<T>(F_U), (const T *)FT<T>(F_P));
 }
 }  // namespace
 
@@ -1590,24 +1611,20 @@ EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const doub
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
-    REQUIRE_FP64();
     upload_coefs(ch_alphas, ch_betas, *n_coefs);
-    if (!cheby_p<1>(*theta, 0.0, 0.0)) return;
-    launch("cheby_calc_u", k_add, interior(), g.pitch, F(F_U), (const double *)F(F_P));
+    BY_PRECISION(cheby_init_impl, *theta);
 }
 
 EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
 {
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
-    REQUIRE_FP64();
     const int k = *cheby_calc_step - 1;
     if (k < 0 || k >= (int)g.ch_a.size()) {
         die("cheby_iterate: step %d outside the %d uploaded coefficients", k + 1, (int)g.ch_a.size());
         return;
     }
-    if (!cheby_p<0>(0.0, g.ch_a[k], g.ch_b[k])) return;
-    launch("cheby_calc_u", k_add, interior(), g.pitch, F(F_U), (const double *)F(F_P));
+    BY_PRECISION(cheby_iterate_impl, g.ch_a[k], g.ch_b[k]);
 }
 
 EXPORT void tea_leaf_jacobi_solve_kernel_cuda_(double *error)
@@ -1889,7 +1906,7 @@ EXPORT void tea_engine_mp_begin(double *rr)
     CHECK_INIT();
     if (!g.slab32) { die("tea_engine_mp_begin: call tea_engine_set_precision(32) first"); return; }
     if (!launch("mp_begin", k_mp_begin, interior(), g.pitch, g.pitch32, (const double *)F(F_R), g.fs32[F_R],
-                g.fs32[F_U], red_ctx()))
+                g.fs32[X_U0], g.fs32[F_U], red_ctx()))
         return;
     fetch(rr, 1);
 }

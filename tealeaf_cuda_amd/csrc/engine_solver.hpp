@@ -1399,9 +1399,11 @@ __global__ __launch_bounds__(BLOCK) void k_mp_coef(Ext e, long pitch, long pitch
     STRIP_END
 }
 
-// r32 = (float) r64; u32 = 0; sum r32*r32 (compensated FP64).
+// r32 = (float) r64; the FP32 right-hand side u0_32 = r32 too (Chebyshev reads
+// it); u32 = 0; sum r32*r32 (compensated FP64).
 __global__ __launch_bounds__(BLOCK) void k_mp_begin(Ext e, long pitch, long pitch32, const double *__restrict__ r,
-                                                    float *__restrict__ r32, float *__restrict__ u32, Red red)
+                                                    float *__restrict__ r32, float *__restrict__ u0_32,
+                                                    float *__restrict__ u32, Red red)
 {
     DD acc[1] = {{0, 0}};
     STRIP_BEGIN
@@ -1410,6 +1412,7 @@ __global__ __launch_bounds__(BLOCK) void k_mp_begin(Ext e, long pitch, long pitc
         const long j = (long)s.c + (long)(s.r0 + k) * pitch32;
         const float rv = (float)r[i];
         r32[j] = rv;
+        u0_32[j] = rv;
         u32[j] = 0.0f;
         dd_add_prod(acc[0], (double)rv, (double)rv);
     }

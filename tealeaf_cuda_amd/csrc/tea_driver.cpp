@@ -371,8 +371,8 @@ Deck read_deck(const std::string &path)
         float c = (float)(d.x_cells * d.y_cells);
         d.tl_ppcg_inner_steps = 4 * (int)std::sqrt(std::sqrt(c));
     }
-    if (d.refine_bits && !(d.tl_use_cg || d.tl_use_ppcg))
-        fail("read_input: tl_inner_precision needs tl_use_cg or tl_use_ppcg");
+    if (d.refine_bits && !(d.tl_use_cg || d.tl_use_ppcg || d.tl_use_chebyshev))
+        fail("read_input: tl_inner_precision needs tl_use_cg, tl_use_ppcg or tl_use_chebyshev");
     if (d.refine_bits && d.reflective_boundary)
         fail("read_input: tl_inner_precision does not support reflective_boundary");
     if (!d.warm_start.empty() && !(d.tl_use_cg || d.tl_use_ppcg))
@@ -820,7 +820,8 @@ void Run::tea_leaf()
             fortran_e(std::sqrt(std::fabs(initial_residual)) / std::sqrt(std::fabs(target_residual)), 14, 7).c_str());
     int refinements = 0, iterations = 0, ppcg_inner_iters = 0;
     Solve sol;
-    if (d.refine_bits && d.restart) {
+    if (d.refine_bits && (d.restart || d.tl_use_chebyshev)) {
+        // Chebyshev has no recurrence residual to replace: it always refines by restarts
         sol = refine(initial_residual, target_residual, &refinements, &iterations, &ppcg_inner_iters);
     } else if (d.refine_bits) {
         sol = refine_reliable(initial_residual, target_residual, &refinements, &iterations, &ppcg_inner_iters);
@@ -1087,22 +1088,27 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
     return Solve{n, error, initial_residual, ppcg_inner_iters};
 }
 
-// Mixed-precision iterative refinement (no reference counterpart): the FP64
+// Iterative refinement by restarts (no reference counterpart): the FP64
 // residual r = u0 - A u is measured on the device; while
 // ||r|| > tl_eps * ||r0|| the correction equation A d = r is solved by the
-// same CG/PPCG loop on FP32 fields (FP64 dot products and host numerics) to
-// tl_inner_eps relative, and u += d in FP64.  The eigenvalue estimate comes
-// from the first inner solve's CG pre-steps and is kept across solves.
+// same CG/PPCG/Chebyshev loop to tl_inner_eps relative, and u += d in FP64.
+// With tl_inner_precision fp32 the inner solves run on FP32 fields (FP64 dot
+// products and host numerics; d starts at 0); with fp64 they run on the FP64
+// fields and update u in place.  The eigenvalue estimate comes from the first
+// inner solve's CG pre-steps and is kept across solves.
 Run::Solve Run::refine(double initial_residual_sq, double target_residual_sq, int *refinements, int *iterations,
                        int *inner)
 {
-    if (!b.set_precision || !b.mp_setup || !b.mp_begin || !b.mp_end)
+    const bool fp32 = d.refine_bits == 32;
+    if (fp32 && (!b.set_precision || !b.mp_setup || !b.mp_begin || !b.mp_end))
         fail("backend has no mixed-precision entry points (tl_inner_precision fp32)");
     const double ir = std::sqrt(std::fabs(target_residual_sq));
-    if (b.set_precision(32) != 0) b.check();
-    b.set_precision(64);
-    b.mp_setup();
-    b.check();
+    if (fp32) {
+        if (b.set_precision(32) != 0) b.check();
+        b.set_precision(64);
+        b.mp_setup();
+        b.check();
+    }
     double rr = initial_residual_sq;
     double err = std::sqrt(std::fabs(rr));
     *refinements = 0;
@@ -1119,14 +1125,19 @@ Run::Solve Run::refine(double initial_residual_sq, double target_residual_sq, in
         }
         if (d.verbose_on) log(" Refinement %d true residual %s\n", k, fortran_e(err / ir, 14, 7).c_str());
         if (err < d.eps * ir) break;
-        double rr32 = 0.0;
-        b.set_precision(32);
-        b.mp_begin(&rr32);
-        b.check();
-        Solve s = iterate(rr32, d.inner_eps);
-        b.mp_end();
-        b.check();
-        b.set_precision(64);
+        Solve s;
+        if (fp32) {
+            double rr32 = 0.0;
+            b.set_precision(32);
+            b.mp_begin(&rr32);
+            b.check();
+            s = iterate(rr32, d.inner_eps);
+            b.mp_end();
+            b.check();
+            b.set_precision(64);
+        } else {
+            s = iterate(rr, d.inner_eps);
+        }
         *refinements += 1;
         *iterations += s.n - 1;
         *inner += s.ppcg_inner;

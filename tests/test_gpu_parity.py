@@ -64,6 +64,10 @@ DECKS = {
                                   mixed_refinement="restart", inner_eps=1e-4),
     "refined_fp64_ppcg_block": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
                                     inner_precision="fp64"),
+    "mixed_cheby_block": dict(solver="chebyshev", preconditioner="jac_block", ch_cg_presteps=20,
+                              inner_precision="fp32", inner_eps=1e-5),
+    "mixed_cheby_diag": dict(solver="chebyshev", preconditioner="jac_diag", ch_cg_presteps=20,
+                             inner_precision="fp32", inner_eps=1e-4),
     "mixed_random_layout": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
                                 states=random_layout(12, cells=128), x_cells=128, y_cells=128,
                                 inner_precision="fp32"),
@@ -305,6 +309,36 @@ def test_fp32_entry_points_bitwise(prec, oracle_so, oracle_env, tmp_path):
         assert outs[0] == outs[1]
         _compare(g, c, "mp_end")
         assert np.array_equal(g.get_field32(rt.FIELD_R), c.get_field32(rt.FIELD_R))
+    finally:
+        g.finalize()
+        c.finalize()
+
+
+@pytest.mark.parametrize("prec", ["none", "jac_diag", "jac_block"])
+def test_fp32_cheby_entry_points_bitwise(prec, oracle_so, oracle_env, tmp_path):
+    """FP32 mode, Chebyshev: cheby_init / cheby_iterate and the 2-norms of the
+    FP32 solver's fields on random float fields, compared bit for bit."""
+    oracle_env("dot2")
+    g, c = Engine(), Engine(oracle_so)
+    calls = dict(CHEBY_CALLS)
+    calls.update({k: CALLS[k] for k in ("calc_2norm_0", "calc_2norm_1", "calc_2norm_2")})
+    try:
+        for k, (name, fn) in enumerate(sorted(calls.items())):
+            print(f"[gpu parity] fp32 chebyshev entry point {prec}/{name}", flush=True)
+            for e in (g, c):
+                _prepare(e, tmp_path, "chebyshev", prec)
+                e.set_precision(32)
+                e.mp_setup()
+            _randomise32((g, c), seed=300 + k)
+            for e in (g, c):
+                e.cg_init()
+            _randomise32((g, c), seed=400 + k)
+            out_g, out_c = fn(g), fn(c)
+            assert out_g == out_c, name
+            for fid in range(1, 20):
+                if fid in (rt.XFIELD_CP, rt.XFIELD_BFP):
+                    continue  # FP32 mode keeps no Thomas factors: the solves recompute them in registers
+                assert np.array_equal(g.get_field32(fid), c.get_field32(fid), equal_nan=True), f"{name}: {fid}"
     finally:
         g.finalize()
         c.finalize()

@@ -2,14 +2,16 @@
 
 The refinement loop has no counterpart in the reference; what pins it:
 
-* the FP32 instantiation of every CG/PPCG kernel equals the FP64 one exactly
+* the FP32 instantiation of every CG/PPCG/Chebyshev kernel equals the FP64 one exactly
   on inputs where float arithmetic is exact (small integers and dyadic
   scalars), so the FP32 kernels compute the same expressions as the FP64
   kernels that the reference's golden files pin; with the jac_diag / jac_block
   preconditioners (divisions) they agree to float rounding;
-* mp_begin / mp_end are the roundings r32 = fl32(r) and u += u32 exactly;
-* both refinement schemes (restart: FP32 inner solves to tl_inner_eps;
-  reliable: residual replacement keeping the search direction) reach the FP64
+* mp_begin / mp_end are the roundings r32 = fl32(r) (also the FP32 right-hand
+  side u0_32, which Chebyshev reads) and u += u32 exactly;
+* both refinement schemes (restart: FP32 inner solves to tl_inner_eps, the
+  scheme Chebyshev always uses; reliable: residual replacement keeping the
+  search direction) reach the FP64
   true relative residual tl_eps on every step
   (tl_check_result measures ||u0 - A u|| / ||r0|| in FP64 after the solve), and
   its field summaries agree with a tightly converged FP64 run to the accuracy
@@ -80,19 +82,28 @@ EXACT_CALLS = {
     "ppcg_calc_2norm": lambda e: e.ppcg_calc_2norm(),
     "ppcg_calc_p": lambda e: e.ppcg_calc_p(0.5),
 }
+EXACT_CHEBY_CALLS = {
+    "cheby_init": lambda e: e.cheby_init([0.5, 0.25], [0.75, 0.5], 2.0),
+    "cheby_iterate": lambda e: (e.cheby_init([0.5, 0.25], [0.75, 0.5], 2.0), e.cheby_iterate(2)),
+    "calc_2norm_0": lambda e: e.calc_2norm(0),
+    "calc_2norm_1": lambda e: e.calc_2norm(1),
+    "calc_2norm_2": lambda e: e.calc_2norm(2),
+}
 
 
-@pytest.mark.parametrize("name", sorted(EXACT_CALLS))
-def test_fp32_kernels_equal_fp64_on_exact_inputs(name, oracle_so, oracle_env, tmp_path):
+@pytest.mark.parametrize("solver,name", [("ppcg", n) for n in sorted(EXACT_CALLS)]
+                         + [("chebyshev", n) for n in sorted(EXACT_CHEBY_CALLS)])
+def test_fp32_kernels_equal_fp64_on_exact_inputs(solver, name, oracle_so, oracle_env, tmp_path):
     oracle_env("dot2")
+    calls = EXACT_CALLS if solver == "ppcg" else EXACT_CHEBY_CALLS
     eng = Engine(oracle_so)
     try:
-        vals = _integer_chunk(eng, tmp_path, "ppcg", "none", seed=len(name))
+        vals = _integer_chunk(eng, tmp_path, solver, "none", seed=len(name))
         outs = {}
         fields = {}
         for bits in (64, 32):
             _load(eng, vals, bits)
-            outs[bits] = EXACT_CALLS[name](eng)
+            outs[bits] = calls[name](eng)
             fields[bits] = {fid: _read(eng, fid, bits) for fid in SOLVE_FIELDS}
         assert outs[32] == outs[64]
         for fid in SOLVE_FIELDS:
@@ -132,6 +143,7 @@ def test_mp_begin_end_are_roundings(oracle_so, oracle_env, tmp_path):
         rr = eng.mp_begin()
         r32 = interior(eng, eng.get_field32(rt.FIELD_R))
         assert np.array_equal(r32, interior(eng, r).astype(np.float32))
+        assert np.array_equal(interior(eng, eng.get_field32(rt.XFIELD_U0)), r32)
         assert not interior(eng, eng.get_field32(rt.FIELD_U)).any()
         x = r32.astype(np.float64).ravel()
         assert rr == pytest.approx(float(np.dot(x, x)), rel=1e-15)
@@ -161,13 +173,16 @@ MIXED_CASES = [
     ("ppcg", "none", dict(reliable_delta=0.01)),
     ("cg", "jac_diag", dict(reliable_delta=0.3)),
     ("cg", "jac_block", dict(mixed_refinement="reliable")),
+    ("chebyshev", "jac_block", dict(inner_eps=1e-5)),
+    ("chebyshev", "jac_diag", dict(inner_eps=1e-4)),
+    ("chebyshev", "none", dict(mixed_refinement="reliable", inner_eps=1e-5)),   # restarts regardless
 ]
 
 
 @pytest.mark.parametrize("solver,prec,kw", MIXED_CASES)
 def test_mixed_reaches_fp64_true_residual(solver, prec, kw, oracle_so, oracle_env, tmp_path):
     oracle_env("dot2")
-    inner = dict(ppcg_inner_steps=10) if solver == "ppcg" else {}
+    inner = dict(ppcg_inner_steps=10) if solver == "ppcg" else (dict(ch_cg_presteps=20) if solver == "chebyshev" else {})
     mixed = _run(tmp_path, oracle_so, "mixed", solver=solver, preconditioner=prec, inner_precision="fp32",
                  **kw, **inner)
     ref = _run(tmp_path, oracle_so, "fp64", solver=solver, preconditioner=prec, eps=1e-14, **inner)
@@ -180,6 +195,23 @@ def test_mixed_reaches_fp64_true_residual(solver, prec, kw, oracle_so, oracle_en
         np.testing.assert_allclose(a.values(), b.values(), rtol=1e-9)
     text = (tmp_path / "mixed.out").read_text()
     assert "FP32 refinements" in text
+
+
+@pytest.mark.parametrize("solver", ["cg", "ppcg", "chebyshev"])
+def test_fp64_restart_is_the_fp64_solve(solver, oracle_so, oracle_env, tmp_path):
+    """tl_inner_precision fp64 with restarts runs its inner solves on the FP64
+    fields: with tl_inner_eps equal to a plain FP64 run's tl_eps, its first
+    inner solve is that run, bit for bit, and already meets tl_eps, so it is
+    the only one."""
+    oracle_env("dot2")
+    extra = dict(ppcg_inner_steps=10) if solver == "ppcg" else (dict(ch_cg_presteps=20) if solver == "chebyshev" else {})
+    plain = _run(tmp_path, oracle_so, "plain", solver=solver, preconditioner="jac_block", eps=1e-13, **extra)
+    rst = _run(tmp_path, oracle_so, "rst", solver=solver, preconditioner="jac_block", inner_precision="fp64",
+               mixed_refinement="restart", inner_eps=1e-13, eps=1e-10, **extra)
+    assert rst.refinements == [1] * len(plain.iterations)
+    assert rst.iterations == plain.iterations
+    for a, b in zip(rst.summaries, plain.summaries):
+        assert a.values() == b.values()
 
 
 def test_reliable_keeps_the_krylov_process(oracle_so, oracle_env, tmp_path):
@@ -200,7 +232,6 @@ def test_reliable_keeps_the_krylov_process(oracle_so, oracle_env, tmp_path):
 
 
 @pytest.mark.parametrize("kw,msg", [
-    (dict(solver="chebyshev", preconditioner="none"), "tl_inner_precision"),
     (dict(solver="jacobi", preconditioner="none"), "tl_inner_precision"),
     (dict(solver="ppcg", preconditioner="none", reflective_boundary=True), "reflective"),
 ])
@@ -209,13 +240,11 @@ def test_mixed_rejects_unsupported_decks(kw, msg, oracle_so, tmp_path):
         _run(tmp_path, oracle_so, "bad", inner_precision="fp32", end_step=1, **kw)
 
 
-def test_fp32_mode_rejects_chebyshev_and_jacobi_calls(oracle_so, tmp_path):
+def test_fp32_mode_rejects_jacobi_calls(oracle_so, tmp_path):
     eng = Engine(oracle_so)
     try:
-        setup_chunk(eng, tmp_path, solver="chebyshev")
+        setup_chunk(eng, tmp_path, solver="jacobi")
         eng.set_precision(32)
-        with pytest.raises(TeaError):
-            eng.cheby_init([0.2], [0.9], 1.3)
         with pytest.raises(TeaError):
             eng.jacobi_solve()
         with pytest.raises(TeaError):

@@ -1283,8 +1283,8 @@ constexpr int INNER_WAVES = 16;
 // MI355X (block Jacobi; the ratios hold at 8192^2), with the kernel
 // use_march picks: the plan of passes for a run of inner steps minimises
 // their sum.
-constexpr double PASS_US_F32[MAX_PASS_STEPS] = {125, 140, 158, 177, 310};
-constexpr double PASS_US_F64[MAX_PASS_STEPS] = {223, 266, 319, 361, 739};
+constexpr double PASS_US_F32[MAX_PASS_STEPS] = {131, 137, 161, 169, 204};
+constexpr double PASS_US_F64[MAX_PASS_STEPS] = {233, 240, 284, 335, 452};
 
 // The streaming pass (k_ppcg_inner_march) is the faster kernel for every NS
 // in both precisions (measured against k_ppcg_inner_n[_x2]); it addresses
@@ -1308,15 +1308,16 @@ int cu_count()
     return v;
 }
 
-// One wave per band of columns and chunk of strips: enough waves for the
-// resident waves per SIMD the kernel's registers allow (two up to four FP32
-// steps and for one FP64 step, else one), chunks as long as that leaves.
+// One wave per band of columns and chunk of strips: two waves per SIMD in
+// FP32 (its kernels fit 256 registers up to five steps), one in FP64 (the
+// longer chunks that leaves measured faster than two waves per SIMD even
+// where the registers would allow two), chunks as long as that leaves.
 template <typename T, int NS> void launch_inner_march(const Ext &e, long pt, const StepCoefs &cf, const char *name)
 {
-    constexpr int OX = WAVE - 2 * (NS - 1);
+    constexpr int OX = march_out_x(NS);
     const int nbands = (e.x1 - e.x0 + OX) / OX, nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
     if (nbands < 1 || nstrips < 1) return;
-    const int per_simd = sizeof(T) == 4 ? (NS <= 4 ? 2 : 1) : (NS == 1 ? 2 : 1);
+    const int per_simd = sizeof(T) == 4 ? 2 : 1;
     int nchunks = cu_count() * 4 * per_simd / nbands;
     nchunks = nchunks < 1 ? 1 : (nchunks > nstrips ? nstrips : nchunks);
     const int chunk = (nstrips + nchunks - 1) / nchunks;

@@ -825,19 +825,22 @@ template <typename T> struct FieldBuf {
 };
 
 // One block-Jacobi strip (4 rows of one column) as read from memory: the
-// fields, the coefficients, and for the wave's edge lanes the neighbouring
-// column's sd (lane 0: left, last lane: right) and the right-face Kx.
+// fields, the coefficients, and for lane 0 the left neighbour column's sd.
 template <typename T> struct MarchRaw {
-    T sd[STRIP], rt[STRIP], ut[STRIP], kx[STRIP], ky[STRIP + 1], mi[STRIP], kxe[STRIP], sde[STRIP];
+    T sd[STRIP], rt[STRIP], ut[STRIP], kx[STRIP], ky[STRIP + 1], mi[STRIP], sde[STRIP];
 };
 // A strip inside the pipeline: what it carries from step to step and the
-// coefficients every step uses (the diagonal is re-formed per step).
+// coefficients every step uses (the right-face Kx, the next lane's Kx, and
+// the diagonal are re-formed per step); len: its rows inside the extent.
 template <typename T> struct MarchStrip {
-    T sd[STRIP], rt[STRIP], ut[STRIP], kx[STRIP], kxr[STRIP], ky[STRIP + 1], cp[STRIP], bfp[STRIP], mi[STRIP];
-    int r0, act;
+    T sd[STRIP], rt[STRIP], ut[STRIP], kx[STRIP], ky[STRIP + 1], cp[STRIP], bfp[STRIP], mi[STRIP];
+    int r0, len;
 };
 
 constexpr int MARCH_WAVES = 4;   // independent waves per workgroup
+// Columns a streaming pass of ns steps writes per band of WAVE lanes (ns - 1
+// halo lanes at the left, ns at the right).
+constexpr int march_out_x(int ns) { return WAVE - 2 * ns + 1; }
 
 // Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, VALU moves with
 // no LDS round trip): lane l receives v from lane l-1 (from_below) or l+1
@@ -862,16 +865,19 @@ __device__ __forceinline__ double lane_from_above(double v)
 // NS = 1..5 PPCG inner steps in one streaming pass (temporal blocking; no
 // reference counterpart): the composition of NS k_ppcg_inner_fused<T>
 // launches with coefficients (cf.a[k], cf.b[k]), bit-identical to them.
-// Each wave owns a band of WAVE columns (one per lane; NS-1 halo lanes at each
-// side, so WAVE - 2(NS-1) columns are written) and a chunk of `chunk` strips,
+// Each wave owns a band of WAVE columns (one per lane; NS-1 halo lanes at the
+// left and NS at the right, so WAVE - 2NS + 1 columns are written) and a
+// chunk of `chunk` strips,
 // and walks down it one strip per iteration: at iteration i strip i enters
 // (read one iteration ahead) and strip i-1-m moves from step m to step m+1
 // for m = 0..NS-1, so strip i-NS leaves the pipeline finished.  Step m+1 of a
 // strip needs step m of the strip itself and of the rows just below and above
 // it (the previous strip's last row, kept from the iteration before, and the
 // next strip's first row, formed earlier in this iteration); columns come from
-// lane shifts (memory at the edge lanes in the first step, stale values there
-// afterwards, which only reach halo lanes).  The strips in the pipeline move
+// lane shifts (memory at lane 0 in the first step, stale values at the edge
+// lanes afterwards, which only reach halo lanes; the last lane has no right
+// neighbour's Kx, so its values are stale from the first step, hence one more
+// halo lane at the right).  The strips in the pipeline move
 // down one place per iteration (register moves: a ring of slots unrolled NS+1
 // times needs more registers than the 256 that keep two waves per SIMD).
 // Reads and writes go through bounded buffer resources, so the guards of the
@@ -888,7 +894,8 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     const T *__restrict__ Ky, StepCoefs cf)
 {
     static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
-    constexpr int HL = NS - 1, OUT_X = WAVE - 2 * HL;
+    constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS);
+    static_assert(OUT_X == WAVE - HL - HR, "band layout");
     const int lane = threadIdx.x & (WAVE - 1);
     if (e.x1 < e.x0 || e.y1 < e.y0) return;
     const int nbands = (e.x1 - e.x0 + OUT_X) / OUT_X;
@@ -902,9 +909,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     const int c = e.x0 + band * OUT_X - HL + lane;
     const bool col_in = c >= e.x0 && c <= e.x1;
     const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
-    const bool out_lane = lane >= HL && lane <= WAVE - 1 - HL;
-    const bool right_lane = lane < WAVE - 1;
-    const bool edge_kx = !right_lane || c + 1 > e.x1;   // right-face Kx not in the next lane
+    const bool out_lane = lane >= HL && lane <= WAVE - 1 - HR;
     // Fields are read and written through buffer resources over their rows
     // 0 .. y1+1 (the host launches this kernel only for fields under 4 GiB):
     // an access the reference semantics exclude gets the offset OOB, which
@@ -917,14 +922,13 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     const int s_first = ck * chunk;
     const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
 
-    auto strip_act = [&](int r0) {
+    auto strip_len = [&](int r0) {   // rows of the strip inside the extent (uniform across the wave)
         int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
-        len = len < 0 ? 0 : (len > STRIP ? STRIP : len);
-        return col_in ? len : 0;
+        return len < 0 ? 0 : (len > STRIP ? STRIP : len);
     };
     // Issues the reads of strip s.
     auto fetch = [&](int s, MarchRaw<T> &m) {
-        const int r0 = e.y0 + s * STRIP, act = strip_act(r0);
+        const int r0 = e.y0 + s * STRIP, len = strip_len(r0), act = col_in ? len : 0;
         const unsigned b0 = off(r0);
         constexpr unsigned W = sizeof(T);
 #pragma unroll
@@ -938,18 +942,18 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
         for (int k = 0; k < STRIP; k++) {
             const bool a = k < act;
             const unsigned bk = b0 + k * row_b;
-            m.kx[k] = b_kx.load(a ? bk : OOB);
+            m.kx[k] = b_kx.load(col_mem && k < len ? bk : OOB);   // column x1+1 too: the right face of x1
             m.rt[k] = b_rt.load(a ? bk : OOB);
             m.ut[k] = b_ut.load(a && out_lane ? bk : OOB);
             m.mi[k] = PREC == PREC_DIAG ? b_mi.load(a ? bk : OOB) : T(0.0);
-            m.kxe[k] = b_kx.load(a && edge_kx ? bk + W : OOB);
-            m.sde[k] = b_sd.load(a && lane == 0 ? bk - W : (a && !right_lane ? bk + W : OOB));
+            m.sde[k] = b_sd.load(a && lane == 0 ? bk - W : OOB);
         }
     };
     // Forms the pipeline strip of strip s from its reads.
     auto prepare = [&](int s, const MarchRaw<T> &raw, MarchStrip<T> &m) {
         m.r0 = e.y0 + s * STRIP;
-        m.act = strip_act(m.r0);
+        m.len = strip_len(m.r0);
+        const int act = col_in ? m.len : 0;
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
             m.sd[k] = raw.sd[k];
@@ -963,33 +967,31 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
         T diag[STRIP];
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
-            const T sh = lane_from_above(m.kx[k]);
-            m.kxr[k] = T(0.0);
-            if (k < m.act) m.kxr[k] = edge_kx ? raw.kxe[k] : sh;
-            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (m.kxr[k] + m.kx[k]);
+            const T kxr = lane_from_above(m.kx[k]);   // Kx on the right face: the next lane's
+            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (kxr + m.kx[k]);
         }
-        if (PREC == PREC_BLOCK && m.act > 0) strip_factor<T>(m.act, diag, m.ky, m.cp, m.bfp);
+        if (PREC == PREC_BLOCK && act > 0) strip_factor<T>(act, diag, m.ky, m.cp, m.bfp);
     };
 
     // Moves strip m from step st to st+1 (lo, hi: sd of the rows below and
-    // above it at step st; sde: the edge lanes' neighbour column at step 0).
+    // above it at step st; sde: lane 0's left neighbour column at step 0).
     auto advance = [&](MarchStrip<T> &m, T lo, T hi, const T (&sde)[STRIP], int st) {
         const T a = (T)cf.a[st], b = (T)cf.b[st];
-        const int ak = (st < NS - 1 || out_lane) ? m.act : 0;
-        T rv[STRIP], zv[STRIP], diag[STRIP];
+        const int ak = (st < NS - 1 || out_lane) && col_in ? m.len : 0;
+        T rv[STRIP], zv[STRIP], diag[STRIP], kxr[STRIP];
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
             const T from_left = lane_from_below(m.sd[k]);
             const T from_right = lane_from_above(m.sd[k]);
-            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (m.kxr[k] + m.kx[k]);
+            kxr[k] = lane_from_above(m.kx[k]);
+            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (kxr[k] + m.kx[k]);
             rv[k] = zv[k] = T(0.0);
             if (k < ak) {
                 const T below = k == 0 ? lo : m.sd[k - 1];
                 const T above = k == STRIP - 1 ? hi : m.sd[k + 1];
                 const T left = (st > 0 || lane > 0) ? from_left : sde[k];
-                const T right = (st > 0 || right_lane) ? from_right : sde[k];
                 const T res = diag[k] * m.sd[k] - (m.ky[k + 1] * above + m.ky[k] * below)
-                              - (m.kxr[k] * right + m.kx[k] * left);
+                              - (kxr[k] * from_right + m.kx[k] * left);
                 rv[k] = m.rt[k] - res;
             }
         }
@@ -1014,7 +1016,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
 #pragma unroll
     for (int m = 0; m < NS; m++) {
         H[m] = MarchStrip<T>{};
-        H[m].act = 0;
+        H[m].len = 0;
         H[m].r0 = e.y0;
         P[m] = T(0.0);
     }
@@ -1043,7 +1045,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
         const unsigned b0 = off(f.r0);
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
-            const unsigned bk = out && k < f.act ? b0 + k * row_b : OOB;
+            const unsigned bk = out && col_in && k < f.len ? b0 + k * row_b : OOB;
             b_sdo.store(bk, f.sd[k]);
             b_rto.store(bk, f.rt[k]);
             b_ut.store(bk, f.ut[k]);

@@ -1283,8 +1283,8 @@ constexpr int INNER_WAVES = 16;
 // MI355X (block Jacobi; the ratios hold at 8192^2), with the kernel
 // use_march picks: the plan of passes for a run of inner steps minimises
 // their sum.
-constexpr double PASS_US_F32[MAX_PASS_STEPS] = {131, 137, 161, 169, 204};
-constexpr double PASS_US_F64[MAX_PASS_STEPS] = {233, 240, 284, 335, 452};
+constexpr double PASS_US_F32[MAX_PASS_STEPS] = {132, 140, 158, 166, 198};
+constexpr double PASS_US_F64[MAX_PASS_STEPS] = {230, 237, 269, 322, 438};
 
 // The streaming pass (k_ppcg_inner_march) is the faster kernel for every NS
 // in both precisions (measured against k_ppcg_inner_n[_x2]); it addresses

@@ -844,9 +844,11 @@ constexpr int march_out_x(int ns) { return WAVE - 2 * ns + 1; }
 
 // Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, VALU moves with
 // no LDS round trip): lane l receives v from lane l-1 (from_below) or l+1
-// (from_above); the end lane keeps its own value, as __shfl_up/__shfl_down.
-__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+// (from_above); the end lane (lane 0, resp. the last lane), which has no
+// source, receives 0 (bound_ctrl), so no copy of v has to be made first.
+// The streaming pass never reads those end values.
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
 __device__ __forceinline__ float lane_from_below(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
 __device__ __forceinline__ float lane_from_above(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
 __device__ __forceinline__ double lane_from_below(double v)

@@ -145,6 +145,10 @@ struct Chunk {
     double *h_out = nullptr;         // pinned: the finished values
     double *d_out = nullptr;         // device alias of h_out
     double *d_scal = nullptr;        // finished values kept in device memory for a following launch
+    // rtemp = r and utemp = sd owed by tea_engine_ppcg_begin_ur (0: none, else
+    // the field precision in bits): the next streaming inner pass reads r and
+    // sd in their place; any other entry point writes them out first.
+    int pending_rt_ut = 0;
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -214,8 +218,11 @@ bool die(const char *fmt, ...)
         hipError_t e_ = (call);                                                                   \
         if (e_ != hipSuccess) { die("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); return ret; } \
     } while (0)
-#define CHECK_INIT()                                                                              \
+void settle_pending();
+#define CHECK_INIT_KEEP_PENDING()                                                                 \
     do { if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return; } } while (0)
+#define CHECK_INIT()                                                                              \
+    do { CHECK_INIT_KEEP_PENDING(); settle_pending(); } while (0)
 // Kernels are registered per solver in the reference (init_cuda.cu:251-282) and
 // launching an unregistered one throws there; the same calls are errors here.
 #define REQUIRE_SOLVER(mask)                                                                      \
@@ -1085,11 +1092,13 @@ void pack_unpack(int *fields, int *offsets, int d, int face, int pack, double *h
 
 EXPORT void cuda_pack_buffers_(int *fields, int *offsets, int *depth, int *face, double *host_buffer)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     pack_unpack(fields, offsets, *depth, *face, 1, host_buffer);
 }
 
 EXPORT void cuda_unpack_buffers_(int *fields, int *offsets, int *depth, int *face, double *host_buffer)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     pack_unpack(fields, offsets, *depth, *face, 0, host_buffer);
 }
 
@@ -1171,6 +1180,23 @@ template <typename T> const char *nm(const char *base)
     auto it = names.find(base);
     if (it == names.end()) it = names.emplace(base, std::string(base) + "_f32").first;
     return it->second.c_str();
+}
+
+// Writes out rtemp = r and utemp = sd owed by tea_engine_ppcg_begin_ur (on the
+// interior, as ppcg_init_sd_new writes them).
+template <typename T> void settle_fields()
+{
+    const Ext in = interior();
+    launch(nm<T>("settle_rtemp"), k_copy<T>, in, pitch_of<T>(), FT<T>(X_RTEMP), (const T *)FT<T>(F_R));
+    launch(nm<T>("settle_utemp"), k_copy<T>, in, pitch_of<T>(), FT<T>(X_UTEMP), (const T *)FT<T>(F_SD));
+}
+
+void settle_pending()
+{
+    const int bits = g.pending_rt_ut;
+    g.pending_rt_ut = 0;
+    if (bits == 32) settle_fields<float>();
+    else if (bits == 64) settle_fields<double>();
 }
 
 #define PREC_SWITCH_T(T, PREC_VALUE, NAME, KERNEL, ...)                                           \
@@ -1284,16 +1310,21 @@ constexpr int INNER_WAVES = 16;
 // use_march picks: the plan of passes for a run of inner steps minimises
 // their sum.
 constexpr double PASS_US_F32[MAX_PASS_STEPS] = {132, 140, 158, 166, 198};
-constexpr double PASS_US_F64[MAX_PASS_STEPS] = {230, 237, 269, 322, 438};
+constexpr double PASS_US_F64[MAX_PASS_STEPS] = {230, 237, 269, 322, 766};
 
 // The streaming pass (k_ppcg_inner_march) is the faster kernel for every NS
 // in both precisions (measured against k_ppcg_inner_n[_x2]); it addresses
-// cells with 32-bit byte offsets, so fields must be under 4 GiB.
+// cells with 32-bit byte offsets, so fields must be under 4 GiB.  Five FP64
+// steps per pass, whose pipeline spills most of its registers into AGPRs,
+// run on the tile kernel: with separate utemp read and write resources and
+// utemp read only for the wave's own strips that variant gave results that
+// changed from run to run, and no plan uses it (ten FP64 steps go as 4+3+3).
 // TEA_INNER_MARCH=0 selects the tile kernels.
 template <typename T, int NS> bool use_march()
 {
     const char *s = getenv("TEA_INNER_MARCH");
     if (s && s[0] == '0') return false;
+    if (sizeof(T) == 8 && NS == 5) return false;
     return (double)pitch_of<T>() * (double)g.rows * sizeof(T) < 4294967296.0 - 65536.0;
 }
 
@@ -1323,9 +1354,13 @@ template <typename T, int NS> void launch_inner_march(const Ext &e, long pt, con
     const int chunk = (nstrips + nchunks - 1) / nchunks;
     nchunks = (nstrips + chunk - 1) / chunk;
     const unsigned grid = (unsigned)((nbands * nchunks + MARCH_WAVES - 1) / MARCH_WAVES);
+    // rtemp = r and utemp = sd still owed by tea_engine_ppcg_begin_ur: read r and sd instead
+    const bool owed = g.pending_rt_ut == 8 * (int)sizeof(T);
+    const T *rt_in = owed ? FT<T>(F_R) : FT<T>(X_RTEMP), *ut_in = owed ? FT<T>(F_SD) : FT<T>(X_UTEMP);
+    g.pending_rt_ut = 0;
     auto go = [&](auto kernel) {
         klaunch(name, kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk, (const T *)FT<T>(F_SD),
-                FT<T>(X_SD2), (const T *)FT<T>(X_RTEMP), FT<T>(X_RT2), FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
+                FT<T>(X_SD2), rt_in, FT<T>(X_RT2), ut_in, FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
                 (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
     };
     if (g.prec == PREC_BLOCK) go(k_ppcg_inner_march<T, PREC_BLOCK, NS>);
@@ -1361,6 +1396,7 @@ template <typename T, int NS> void launch_inner_n(const Ext &e, long pt, const S
         launch_inner_march<T, NS>(e, pt, cf, nm<T>(names[NS - 1]));
         return;
     }
+    settle_pending();
     const long ntiles = (long)((e.x1 - e.x0 + Tile::OUT_X) / Tile::OUT_X) * ((e.y1 - e.y0 + Tile::OUT_Y) / Tile::OUT_Y);
     const unsigned grid = (unsigned)std::min<long>(ntiles, grid_max());
     auto go = [&](auto kernel) {
@@ -1494,15 +1530,18 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
 }
 
 // pw_rro: null (alpha given) or device memory holding p.w, r.z.
-template <typename T> bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta)
+// defer: leave rtemp = r and utemp = sd to the next streaming inner pass
+// (recorded in g.pending_rt_ut).
+template <typename T> bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer = false)
 {
+    T *rt = defer ? nullptr : FT<T>(X_RTEMP), *ut = defer ? nullptr : FT<T>(X_UTEMP);
     bool ok;
     if constexpr (std::is_same<T, float>::value) {
         auto go = [&](auto kernel) {
             return launch_x2("k_ppcg_ur_sd_f32", kernel, interior(), pitch_of<T>(), alpha, pw_rro, theta, FT<T>(F_U),
                              (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
                              (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD),
-                             FT<T>(X_RTEMP), FT<T>(X_UTEMP), red_ctx());
+                             rt, ut, red_ctx());
         };
         ok = g.prec == PREC_BLOCK ? go(k_ppcg_ur_sd_x2<PREC_BLOCK>)
              : g.prec == PREC_DIAG ? go(k_ppcg_ur_sd_x2<PREC_DIAG>)
@@ -1511,8 +1550,9 @@ template <typename T> bool ppcg_ur_launch(double alpha, const double *pw_rro, do
         ok = PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, pw_rro, theta,
                            FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
                            (const T *)FT<T>(X_MI), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD),
-                           FT<T>(X_RTEMP), FT<T>(X_UTEMP), red_ctx());
+                           rt, ut, red_ctx());
     }
+    if (ok && defer) g.pending_rt_ut = 8 * (int)sizeof(T);
     return ok;
 }
 
@@ -1525,7 +1565,7 @@ template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
 template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double theta, double *pw_rro, double *rrn)
 {
     if (!ppcg_begin_launch<T>(with_p, beta) || !finish(0, 2, g.d_scal)) return;
-    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta) || !finish(2, 1, nullptr)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>()) || !finish(2, 1, nullptr)) return;
     if (collect(pw_rro, 0, 2)) *rrn = g.h_out[2];
 }
 
@@ -1656,16 +1696,25 @@ EXPORT void tea_leaf_ppcg_init_kernel_cuda_(const int *step, double *rro)
     BY_PRECISION(ppcg_init_impl, st, rro);
 }
 
+// Both overwrite sd, rtemp and utemp: rtemp and utemp still owed need no writing.
+static void drop_or_settle_pending()
+{
+    if (g.pending_rt_ut == (g.fp32 ? 32 : 64)) g.pending_rt_ut = 0;
+    else settle_pending();
+}
+
 EXPORT void tea_leaf_ppcg_init_sd_kernel_cuda_(const double *theta)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();
+    drop_or_settle_pending();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     BY_PRECISION(ppcg_init_sd_impl, *theta);
 }
 
 EXPORT void tea_leaf_ppcg_init_sd_new_kernel_cuda_(const double *theta)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();
+    drop_or_settle_pending();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     BY_PRECISION(ppcg_init_sd_new_impl, *theta);
 }
@@ -1764,7 +1813,7 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
 // bounds_extra 0 and every face external (one chunk).
 EXPORT void tea_engine_ppcg_inner_n(int step, int nsteps)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();   // a streaming pass takes rtemp and utemp still owed from r and sd
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     const int k = step - 1;
     if (nsteps < 1 || nsteps > MAX_PASS_STEPS) {
@@ -1807,6 +1856,7 @@ EXPORT void tea_engine_info(int *nx, int *ny, int *halo_depth, int *solver, int 
 
 EXPORT int tea_engine_get_field(int id, double *host)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     if (!g.initialised || id < 1 || id >= X_COUNT) return -1;
     const size_t w = (size_t)g.nx + 2 * g.h;
     HIP_OK_R(hipMemcpy2DAsync(host, w * sizeof(double), g.f[id] + (g.lead - g.h), g.pitch * sizeof(double),
@@ -1817,6 +1867,7 @@ EXPORT int tea_engine_get_field(int id, double *host)
 
 EXPORT int tea_engine_set_field(int id, const double *host)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     if (!g.initialised || id < 1 || id >= X_COUNT) return -1;
     if (id == F_SD) g.sd_halo_dirty = true;
     if (id == F_P) g.p_halo_dirty[0] = true;
@@ -1880,6 +1931,7 @@ EXPORT void tea_engine_finalize(void) { release_all(); }
 
 EXPORT int tea_engine_set_precision(int bits)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return -1; }
     if (bits != 32 && bits != 64) { die("tea_engine_set_precision: %d bits is not 32 or 64", bits); return -1; }
     if (bits == 32 && !g.slab32) {
@@ -1921,6 +1973,7 @@ EXPORT void tea_engine_mp_end(void)
 
 EXPORT int tea_engine_get_field32(int id, float *host)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     if (!g.initialised || !g.slab32 || id < 1 || id >= X_COUNT) return -1;
     const size_t w = (size_t)g.nx + 2 * g.h;
     HIP_OK_R(hipMemcpy2DAsync(host, w * sizeof(float), g.fs32[id], g.pitch32 * sizeof(float), w * sizeof(float),
@@ -1931,6 +1984,7 @@ EXPORT int tea_engine_get_field32(int id, float *host)
 
 EXPORT int tea_engine_set_field32(int id, const float *host)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     if (!g.initialised || !g.slab32 || id < 1 || id >= X_COUNT) return -1;
     const size_t w = (size_t)g.nx + 2 * g.h;
     HIP_OK_R(hipMemcpy2DAsync(g.fs32[id], g.pitch32 * sizeof(float), host, w * sizeof(float), w * sizeof(float),
@@ -2084,6 +2138,7 @@ EXPORT void tea_engine_warm_start_coefficients(double *c)
 
 EXPORT int tea_engine_copy_field(int id, double *dst)
 {
+    if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
     if (!g.initialised || id < 1 || id >= X_COUNT || !dst) return -1;
     const double *src = g.fs[id] + (long)g.h + (long)g.h * g.pitch;
     HIP_OK_R(hipMemcpy2DAsync(dst, (size_t)g.nx * sizeof(double), src, (size_t)g.pitch * sizeof(double),

@@ -886,13 +886,15 @@ __device__ __forceinline__ double lane_from_above(double v)
 // reference semantics (rows and columns outside the extent, halo lanes) are
 // offsets, not branches.  The NS strips before the
 // chunk and the NS after it are read and advanced only as far as the chunk's
-// strips need.  No barriers: the waves are independent.  sd and rtemp are
-// read from the _in and written to the _out buffers, utemp in place; cells
+// strips need.  No barriers: the waves are independent.  sd, rtemp and utemp
+// are read from the _in and written to the _out buffers (utemp's may be one
+// buffer: a cell's utemp is written only by the wave that owns the cell, after
+// that wave has read it); cells
 // outside the extent keep their sd.
 template <typename T, int PREC, int NS>
 __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     Ext e, long pitch, int chunk, const T *__restrict__ sd_in, T *__restrict__ sd_out, const T *__restrict__ rt_in,
-    T *__restrict__ rt_out, T *__restrict__ utemp, const T *__restrict__ Mi, const T *__restrict__ Kx,
+    T *__restrict__ rt_out, const T *ut_in, T *ut_out, const T *__restrict__ Mi, const T *__restrict__ Kx,
     const T *__restrict__ Ky, StepCoefs cf)
 {
     static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
@@ -918,8 +920,8 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
     // reads 0 and writes nothing, so no access needs a branch.
     const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
     const unsigned bytes = (unsigned)(e.y1 + 2) * row_b;
-    const FieldBuf<T> b_sd(sd_in, bytes), b_rt(rt_in, bytes), b_ut(utemp, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
-        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes);
+    const FieldBuf<T> b_sd(sd_in, bytes), b_rt(rt_in, bytes), b_ut(ut_in, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
+        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes), b_uto(ut_out, bytes);
     auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
     const int s_first = ck * chunk;
     const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
@@ -1050,7 +1052,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
             const unsigned bk = out && col_in && k < f.len ? b0 + k * row_b : OOB;
             b_sdo.store(bk, f.sd[k]);
             b_rto.store(bk, f.rt[k]);
-            b_ut.store(bk, f.ut[k]);
+            b_uto.store(bk, f.ut[k]);
         }
 #pragma unroll
         for (int m = NS - 1; m > 0; m--) H[m] = H[m - 1];
@@ -1300,13 +1302,17 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2(Ext e, long pitch, doub
                 if (nq == 2) {
                     if (PREC != PREC_NONE) *(float2 *)(z + i) = make_float2(src[0], src[1]);
                     *(float2 *)(sd + i) = make_float2(sdv[0], sdv[1]);
-                    *(float2 *)(rtemp + i) = make_float2(rv[0][k], rv[1][k]);
-                    *(float2 *)(utemp + i) = make_float2(sdv[0], sdv[1]);
+                    if (rtemp) {   // null: rtemp = r and utemp = sd left to the first inner pass
+                        *(float2 *)(rtemp + i) = make_float2(rv[0][k], rv[1][k]);
+                        *(float2 *)(utemp + i) = make_float2(sdv[0], sdv[1]);
+                    }
                 } else {
                     if (PREC != PREC_NONE) z[i] = src[0];
                     sd[i] = sdv[0];
-                    rtemp[i] = rv[0][k];
-                    utemp[i] = sdv[0];
+                    if (rtemp) {
+                        rtemp[i] = rv[0][k];
+                        utemp[i] = sdv[0];
+                    }
                 }
             }
         }
@@ -1358,8 +1364,10 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd(Ext e, long pitch, double 
             }
             const T sdv = src * theta_r;
             sd[i] = sdv;
-            rtemp[i] = rv[k];
-            utemp[i] = sdv;
+            if (rtemp) {   // null: rtemp = r and utemp = sd left to the first inner pass
+                rtemp[i] = rv[k];
+                utemp[i] = sdv;
+            }
         }
     }
     STRIP_END

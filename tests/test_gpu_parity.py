@@ -193,6 +193,14 @@ PPCG_CALLS = {
     **{f"fused_ppcg_inner_n{n}": (lambda n: lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_inner_n(7 - n, n)))(n) for n in range(1, 6)},
+    # begin_ur leaves rtemp = r and utemp = sd to the next streaming pass; with
+    # another call in between they are written out first
+    "fused_ppcg_begin_ur_then_inner_n5": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_begin_ur(True, 0.45, 1.7), e.ppcg_inner_n(1, 5)),
+    "fused_ppcg_begin_ur_then_norm_then_inner_n3": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_begin_ur(False, 0.0, 1.7), e.calc_2norm(1), e.ppcg_inner_n(2, 3)),
     # the same passes through the tile kernels instead of the streaming kernel
     **{f"fused_ppcg_inner_n{n}_tile": (lambda n: _tile_kernels(lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),

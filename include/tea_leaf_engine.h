@@ -233,6 +233,10 @@ void tea_engine_ppcg_begin(int with_p, double beta, double *pw_rro);
 void tea_engine_ppcg_ur(double alpha, double theta, double *rrn);
 void tea_engine_ppcg_end(double *rrn);
 void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, double *pw_rro, double *rrn);
+/* tea_engine_ppcg_end(rrn_end), then tea_engine_ppcg_begin_ur(1, *rrn_end /
+ * rro_prev, theta, pw_rro, rrn) with the division done on the device (one host
+ * wait instead of two); rro_prev is the r.z of the previous begin. */
+void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end, double *pw_rro, double *rrn);
 /* Inner steps `step` .. `step + nsteps - 1` (1-based; nsteps 1..5) in one
  * pass over HBM (temporal blocking): the same results as
  * tea_leaf_ppcg_inner_kernel_cuda_ called for each of the steps with

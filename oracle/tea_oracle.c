@@ -843,6 +843,14 @@ EXPORT void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, doub
     tea_engine_ppcg_ur(pw_rro[1] / pw_rro[0], theta, rrn);
 }
 
+EXPORT void tea_engine_ppcg_end(double *rrn);
+EXPORT void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end, double *pw_rro,
+                                         double *rrn)
+{
+    tea_engine_ppcg_end(rrn_end);
+    tea_engine_ppcg_begin_ur(1, *rrn_end / rro_prev, theta, pw_rro, rrn);
+}
+
 EXPORT void tea_engine_ppcg_end(double *rrn)
 {
     tea_leaf_ppcg_update_z_kernel_cuda_();

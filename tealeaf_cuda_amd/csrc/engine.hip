@@ -1494,7 +1494,9 @@ template <typename T> void ppcg_calc_p_impl(double beta)
 }
 
 // Fused outer iteration (see k_ppcg_begin / k_ppcg_ur_sd / k_ppcg_end).
-template <typename T> bool ppcg_begin_launch(int with_p, double beta)
+// beta_num: null (beta given), or device memory whose value over beta is the
+// beta to apply (formed on the device as the host would form it).
+template <typename T> bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr)
 {
     const long pt = pitch_of<T>();
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
@@ -1507,11 +1509,11 @@ template <typename T> bool ppcg_begin_launch(int with_p, double beta)
     }
     bool ok;
     if constexpr (std::is_same<T, float>::value)
-        ok = launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
+        ok = launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
                        FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX),
                        (const T *)FT<T>(X_KY), (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     else
-        ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, (const T *)FT<T>(F_P),
+        ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
                     FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
                     (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     if (!ok) return false;
@@ -1567,6 +1569,23 @@ template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double th
     if (!ppcg_begin_launch<T>(with_p, beta) || !finish(0, 2, g.d_scal)) return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>()) || !finish(2, 1, nullptr)) return;
     if (collect(pw_rro, 0, 2)) *rrn = g.h_out[2];
+}
+
+// end (update_z + calc_rrn), then begin_ur with beta = rrn_end / rro_prev
+// formed on the device: one wait instead of two.
+template <typename T>
+void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, double *pw_rro, double *rrn)
+{
+    if (!launch(nm<T>("ppcg_end"), k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP), FT<T>(X_Z),
+                (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()) ||
+        !finish(3, 1, g.d_scal + 3))
+        return;
+    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3) || !finish(0, 2, g.d_scal)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>()) || !finish(2, 1, nullptr)) return;
+    if (collect(pw_rro, 0, 2)) {
+        *rrn = g.h_out[2];
+        *rrn_end = g.h_out[3];
+    }
 }
 
 template <typename T> void ppcg_end_impl(double *rrn)
@@ -1799,6 +1818,14 @@ EXPORT void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, doub
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     BY_PRECISION(ppcg_begin_ur_impl, with_p, beta, theta, pw_rro, rrn);
+}
+
+EXPORT void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end, double *pw_rro,
+                                         double *rrn)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    BY_PRECISION(ppcg_end_begin_ur_impl, rro_prev, theta, rrn_end, pw_rro, rrn);
 }
 
 EXPORT void tea_engine_ppcg_end(double *rrn)

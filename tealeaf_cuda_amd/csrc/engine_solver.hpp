@@ -1092,12 +1092,13 @@ __global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const T *
 // adjacent lanes; the wave's edge lanes form those themselves.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int with_p, double beta_in,
+                                                      const double *__restrict__ beta_num,
                                                       const T *__restrict__ p, T *__restrict__ pn,
                                                       const T *__restrict__ z, T *__restrict__ w,
                                                       const T *__restrict__ Kx, const T *__restrict__ Ky,
                                                       const T *__restrict__ r, T *__restrict__ rstore, Red red)
 {
-    const T beta = (T)beta_in;
+    const T beta = (T)(beta_num ? beta_num[0] / beta_in : beta_in);
     const int lane = threadIdx.x & (WAVE - 1);
     DD acc[2] = {{0, 0}, {0, 0}};
     auto pv = [&](long j, int c, int rr) -> T {
@@ -1139,13 +1140,14 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int wit
 // k_ppcg_begin for FP32 fields with two columns per lane (8-byte accesses);
 // tiles of 2*TILE_X x TILE_Y cells.  The same arithmetic cell by cell.
 __global__ __launch_bounds__(BLOCK) void k_ppcg_begin_x2(Ext e, long pitch, int with_p, double beta_in,
+                                                         const double *__restrict__ beta_num,
                                                          const float *__restrict__ p, float *__restrict__ pn,
                                                          const float *__restrict__ z, float *__restrict__ w,
                                                          const float *__restrict__ Kx, const float *__restrict__ Ky,
                                                          const float *__restrict__ r, float *__restrict__ rstore,
                                                          Red red)
 {
-    const float beta = (float)beta_in;
+    const float beta = (float)(beta_num ? beta_num[0] / beta_in : beta_in);
     const int lane = threadIdx.x & (WAVE - 1);
     DD acc[2] = {{0, 0}, {0, 0}};
     auto inside = [&](int c, int rr) { return c >= e.x0 && c <= e.x1 && rr >= e.y0 && rr <= e.y1; };

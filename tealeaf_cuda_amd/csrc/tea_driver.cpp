@@ -93,6 +93,7 @@ struct Backend {
     void (*ppcg_end)(double *) = nullptr;
     void (*ppcg_inner_n)(int, int) = nullptr;
     void (*ppcg_begin_ur)(int, double, double, double *, double *) = nullptr;
+    void (*ppcg_end_begin_ur)(double, double, double *, double *, double *) = nullptr;
     int (*ppcg_inner_plan)(int) = nullptr;
 
     void check() const
@@ -133,6 +134,8 @@ Backend load_backend(const char *path)
     b.ppcg_begin_ur =
         reinterpret_cast<void (*)(int, double, double, double *, double *)>(dlsym(b.handle, "tea_engine_ppcg_begin_ur"));
     b.ppcg_inner_plan = reinterpret_cast<int (*)(int)>(dlsym(b.handle, "tea_engine_ppcg_inner_plan"));
+    b.ppcg_end_begin_ur = reinterpret_cast<void (*)(double, double, double *, double *, double *)>(
+        dlsym(b.handle, "tea_engine_ppcg_end_begin_ur"));
     return b;
 }
 
@@ -887,6 +890,13 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
     const bool fused = d.tl_use_ppcg && fused_outer();
     bool p_pending = false;       // fused: ppcg_calc_p(beta_pending) still to apply
     double beta_pending = 0.0;
+    // fused, residual replacement on: the end of an outer iteration (update_z,
+    // calc_rrn) is issued with the next iteration's begin, beta = rrn / rro
+    // formed on the device (the recurrence's error is not a stopping test
+    // here, and nothing is printed per iteration)
+    const bool defer_end = fused && rel.on && !d.verbose_on && b.ppcg_begin_ur && b.ppcg_end_begin_ur;
+    bool end_pending = false;
+    double rro_begin = 0.0;       // r.z of the latest begin
     int fields[NUM_FIELDS];
     std::fill(fields, fields + NUM_FIELDS, 0);
 
@@ -994,7 +1004,16 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                 // the same calls as below, fused; ppcg_calc_p of the previous
                 // iteration is folded into this iteration's first pass
                 double two[2];
-                if (b.ppcg_begin_ur) {   // alpha = rro / pw formed on the device
+                if (end_pending) {   // the previous iteration's end, then begin with beta formed on the device
+                    double rrn_end = 0.0;
+                    b.ppcg_end_begin_ur(rro_begin, theta, &rrn_end, two, &rrn);
+                    b.check();
+                    beta = rrn_end / rro_begin;
+                    end_pending = false;
+                    p_pending = false;
+                    pw = two[0];
+                    rro = two[1];
+                } else if (b.ppcg_begin_ur) {   // alpha = rro / pw formed on the device
                     b.ppcg_begin_ur(p_pending ? 1 : 0, beta_pending, theta, two, &rrn);
                     b.check();
                     p_pending = false;
@@ -1010,6 +1029,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                     b.ppcg_ur(alpha, theta, &rrn);
                     b.check();
                 }
+                rro_begin = rro;
                 bool replaced = false;
                 if (rel.on && reliable_step(rrn, true, &replaced)) {
                     converged = true;
@@ -1021,11 +1041,15 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                     }
                     run_ppcg_inner_steps(&ppcg_inner_iters, true);
                     ppcg_inner_iters += d.tl_ppcg_inner_steps;
-                    b.ppcg_end(&rrn);
-                    b.check();
-                    beta = rrn / rro;
-                    p_pending = true;
-                    beta_pending = beta;
+                    if (defer_end) {
+                        end_pending = true;
+                    } else {
+                        b.ppcg_end(&rrn);
+                        b.check();
+                        beta = rrn / rro;
+                        p_pending = true;
+                        beta_pending = beta;
+                    }
                 }
                 error = rrn;
                 rro = rrn;
@@ -1084,6 +1108,11 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
         old_error = error;
     }
     // a DO loop that runs to completion leaves n = max_iters + 1
+    if (end_pending) {   // stopped by tl_max_iters: complete the last outer iteration
+        b.ppcg_end(&rrn);
+        b.check();
+        error = std::sqrt(std::fabs(rrn));
+    }
 
     return Solve{n, error, initial_residual, ppcg_inner_iters};
 }

@@ -111,6 +111,8 @@ _EXT = {
     "tea_engine_ppcg_inner_n": ([ctypes.c_int, ctypes.c_int], None),
     "tea_engine_ppcg_begin_ur": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_double)], None),
+    "tea_engine_ppcg_end_begin_ur": ([ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_ppcg_inner_plan": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
@@ -342,6 +344,12 @@ class Engine:
         two, rrn = (ctypes.c_double * 2)(), ctypes.c_double()
         self._call("tea_engine_ppcg_begin_ur", int(bool(with_p)), float(beta), float(theta), two, ctypes.byref(rrn))
         return two[0], two[1], rrn.value
+
+    def ppcg_end_begin_ur(self, rro_prev: float, theta: float):
+        rrn_end, two, rrn = ctypes.c_double(), (ctypes.c_double * 2)(), ctypes.c_double()
+        self._call("tea_engine_ppcg_end_begin_ur", float(rro_prev), float(theta), ctypes.byref(rrn_end), two,
+                   ctypes.byref(rrn))
+        return rrn_end.value, two[0], two[1], rrn.value
 
     def ppcg_ur(self, alpha: float, theta: float) -> float:
         out = ctypes.c_double()

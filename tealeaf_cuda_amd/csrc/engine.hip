@@ -1496,8 +1496,11 @@ template <typename T> void ppcg_calc_p_impl(double beta)
 // Fused outer iteration (see k_ppcg_begin / k_ppcg_ur_sd / k_ppcg_end).
 // beta_num: null (beta given), or device memory whose value over beta is the
 // beta to apply (formed on the device as the host would form it).
-template <typename T> bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr)
+// z_src: where z is read (null: the z field).
+template <typename T>
+bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr, const T *z_src = nullptr)
 {
+    const T *zf = z_src ? z_src : (const T *)FT<T>(X_Z);
     const long pt = pitch_of<T>();
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
     bool &dirty = g.p_halo_dirty[std::is_same<T, double>::value ? 0 : 1];
@@ -1510,11 +1513,11 @@ template <typename T> bool ppcg_begin_launch(int with_p, double beta, const doub
     bool ok;
     if constexpr (std::is_same<T, float>::value)
         ok = launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
-                       FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX),
+                       FT<T>(X_P2), zf, FT<T>(X_W), (const T *)FT<T>(X_KX),
                        (const T *)FT<T>(X_KY), (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     else
         ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
-                    FT<T>(X_P2), (const T *)FT<T>(X_Z), FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
+                    FT<T>(X_P2), zf, FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
                     (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     if (!ok) return false;
     if (std::is_same<T, double>::value) {
@@ -1572,15 +1575,19 @@ template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double th
 }
 
 // end (update_z + calc_rrn), then begin_ur with beta = rrn_end / rro_prev
-// formed on the device: one wait instead of two.
+// formed on the device: one wait instead of two.  With a preconditioner, ur
+// rewrites z, so z = utemp is not stored: begin reads utemp in its place.
 template <typename T>
 void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, double *pw_rro, double *rrn)
 {
-    if (!launch(nm<T>("ppcg_end"), k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP), FT<T>(X_Z),
+    const bool z_from_utemp = g.prec != PREC_NONE;
+    const T *ut = FT<T>(X_UTEMP);
+    if (!launch(nm<T>("ppcg_end"), k_ppcg_end<T>, interior(), pitch_of<T>(), ut, z_from_utemp ? nullptr : FT<T>(X_Z),
                 (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()) ||
         !finish(3, 1, g.d_scal + 3))
         return;
-    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3) || !finish(0, 2, g.d_scal)) return;
+    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3, z_from_utemp ? ut : nullptr) || !finish(0, 2, g.d_scal))
+        return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>()) || !finish(2, 1, nullptr)) return;
     if (collect(pw_rro, 0, 2)) {
         *rrn = g.h_out[2];

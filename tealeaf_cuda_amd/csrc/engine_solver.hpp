@@ -1376,7 +1376,8 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd(Ext e, long pitch, double 
     block_reduce<1>(acc, red);
 }
 
-// ppcg_update_z (z = utemp) then ppcg_calc_rrn ((r - rstore).z).
+// ppcg_update_z (z = utemp) then ppcg_calc_rrn ((r - rstore).z).  z null:
+// the store is left out (the caller overwrites z before anything reads it).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_ppcg_end(Ext e, long pitch, const T *__restrict__ utemp,
                                                     T *__restrict__ z, const T *__restrict__ r,
@@ -1389,7 +1390,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_end(Ext e, long pitch, const T *
         if (k < s.len) {
             const long i = i0 + k * pitch;
             const T zv = utemp[i];
-            z[i] = zv;
+            if (z) z[i] = zv;
             dd_add_prod(acc[0], (double)(r[i] - rstore[i]), (double)zv);
         }
     STRIP_END

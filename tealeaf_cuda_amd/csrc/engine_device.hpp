@@ -121,7 +121,19 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish(const DD *__restrict_
     for (int j = 0; j < nv; j++) {
         DD acc{0.0, 0.0};
         const DD *p = partials + (size_t)j * max_blocks;
-        for (int b = threadIdx.x; b < nblocks; b += FINISH_THREADS) dd_merge(acc, p[b]);
+        // lane t merges partials t, t+1024, ... in that order; eight loads are
+        // issued before the first merge waits
+        for (int b0 = threadIdx.x; b0 < nblocks; b0 += 8 * FINISH_THREADS) {
+            DD x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int b = b0 + u * FINISH_THREADS;
+                x[u] = b < nblocks ? p[b] : DD{0.0, 0.0};
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (b0 + u * FINISH_THREADS < nblocks) dd_merge(acc, x[u]);
+        }
         acc = wave_reduce(acc);
         if (lane == 0) lds[wave] = acc;
         __syncthreads();

@@ -265,6 +265,14 @@ void tea_engine_mp_setup(void);
 void tea_engine_mp_begin(double *rr);
 /* u += (double) u32. */
 void tea_engine_mp_end(void);
+/* A residual replacement of the FP32 solver in one call (no reference
+ * counterpart; PPCG, FP32 mode): tea_engine_mp_end,
+ * tea_engine_set_precision(64), tea_leaf_calc_residual_cuda_,
+ * tea_leaf_calc_2norm_kernel_cuda_(1) -> *rr64, tea_engine_set_precision(32),
+ * tea_engine_mp_begin -> *rr32, tea_leaf_ppcg_init_kernel_cuda_ (step 2) and
+ * tea_leaf_ppcg_init_sd_new_kernel_cuda_(theta), with the same results.
+ * Leaves the FP32 mode on. */
+void tea_engine_mp_replace(double theta, double *rr64, double *rr32);
 /* Copy an FP32 shadow field to/from a host array of (nx+2h)*(ny+2h) floats. */
 int tea_engine_get_field32(int field_id, float *host);
 int tea_engine_set_field32(int field_id, const float *host);

@@ -919,6 +919,24 @@ EXPORT void tea_engine_mp_end(void)
     FOR_EXTENT(IX0, IX1, IY0, IY1, { U[i] += (double)U32[i]; });
 }
 
+/* tea_engine_mp_replace: the calls it stands for, in order */
+EXPORT void tea_engine_mp_replace(double theta, double *rr64, double *rr32)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    if (!g.fp32 || !g.f32[1]) { die("tea_engine_mp_replace: call from the FP32 solver mode"); return; }
+    int one = 1, two = 2;
+    double unused = 0.0;
+    tea_engine_mp_end();
+    tea_engine_set_precision(64);
+    tea_leaf_calc_residual_cuda_();
+    tea_leaf_calc_2norm_kernel_cuda_(&one, rr64);
+    tea_engine_set_precision(32);
+    tea_engine_mp_begin(rr32);
+    tea_leaf_ppcg_init_kernel_cuda_(&two, &unused);
+    tea_leaf_ppcg_init_sd_new_kernel_cuda_(&theta);
+}
+
 EXPORT int tea_engine_get_field32(int id, float *host)
 {
     if (!g.initialised || !g.f32[1] || id < 1 || id >= X_COUNT) return -1;

@@ -48,6 +48,7 @@ enum FieldId {
     X_SD2 = X_COUNT,    // second sd buffer of the fused inner step (internal)
     X_P2,               // second p buffer of the fused outer iteration (internal)
     X_RT2,              // second rtemp buffer of the two-step inner kernel (internal)
+    X_U2,               // second u buffer of the fused residual replacement (internal)
     N_SLOTS
 };
 enum { SOLVER_JACOBI = 0, SOLVER_CG = 1, SOLVER_CHEBY = 2, SOLVER_PPCG = 3 };
@@ -2003,6 +2004,43 @@ EXPORT void tea_engine_mp_end(void)
     CHECK_INIT();
     if (!g.slab32) { die("tea_engine_mp_end: call tea_engine_set_precision(32) first"); return; }
     launch("mp_end", k_mp_end, interior(), g.pitch, g.pitch32, F(F_U), (const float *)g.fs32[F_U]);
+}
+
+// mp_end, set_precision(64), calc_residual, calc_2norm(1) -> rr64,
+// set_precision(32), mp_begin -> rr32, ppcg_init step 2, init_sd_new(theta):
+// one pass (k_mp_replace, u' into the spare u buffer, which becomes u), the
+// u32 = 0 fill, one wait.  rtemp and utemp still owed need no writing.
+EXPORT void tea_engine_mp_replace(double theta, double *rr64, double *rr32)
+{
+    CHECK_INIT_KEEP_PENDING();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    if (!g.fp32 || !g.slab32) { die("tea_engine_mp_replace: call from the FP32 solver mode"); return; }
+    drop_or_settle_pending();
+    const Ext in = interior();
+    const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+    {   // the spare takes u's halo, which u' keeps
+        Timed t("u_halo_ring");
+        hipLaunchKernelGGL(k_copy_ring<double>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream, F(X_U2),
+                           (const double *)F(F_U), g.pitch, g.h, g.nx, g.ny);
+    }
+    auto go = [&](auto kernel) {
+        return launch("mp_replace", kernel, in, g.pitch, g.pitch32, (const double *)F(F_U), F(X_U2),
+                      (const double *)F(X_U0), F(F_R), (const double *)F(X_KX), (const double *)F(X_KY),
+                      (const float *)g.fs32[F_U], g.fs32[F_R], g.fs32[X_U0], g.fs32[X_Z], g.fs32[F_SD],
+                      g.fs32[X_RTEMP], g.fs32[X_UTEMP], (const float *)g.fs32[X_MI], (const float *)g.fs32[X_KX],
+                      (const float *)g.fs32[X_KY], theta, red_ctx());
+    };
+    const bool ok = g.prec == PREC_BLOCK ? go(k_mp_replace<PREC_BLOCK>)
+                    : g.prec == PREC_DIAG ? go(k_mp_replace<PREC_DIAG>)
+                                          : go(k_mp_replace<PREC_NONE>);
+    if (!ok || !finish(0, 2, nullptr)) return;
+    std::swap(g.f[F_U], g.f[X_U2]);
+    std::swap(g.fs[F_U], g.fs[X_U2]);
+    if (!launch("mp_zero_u32", k_fill<float>, in, g.pitch32, g.fs32[F_U], 0.0f)) return;
+    double v[2];
+    if (!collect(v, 0, 2)) return;
+    *rr64 = v[0];
+    *rr32 = v[1];
 }
 
 EXPORT int tea_engine_get_field32(int id, float *host)

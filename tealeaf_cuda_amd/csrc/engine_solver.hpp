@@ -1448,4 +1448,89 @@ __global__ __launch_bounds__(BLOCK) void k_mp_end(Ext e, long pitch, long pitch3
     STRIP_END
 }
 
+// a = v on the extent
+template <typename T> __global__ __launch_bounds__(BLOCK) void k_fill(Ext e, long pitch, T *__restrict__ a, T v)
+{
+    STRIP_BEGIN
+    for (int k = 0; k < s.len; k++) a[i0 + k * pitch] = v;
+    STRIP_END
+}
+
+// A residual replacement of the FP32 inner solver in one pass: the composition
+// of k_mp_end (u' = u + (double) u32 on the interior), k_calc_residual
+// (r = u0 - A u'), the FP64 r.r, k_mp_begin (r32 = (float) r, u0_32 = r32,
+// sum r32*r32), k_ppcg_init step 2 (z32 = M^-1 r32 with the FP32
+// coefficients) and k_ppcg_init_sd_new (sd32 = (z32 or r32) * (1/theta),
+// rtemp32 = r32, utemp32 = sd32), with their expressions, so the same values.
+// u' goes to u_out (u is read at the neighbours other lanes own); u32 = 0 is
+// left to a following launch (u32 is read at those neighbours too).  Each lane
+// forms u' down its column (the strip and the rows above and below it) and
+// takes its x neighbours from the adjacent lanes; edge lanes form those
+// themselves.
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_mp_replace(
+    Ext e, long pitch, long pitch32, const double *__restrict__ u, double *__restrict__ u_out,
+    const double *__restrict__ u0, double *__restrict__ r, const double *__restrict__ Kx,
+    const double *__restrict__ Ky, const float *__restrict__ u32, float *__restrict__ r32, float *__restrict__ u0_32,
+    float *__restrict__ z32, float *__restrict__ sd32, float *__restrict__ rtemp32, float *__restrict__ utemp32,
+    const float *__restrict__ Mi32, const float *__restrict__ Kx32, const float *__restrict__ Ky32, double theta_in,
+    Red red)
+{
+    const float theta_r = 1.0f / (float)theta_in;
+    const int lane = threadIdx.x & (WAVE - 1);
+    DD acc[2] = {{0, 0}, {0, 0}};
+    auto inside = [&](int c, int rr) { return c >= e.x0 && c <= e.x1 && rr >= e.y0 && rr <= e.y1; };
+    auto unew = [&](int c, int rr) -> double {
+        const long i = (long)c + (long)rr * pitch;
+        return inside(c, rr) ? u[i] + (double)u32[(long)c + (long)rr * pitch32] : u[i];
+    };
+    STRIP_BEGIN
+    const int c = s.c;
+    const long j0 = (long)c + (long)s.r0 * pitch32;
+    double col[STRIP + 2];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; q++) {
+        col[q] = 0.0;
+        if (s.len > 0 && q <= s.len + 1) col[q] = unew(c, s.r0 - 1 + q);
+    }
+    const bool rlane = lane < WAVE - 1 && c + 1 <= e.x1;
+    float rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        const double from_left = __shfl_up(col[k + 1], 1, WAVE);
+        const double from_right = __shfl_down(col[k + 1], 1, WAVE);
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const int rr = s.r0 + k;
+            const double left = lane > 0 ? from_left : unew(c - 1, rr);
+            const double right = rlane ? from_right : unew(c + 1, rr);
+            const double smvp = (1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * col[k + 1]
+                                - (Ky[i + pitch] * col[k + 2] + Ky[i] * col[k])
+                                - (Kx[i + 1] * right + Kx[i] * left);
+            const double rd = u0[i] - smvp;
+            u_out[i] = col[k + 1];
+            r[i] = rd;
+            dd_add_prod(acc[0], rd, rd);
+            rv[k] = (float)rd;
+            dd_add_prod(acc[1], (double)rv[k], (double)rv[k]);
+        }
+    }
+    if (PREC != PREC_NONE) precond_strip<float, PREC>(rv, zv, s.len, j0, pitch32, Mi32, Kx32, Ky32);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long j = j0 + k * pitch32;
+            r32[j] = rv[k];
+            u0_32[j] = rv[k];
+            if (PREC != PREC_NONE) z32[j] = zv[k];
+            const float sdv = (PREC != PREC_NONE ? zv[k] : rv[k]) * theta_r;
+            sd32[j] = sdv;
+            rtemp32[j] = rv[k];
+            utemp32[j] = sdv;
+        }
+    }
+    STRIP_END
+    block_reduce<2>(acc, red);
+}
+
 }  // namespace tea

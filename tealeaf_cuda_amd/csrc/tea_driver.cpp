@@ -86,6 +86,7 @@ struct Backend {
     void (*mp_setup)(void) = nullptr;
     void (*mp_begin)(double *) = nullptr;
     void (*mp_end)(void) = nullptr;
+    void (*mp_replace)(double, double *, double *) = nullptr;
     int (*warm_start_load)(const float *, int) = nullptr;
     void (*warm_start_apply)(int *) = nullptr;
     void (*ppcg_begin)(int, double, double *) = nullptr;
@@ -125,6 +126,7 @@ Backend load_backend(const char *path)
     b.mp_setup = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_setup"));
     b.mp_begin = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_mp_begin"));
     b.mp_end = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_end"));
+    b.mp_replace = reinterpret_cast<void (*)(double, double *, double *)>(dlsym(b.handle, "tea_engine_mp_replace"));
     b.warm_start_load = reinterpret_cast<int (*)(const float *, int)>(dlsym(b.handle, "tea_engine_warm_start_load"));
     b.warm_start_apply = reinterpret_cast<void (*)(int *)>(dlsym(b.handle, "tea_engine_warm_start_apply"));
     b.ppcg_begin = reinterpret_cast<void (*)(int, double, double *)>(dlsym(b.handle, "tea_engine_ppcg_begin"));
@@ -1243,6 +1245,10 @@ Run::Solve Run::refine_reliable(double initial_residual_sq, double target_residu
     return Solve{s.n, std::sqrt(std::fabs(rel.rr_final)), ir, s.ppcg_inner};
 }
 
+// One residual-replacement test of refine_reliable, given the loop's own
+// measure q: true when the solve ends (converged or stagnated).  After a
+// replacement, *replaced (if given) says whether the caller still has to form
+// sd, rtemp and utemp (tea_engine_mp_replace forms them itself).
 bool Run::reliable_step(double q, bool refresh_z, bool *replaced)
 {
     if (replaced) *replaced = false;
@@ -1254,22 +1260,38 @@ bool Run::reliable_step(double q, bool refresh_z, bool *replaced)
     const double drop = std::sqrt(std::fabs(q / rel.q_ref));
     const double predicted = std::sqrt(std::fabs(rel.rr_ref)) * drop;
     if (drop >= d.reliable_delta && predicted >= rel.target) return false;
-    if (rel.bits == 32) {
-        b.mp_end();
+    // FP32 PPCG: the replacement and the restart from it (r32, z, sd, rtemp,
+    // utemp) in one engine call; when the solve ends here, the restart is
+    // simply left unused
+    const bool fused = rel.bits == 32 && refresh_z && replaced && b.mp_replace;
+    double rr = 0.0;
+    if (fused) {
+        double rr32 = 0.0;
+        b.mp_replace(theta, &rr, &rr32);
         b.check();
+    } else {
+        if (rel.bits == 32) {
+            b.mp_end();
+            b.check();
+            b.set_precision(64);
+            rel.fp32_live = false;
+        }
+        b.tea_leaf_calc_residual_cuda_();
+        b.check();
+        int which = 1;
+        b.tea_leaf_calc_2norm_kernel_cuda_(&which, &rr);
+        b.check();
+    }
+    auto leave_fp32 = [&] {
+        if (!fused) return;
         b.set_precision(64);
         rel.fp32_live = false;
-    }
-    b.tea_leaf_calc_residual_cuda_();
-    b.check();
-    double rr = 0.0;
-    int which = 1;
-    b.tea_leaf_calc_2norm_kernel_cuda_(&which, &rr);
-    b.check();
+    };
     rel.updates++;
     rel.rr_final = rr;
     if (d.verbose_on) log(" Replacement %d true residual %.16E\n", rel.updates, std::sqrt(std::fabs(rr)));
     if (std::sqrt(std::fabs(rr)) < rel.target) {
+        leave_fp32();
         rel.converged = true;
         return true;
     }
@@ -1281,26 +1303,29 @@ bool Run::reliable_step(double q, bool refresh_z, bool *replaced)
     const bool stall = now > 4.0 * predicted || now > 0.99 * std::sqrt(std::fabs(rel.rr_ref));
     rel.stalls = stall ? rel.stalls + 1 : 0;
     if (rel.stalls >= 3) {
+        leave_fp32();
         rel.converged = true;
         rel.stagnated = true;
         log("True residual stagnated at %s of the target\n",
             fortran_e(std::sqrt(std::fabs(rr)) / rel.target, 14, 7).c_str());
         return true;
     }
-    if (rel.bits == 32) {
-        b.set_precision(32);
-        double rr32 = 0.0;
-        b.mp_begin(&rr32);
-        b.check();
-        rel.fp32_live = true;
+    if (!fused) {
+        if (rel.bits == 32) {
+            b.set_precision(32);
+            double rr32 = 0.0;
+            b.mp_begin(&rr32);
+            b.check();
+            rel.fp32_live = true;
+        }
+        if (refresh_z) {
+            int st = 2;
+            double unused = 0.0;
+            b.tea_leaf_ppcg_init_kernel_cuda_(&st, &unused);  // z = M^-1 r for the new r
+            b.check();
+        }
+        if (replaced) *replaced = true;  // the caller forms sd, rtemp, utemp
     }
-    if (refresh_z) {
-        int st = 2;
-        double unused = 0.0;
-        b.tea_leaf_ppcg_init_kernel_cuda_(&st, &unused);  // z = M^-1 r for the new r
-        b.check();
-    }
-    if (replaced) *replaced = true;
     rel.rr_ref = rr;
     rel.q_ref = q;
     return false;

@@ -118,6 +118,7 @@ _EXT = {
     "tea_engine_mp_setup": ([], None),
     "tea_engine_mp_begin": ([_D], None),
     "tea_engine_mp_end": ([], None),
+    "tea_engine_mp_replace": ([ctypes.c_double, _D, _D], None),
     "tea_engine_get_field32": ([ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "tea_engine_set_field32": ([ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
 }
@@ -388,6 +389,11 @@ class Engine:
 
     def mp_end(self):
         self._call("tea_engine_mp_end")
+
+    def mp_replace(self, theta: float):
+        rr64, rr32 = ctypes.c_double(), ctypes.c_double()
+        self._call("tea_engine_mp_replace", float(theta), ctypes.byref(rr64), ctypes.byref(rr32))
+        return rr64.value, rr32.value
 
     def get_field32(self, field_id) -> np.ndarray:
         self.refresh()  # the driver may have (re)initialised the chunk since

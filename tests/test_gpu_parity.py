@@ -321,6 +321,28 @@ def test_fp32_entry_points_bitwise(prec, oracle_so, oracle_env, tmp_path):
         assert outs[0] == outs[1]
         _compare(g, c, "mp_end")
         assert np.array_equal(g.get_field32(rt.FIELD_R), c.get_field32(rt.FIELD_R))
+        # the fused replacement, twice (the second on the swapped u buffers),
+        # with random u (halo included) and u32
+        for rep in range(2):
+            print(f"[gpu parity] fp32 mp_replace {prec} call {rep}", flush=True)
+            u = rng.standard_normal(g.shape())
+            u32 = (rng.standard_normal(g.shape()) * 0.01).astype(np.float32)
+            for e in (g, c):
+                e.set_precision(64)
+                e.set_field(rt.FIELD_U, u)
+                e.set_precision(32)
+                e.set_field32(rt.FIELD_U, u32)
+            outs = [e.mp_replace(1.7) for e in (g, c)]
+            assert outs[0] == outs[1]
+            for fid in range(1, 20):
+                if fid in (rt.XFIELD_CP, rt.XFIELD_BFP):
+                    continue
+                assert np.array_equal(g.get_field32(fid), c.get_field32(fid), equal_nan=True), f"mp_replace: {fid}"
+            for e in (g, c):
+                e.set_precision(64)
+            _compare(g, c, "mp_replace")
+            for e in (g, c):
+                e.set_precision(32)
     finally:
         g.finalize()
         c.finalize()

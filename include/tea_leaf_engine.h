@@ -237,6 +237,14 @@ void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, double *pw_
  * rro_prev, theta, pw_rro, rrn) with the division done on the device (one host
  * wait instead of two); rro_prev is the r.z of the previous begin. */
 void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end, double *pw_rro, double *rrn);
+/* tea_engine_ppcg_end_begin_ur, then inner steps 1..inner_steps
+ * (tea_engine_ppcg_inner_n passes as tea_engine_ppcg_inner_plan splits them),
+ * returning once the end_begin_ur values are known while the passes still
+ * run.  A caller that then finds it needs a residual replacement first lets
+ * tea_engine_mp_replace overwrite what the passes wrote (sd, rtemp, utemp)
+ * and runs them again. */
+void tea_engine_ppcg_end_begin_ur_inner(double rro_prev, double theta, int inner_steps, double *rrn_end,
+                                        double *pw_rro, double *rrn);
 /* Inner steps `step` .. `step + nsteps - 1` (1-based; nsteps 1..5) in one
  * pass over HBM (temporal blocking): the same results as
  * tea_leaf_ppcg_inner_kernel_cuda_ called for each of the steps with

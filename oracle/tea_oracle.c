@@ -851,6 +851,14 @@ EXPORT void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *
     tea_engine_ppcg_begin_ur(1, *rrn_end / rro_prev, theta, pw_rro, rrn);
 }
 
+EXPORT void tea_engine_ppcg_inner_n(int step, int nsteps);
+EXPORT void tea_engine_ppcg_end_begin_ur_inner(double rro_prev, double theta, int inner_steps, double *rrn_end,
+                                               double *pw_rro, double *rrn)
+{
+    tea_engine_ppcg_end_begin_ur(rro_prev, theta, rrn_end, pw_rro, rrn);
+    if (inner_steps > 0) tea_engine_ppcg_inner_n(1, inner_steps);
+}
+
 EXPORT void tea_engine_ppcg_end(double *rrn)
 {
     tea_leaf_ppcg_update_z_kernel_cuda_();

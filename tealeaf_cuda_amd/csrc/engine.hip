@@ -146,6 +146,7 @@ struct Chunk {
     double *h_out = nullptr;         // pinned: the finished values
     double *d_out = nullptr;         // device alias of h_out
     double *d_scal = nullptr;        // finished values kept in device memory for a following launch
+    hipEvent_t ev_scal = nullptr;    // marks finished values that later queued launches need not delay
     // rtemp = r and utemp = sd owed by tea_engine_ppcg_begin_ur (0: none, else
     // the field precision in bits): the next streaming inner pass reads r and
     // sd in their place; any other entry point writes them out first.
@@ -864,6 +865,8 @@ void release_all()
     if (g.d_partials) (void)hipFree(g.d_partials);
     if (g.h_out) (void)hipHostFree(g.h_out);
     if (g.d_scal) (void)hipFree(g.d_scal);
+    if (g.ev_scal) (void)hipEventDestroy(g.ev_scal);
+    g.ev_scal = nullptr;
     if (g.d_halo_buf) (void)hipFree(g.d_halo_buf);
     if (g.slab32) (void)hipFree(g.slab32);
     ws::net_free(g.ws_net);
@@ -944,6 +947,7 @@ EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
     HIP_OK(hipHostMalloc(&g.h_out, MAX_RED * sizeof(double), hipHostMallocDefault));
     HIP_OK(hipHostGetDevicePointer((void **)&g.d_out, g.h_out, 0));
     HIP_OK(hipMalloc(&g.d_scal, MAX_RED * sizeof(double)));
+    if (!g.ev_scal) HIP_OK(hipEventCreateWithFlags(&g.ev_scal, hipEventDisableTiming));
 
     g.halo_buf_elems = (size_t)TEA_NUM_FIELDS * (size_t)g.h * (size_t)((g.nx > g.ny ? g.nx : g.ny) + 2 * g.h);
     HIP_OK(hipMalloc(&g.d_halo_buf, g.halo_buf_elems * sizeof(double)));
@@ -1578,8 +1582,12 @@ template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double th
 // end (update_z + calc_rrn), then begin_ur with beta = rrn_end / rro_prev
 // formed on the device: one wait instead of two.  With a preconditioner, ur
 // rewrites z, so z = utemp is not stored: begin reads utemp in its place.
+// inner_steps > 0: inner steps 1..inner_steps are queued behind ur (passes as
+// tea_engine_ppcg_inner_plan splits them) before the wait, which then waits
+// for ur's values only.
 template <typename T>
-void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, double *pw_rro, double *rrn)
+void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, double *pw_rro, double *rrn,
+                            int inner_steps = 0)
 {
     const bool z_from_utemp = g.prec != PREC_NONE;
     const T *ut = FT<T>(X_UTEMP);
@@ -1590,10 +1598,21 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
     if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3, z_from_utemp ? ut : nullptr) || !finish(0, 2, g.d_scal))
         return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>()) || !finish(2, 1, nullptr)) return;
-    if (collect(pw_rro, 0, 2)) {
-        *rrn = g.h_out[2];
-        *rrn_end = g.h_out[3];
+    if (inner_steps > 0) {
+        if (hipEventRecord(g.ev_scal, g.stream) != hipSuccess) { die("hipEventRecord failed"); return; }
+        for (int step = 1; step <= inner_steps;) {
+            const int n = plan_first_pass(inner_steps - step + 1, g.fp32 ? PASS_US_F32 : PASS_US_F64);
+            ppcg_inner_n_impl<T>(step - 1, n);
+            step += n;
+        }
+        const hipError_t err = hipEventSynchronize(g.ev_scal);
+        if (err != hipSuccess) { die("hipEventSynchronize failed: %s", hipGetErrorString(err)); return; }
+        for (int j = 0; j < 2; j++) pw_rro[j] = g.h_out[j];
+    } else if (!collect(pw_rro, 0, 2)) {
+        return;
     }
+    *rrn = g.h_out[2];
+    *rrn_end = g.h_out[3];
 }
 
 template <typename T> void ppcg_end_impl(double *rrn)
@@ -1834,6 +1853,18 @@ EXPORT void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *
     CHECK_INIT();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     BY_PRECISION(ppcg_end_begin_ur_impl, rro_prev, theta, rrn_end, pw_rro, rrn);
+}
+
+EXPORT void tea_engine_ppcg_end_begin_ur_inner(double rro_prev, double theta, int inner_steps, double *rrn_end,
+                                               double *pw_rro, double *rrn)
+{
+    CHECK_INIT();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    if (inner_steps < 0 || inner_steps > (int)g.ch_a.size()) {
+        die("ppcg_end_begin_ur_inner: %d inner steps, %d coefficients uploaded", inner_steps, (int)g.ch_a.size());
+        return;
+    }
+    BY_PRECISION(ppcg_end_begin_ur_impl, rro_prev, theta, rrn_end, pw_rro, rrn, inner_steps);
 }
 
 EXPORT void tea_engine_ppcg_end(double *rrn)

@@ -95,6 +95,7 @@ struct Backend {
     void (*ppcg_inner_n)(int, int) = nullptr;
     void (*ppcg_begin_ur)(int, double, double, double *, double *) = nullptr;
     void (*ppcg_end_begin_ur)(double, double, double *, double *, double *) = nullptr;
+    void (*ppcg_end_begin_ur_inner)(double, double, int, double *, double *, double *) = nullptr;
     int (*ppcg_inner_plan)(int) = nullptr;
 
     void check() const
@@ -138,6 +139,8 @@ Backend load_backend(const char *path)
     b.ppcg_inner_plan = reinterpret_cast<int (*)(int)>(dlsym(b.handle, "tea_engine_ppcg_inner_plan"));
     b.ppcg_end_begin_ur = reinterpret_cast<void (*)(double, double, double *, double *, double *)>(
         dlsym(b.handle, "tea_engine_ppcg_end_begin_ur"));
+    b.ppcg_end_begin_ur_inner = reinterpret_cast<void (*)(double, double, int, double *, double *, double *)>(
+        dlsym(b.handle, "tea_engine_ppcg_end_begin_ur_inner"));
     return b;
 }
 
@@ -567,11 +570,15 @@ private:
         bool fp32_live = false; // the backend is in FP32 mode with u32 pending
         bool converged = false;
         bool stagnated = false; // the true residual stopped falling: FP64's attainable accuracy
+        double q_last = 0.0;    // the last two measures passed to reliable_step
+        double q_prev = 0.0;
+        int nq = 0;
         int stalls = 0;
         int updates = 0;
         double rr_final = 0.0;
     } rel;
     bool reliable_step(double q, bool refresh_z, bool *replaced);
+    bool replacement_unlikely() const;
     void run_ppcg_inner_steps(int *ppcg_inner_iters, bool fused = false);
     // the fused outer iteration (tea_engine_ppcg_begin/ur/end) is used when the
     // backend has it and no halo update separates the fused calls
@@ -899,6 +906,14 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
     const bool defer_end = fused && rel.on && !d.verbose_on && b.ppcg_begin_ur && b.ppcg_end_begin_ur;
     bool end_pending = false;
     double rro_begin = 0.0;       // r.z of the latest begin
+    // deferred end: the next inner steps may be queued with it, before its
+    // values are known, when no residual replacement is expected (a replacement
+    // after all overwrites what they wrote, and they run again)
+    const char *spec_env = std::getenv("TEA_SPECULATE");
+    const char *passes_env = std::getenv("TEA_INNER_PASSES");
+    const bool may_queue_inner = defer_end && b.ppcg_end_begin_ur_inner && b.ppcg_inner_n && b.ppcg_inner_plan &&
+                                 d.tl_ppcg_inner_steps > 0 && !(passes_env && passes_env[0] == '0') &&
+                                 !(spec_env && spec_env[0] == '0');
     int fields[NUM_FIELDS];
     std::fill(fields, fields + NUM_FIELDS, 0);
 
@@ -1006,9 +1021,15 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                 // the same calls as below, fused; ppcg_calc_p of the previous
                 // iteration is folded into this iteration's first pass
                 double two[2];
+                bool inner_queued = false;
                 if (end_pending) {   // the previous iteration's end, then begin with beta formed on the device
                     double rrn_end = 0.0;
-                    b.ppcg_end_begin_ur(rro_begin, theta, &rrn_end, two, &rrn);
+                    if (may_queue_inner && replacement_unlikely()) {
+                        b.ppcg_end_begin_ur_inner(rro_begin, theta, d.tl_ppcg_inner_steps, &rrn_end, two, &rrn);
+                        inner_queued = true;
+                    } else {
+                        b.ppcg_end_begin_ur(rro_begin, theta, &rrn_end, two, &rrn);
+                    }
                     b.check();
                     beta = rrn_end / rro_begin;
                     end_pending = false;
@@ -1033,6 +1054,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                 }
                 rro_begin = rro;
                 bool replaced = false;
+                const int updates_before = rel.updates;
                 if (rel.on && reliable_step(rrn, true, &replaced)) {
                     converged = true;
                 } else {
@@ -1041,7 +1063,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                         b.tea_leaf_ppcg_init_sd_new_kernel_cuda_(&th);
                         b.check();
                     }
-                    run_ppcg_inner_steps(&ppcg_inner_iters, true);
+                    if (!inner_queued || rel.updates != updates_before) run_ppcg_inner_steps(&ppcg_inner_iters, true);
                     ppcg_inner_iters += d.tl_ppcg_inner_steps;
                     if (defer_end) {
                         end_pending = true;
@@ -1249,9 +1271,27 @@ Run::Solve Run::refine_reliable(double initial_residual_sq, double target_residu
 // measure q: true when the solve ends (converged or stagnated).  After a
 // replacement, *replaced (if given) says whether the caller still has to form
 // sd, rtemp and utemp (tea_engine_mp_replace forms them itself).
+// Whether the next reliable_step can be expected to replace nothing: no
+// reference level yet, or the last two measures, extrapolated geometrically,
+// keep the drop and the predicted residual twice their thresholds.
+bool Run::replacement_unlikely() const
+{
+    if (!rel.on) return false;
+    if (!rel.have_q) return true;
+    if (rel.nq < 2 || !(std::fabs(rel.q_prev) > 0.0) || !(std::fabs(rel.q_ref) > 0.0)) return false;
+    const double ratio = std::fabs(rel.q_last / rel.q_prev);
+    const double q_next = std::fabs(rel.q_last) * (ratio < 1.0 ? ratio : 1.0);
+    const double drop = std::sqrt(q_next / std::fabs(rel.q_ref));
+    const double predicted = std::sqrt(std::fabs(rel.rr_ref)) * drop;
+    return drop >= 2.0 * d.reliable_delta && predicted >= 2.0 * rel.target;
+}
+
 bool Run::reliable_step(double q, bool refresh_z, bool *replaced)
 {
     if (replaced) *replaced = false;
+    rel.q_prev = rel.q_last;
+    rel.q_last = q;
+    rel.nq++;
     if (!rel.have_q) {  // the first measure after a replacement is the reference level
         rel.q_ref = q;
         rel.have_q = true;

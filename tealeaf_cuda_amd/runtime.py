@@ -113,6 +113,9 @@ _EXT = {
                                   ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_ppcg_end_begin_ur": ([ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], None),
+    "tea_engine_ppcg_end_begin_ur_inner": ([ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_ppcg_inner_plan": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
@@ -350,6 +353,12 @@ class Engine:
         rrn_end, two, rrn = ctypes.c_double(), (ctypes.c_double * 2)(), ctypes.c_double()
         self._call("tea_engine_ppcg_end_begin_ur", float(rro_prev), float(theta), ctypes.byref(rrn_end), two,
                    ctypes.byref(rrn))
+        return rrn_end.value, two[0], two[1], rrn.value
+
+    def ppcg_end_begin_ur_inner(self, rro_prev: float, theta: float, inner_steps: int):
+        rrn_end, two, rrn = ctypes.c_double(), (ctypes.c_double * 2)(), ctypes.c_double()
+        self._call("tea_engine_ppcg_end_begin_ur_inner", float(rro_prev), float(theta), int(inner_steps),
+                   ctypes.byref(rrn_end), two, ctypes.byref(rrn))
         return rrn_end.value, two[0], two[1], rrn.value
 
     def ppcg_ur(self, alpha: float, theta: float) -> float:

@@ -98,13 +98,14 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
         assert a.values() == b.values(), (a, b)
 
 
-@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH"])
+@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
     """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0), the fused
-    outer iteration with one inner step per pass (TEA_INNER_PASSES=0), and the
-    inner passes on the tile kernels (TEA_INNER_MARCH=0), on the GPU give the
-    same results as the oracle."""
+    outer iteration with one inner step per pass (TEA_INNER_PASSES=0), the
+    inner passes on the tile kernels (TEA_INNER_MARCH=0), and the refinement
+    loop without inner passes queued ahead of the replacement test
+    (TEA_SPECULATE=0), on the GPU give the same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -190,6 +191,9 @@ PPCG_CALLS = {
     "fused_ppcg_begin_ur_p": lambda e: e.ppcg_begin_ur(True, 0.45, 1.7),
     "fused_ppcg_begin_ur": lambda e: e.ppcg_begin_ur(False, 0.0, 1.7),
     "fused_ppcg_end_begin_ur": lambda e: e.ppcg_end_begin_ur(0.8, 1.7),
+    "fused_ppcg_end_begin_ur_inner6": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_end_begin_ur_inner(0.8, 1.7, 6)),
     "fused_ppcg_inner_n4_then_end_begin_ur": lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_begin_ur(False, 0.0, 1.7), e.ppcg_inner_n(1, 4), e.ppcg_end_begin_ur(0.8, 1.7)),

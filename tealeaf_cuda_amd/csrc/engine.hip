@@ -151,6 +151,9 @@ struct Chunk {
     // the field precision in bits): the next streaming inner pass reads r and
     // sd in their place; any other entry point writes them out first.
     int pending_rt_ut = 0;
+    // FP32 w = A p owed by the fused begin/ur pair (k_ppcg_ur_sd_x2_wp forms w
+    // from p instead of reading it); written out by settle_pending
+    bool pending_w = false;
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -1196,12 +1199,23 @@ template <typename T> void settle_fields()
     launch(nm<T>("settle_utemp"), k_copy<T>, in, pitch_of<T>(), FT<T>(X_UTEMP), (const T *)FT<T>(F_SD));
 }
 
-void settle_pending()
+// rtemp and utemp owed (only)
+void settle_rt_ut()
 {
     const int bits = g.pending_rt_ut;
     g.pending_rt_ut = 0;
     if (bits == 32) settle_fields<float>();
     else if (bits == 64) settle_fields<double>();
+}
+
+void settle_pending()
+{
+    settle_rt_ut();
+    if (g.pending_w) {   // w = A p as cg_calc_w forms it (its p.w is not used)
+        g.pending_w = false;
+        launch("settle_w_f32", k_cg_calc_w<float>, interior(), g.pitch32, (const float *)g.fs32[F_P], g.fs32[X_W],
+               (const float *)g.fs32[X_KX], (const float *)g.fs32[X_KY], red_ctx());
+    }
 }
 
 #define PREC_SWITCH_T(T, PREC_VALUE, NAME, KERNEL, ...)                                           \
@@ -1501,9 +1515,18 @@ template <typename T> void ppcg_calc_p_impl(double beta)
 // Fused outer iteration (see k_ppcg_begin / k_ppcg_ur_sd / k_ppcg_end).
 // beta_num: null (beta given), or device memory whose value over beta is the
 // beta to apply (formed on the device as the host would form it).
-// z_src: where z is read (null: the z field).
+// FP32 block-Jacobi fused pair: begin leaves w = A p' unwritten and ur forms
+// it again from p' (k_ppcg_ur_sd_x2_wp), unless TEA_W_FROM_P=0.
+template <typename T> bool w_from_p()
+{
+    const char *s = getenv("TEA_W_FROM_P");
+    return std::is_same<T, float>::value && g.prec == PREC_BLOCK && !(s && s[0] == '0');
+}
+
+// z_src: where z is read (null: the z field); w_later: w not stored (FP32).
 template <typename T>
-bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr, const T *z_src = nullptr)
+bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr, const T *z_src = nullptr,
+                       bool w_later = false)
 {
     const T *zf = z_src ? z_src : (const T *)FT<T>(X_Z);
     const long pt = pitch_of<T>();
@@ -1518,7 +1541,7 @@ bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr
     bool ok;
     if constexpr (std::is_same<T, float>::value)
         ok = launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
-                       FT<T>(X_P2), zf, FT<T>(X_W), (const T *)FT<T>(X_KX),
+                       FT<T>(X_P2), zf, w_later ? nullptr : FT<T>(X_W), (const T *)FT<T>(X_KX),
                        (const T *)FT<T>(X_KY), (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     else
         ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
@@ -1542,11 +1565,20 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
 // pw_rro: null (alpha given) or device memory holding p.w, r.z.
 // defer: leave rtemp = r and utemp = sd to the next streaming inner pass
 // (recorded in g.pending_rt_ut).
-template <typename T> bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer = false)
+// w_from_p: w formed from p (FP32 block Jacobi; begin left it unwritten).
+template <typename T>
+bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer = false, bool w_from_p = false)
 {
     T *rt = defer ? nullptr : FT<T>(X_RTEMP), *ut = defer ? nullptr : FT<T>(X_UTEMP);
     bool ok;
     if constexpr (std::is_same<T, float>::value) {
+        if (w_from_p && g.prec == PREC_BLOCK) {
+            ok = launch_x2("k_ppcg_ur_sd_f32", k_ppcg_ur_sd_x2_wp, interior(), pitch_of<T>(), alpha, pw_rro, theta,
+                           FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), FT<T>(X_Z), (const T *)FT<T>(X_KX),
+                           (const T *)FT<T>(X_KY), FT<T>(F_SD), rt, ut, red_ctx());
+            if (ok && defer) g.pending_rt_ut = 32;
+            return ok;
+        }
         auto go = [&](auto kernel) {
             return launch_x2("k_ppcg_ur_sd_f32", kernel, interior(), pitch_of<T>(), alpha, pw_rro, theta, FT<T>(F_U),
                              (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
@@ -1574,8 +1606,10 @@ template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
 // begin, then ur with alpha = rro / pw formed on the device: one wait.
 template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double theta, double *pw_rro, double *rrn)
 {
-    if (!ppcg_begin_launch<T>(with_p, beta) || !finish(0, 2, g.d_scal)) return;
-    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>()) || !finish(2, 1, nullptr)) return;
+    const bool wp = w_from_p<T>();
+    if (!ppcg_begin_launch<T>(with_p, beta, nullptr, nullptr, wp) || !finish(0, 2, g.d_scal)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
+    g.pending_w = wp;
     if (collect(pw_rro, 0, 2)) *rrn = g.h_out[2];
 }
 
@@ -1595,9 +1629,11 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
                 (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()) ||
         !finish(3, 1, g.d_scal + 3))
         return;
-    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3, z_from_utemp ? ut : nullptr) || !finish(0, 2, g.d_scal))
+    const bool wp = w_from_p<T>();
+    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3, z_from_utemp ? ut : nullptr, wp) || !finish(0, 2, g.d_scal))
         return;
-    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>()) || !finish(2, 1, nullptr)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
+    g.pending_w = wp;
     if (inner_steps > 0) {
         if (hipEventRecord(g.ev_scal, g.stream) != hipSuccess) { die("hipEventRecord failed"); return; }
         for (int step = 1; step <= inner_steps;) {
@@ -1746,7 +1782,15 @@ EXPORT void tea_leaf_ppcg_init_kernel_cuda_(const int *step, double *rro)
 static void drop_or_settle_pending()
 {
     if (g.pending_rt_ut == (g.fp32 ? 32 : 64)) g.pending_rt_ut = 0;
-    else settle_pending();
+    else settle_rt_ut();
+}
+
+// The fused begin rewrites w: an owed w is dropped, owed rtemp and utemp are
+// written out.
+static void drop_w_settle_rest()
+{
+    g.pending_w = false;
+    settle_rt_ut();
 }
 
 EXPORT void tea_leaf_ppcg_init_sd_kernel_cuda_(const double *theta)
@@ -1842,7 +1886,8 @@ EXPORT void tea_engine_ppcg_ur(double alpha, double theta, double *rrn)
 
 EXPORT void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, double *pw_rro, double *rrn)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();
+    drop_w_settle_rest();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     BY_PRECISION(ppcg_begin_ur_impl, with_p, beta, theta, pw_rro, rrn);
 }
@@ -1850,7 +1895,8 @@ EXPORT void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, doub
 EXPORT void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end, double *pw_rro,
                                          double *rrn)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();
+    drop_w_settle_rest();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     BY_PRECISION(ppcg_end_begin_ur_impl, rro_prev, theta, rrn_end, pw_rro, rrn);
 }
@@ -1858,7 +1904,8 @@ EXPORT void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *
 EXPORT void tea_engine_ppcg_end_begin_ur_inner(double rro_prev, double theta, int inner_steps, double *rrn_end,
                                                double *pw_rro, double *rrn)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();
+    drop_w_settle_rest();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     if (inner_steps < 0 || inner_steps > (int)g.ch_a.size()) {
         die("ppcg_end_begin_ur_inner: %d inner steps, %d coefficients uploaded", inner_steps, (int)g.ch_a.size());

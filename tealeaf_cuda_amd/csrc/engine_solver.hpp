@@ -1138,7 +1138,8 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int wit
 }
 
 // k_ppcg_begin for FP32 fields with two columns per lane (8-byte accesses);
-// tiles of 2*TILE_X x TILE_Y cells.  The same arithmetic cell by cell.
+// tiles of 2*TILE_X x TILE_Y cells.  The same arithmetic cell by cell.  w
+// null: w is not stored (k_ppcg_ur_sd_x2_wp forms it again from p').
 __global__ __launch_bounds__(BLOCK) void k_ppcg_begin_x2(Ext e, long pitch, int with_p, double beta_in,
                                                          const double *__restrict__ beta_num,
                                                          const float *__restrict__ p, float *__restrict__ pn,
@@ -1200,13 +1201,13 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin_x2(Ext e, long pitch, int 
                                      - (ky1.y * col[1][k + 2] + ky0.y * col[1][k])
                                      - (kxr * right + kx.y * col[0][k + 1]);
                     *(float2 *)(pn + i) = make_float2(col[0][k + 1], col[1][k + 1]);
-                    *(float2 *)(w + i) = make_float2(w0, w1);
+                    if (w) *(float2 *)(w + i) = make_float2(w0, w1);
                     *(float2 *)(rstore + i) = rv;
                     dd_add_prod(acc[0], (double)col[1][k + 1], (double)w1);
                     dd_add_prod(acc[1], (double)rv.y, (double)zs[1][k]);
                 } else {
                     pn[i] = col[0][k + 1];
-                    w[i] = w0;
+                    if (w) w[i] = w0;
                     rstore[i] = rv.x;
                 }
             }
@@ -1310,6 +1311,117 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2(Ext e, long pitch, doub
                     }
                 } else {
                     if (PREC != PREC_NONE) z[i] = src[0];
+                    sd[i] = sdv[0];
+                    if (rtemp) {
+                        rtemp[i] = rv[0][k];
+                        utemp[i] = sdv[0];
+                    }
+                }
+            }
+        }
+    }
+    block_reduce<1>(acc, red);
+}
+
+// k_ppcg_ur_sd_x2<PREC_BLOCK> with w = A p formed here, by the expression of
+// k_ppcg_begin_x2, instead of read (begin then leaves w unwritten): p down
+// both columns of the strip and the rows above and below it, x neighbours from
+// the adjacent lanes, the edge lanes' from memory.  p is read, not written, so
+// neighbours need no spare buffer; the cells' values are those of
+// k_ppcg_begin_x2 followed by k_ppcg_ur_sd_x2<PREC_BLOCK>.
+__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2_wp(Ext e, long pitch, double alpha_in,
+                                                            const double *__restrict__ pw_rro, double theta_in,
+                                                            float *__restrict__ u, const float *__restrict__ p,
+                                                            float *__restrict__ r, float *__restrict__ z,
+                                                            const float *__restrict__ Kx, const float *__restrict__ Ky,
+                                                            float *__restrict__ sd, float *__restrict__ rtemp,
+                                                            float *__restrict__ utemp, Red red)
+{
+    const float alpha = (float)(pw_rro ? pw_rro[1] / pw_rro[0] : alpha_in);
+    const float theta_r = 1.0f / (float)theta_in;
+    const int lane = threadIdx.x & (WAVE - 1);
+    DD acc[1] = {{0, 0}};
+    for (TileLoopT<2 * TILE_X> tl_(e); tl_.more(); tl_.advance()) {
+        const Strip s = tl_.strip();
+        const int c = s.c;
+        const long i0 = (long)c + (long)s.r0 * pitch;
+        int act[2] = {s.len, (c + 1 <= e.x1) ? s.len : 0};
+        const bool rlane = lane < WAVE - 1 && c + 2 <= e.x1;
+        float col[2][STRIP + 2], ky[2][STRIP + 1];
+#pragma unroll
+        for (int q = 0; q < STRIP + 2; q++) {
+            col[0][q] = col[1][q] = 0.0f;
+            if (act[0] > 0 && q <= act[0] + 1) {
+                const float2 pp = *(const float2 *)(p + i0 + (long)(q - 1) * pitch);
+                col[0][q] = pp.x;
+                col[1][q] = pp.y;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) {
+            ky[0][k] = ky[1][k] = 0.0f;
+            if (act[0] > 0 && k <= act[0]) {
+                const float2 v = *(const float2 *)(Ky + i0 + (long)k * pitch);
+                ky[0][k] = v.x;
+                ky[1][k] = v.y;
+            }
+        }
+        float rv[2][STRIP], zv[2][STRIP], diag[2][STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const float from_left = __shfl_up(col[1][k + 1], 1, WAVE);
+            const float from_right = __shfl_down(col[0][k + 1], 1, WAVE);
+            rv[0][k] = rv[1][k] = zv[0][k] = zv[1][k] = 0.0f;
+            diag[0][k] = diag[1][k] = 1.0f;
+            if (k < act[0]) {
+                const long i = i0 + (long)k * pitch;
+                const float2 kx = *(const float2 *)(Kx + i);
+                const float left = lane > 0 ? from_left : p[i - 1];
+                const float w0 = (1.0f + (ky[0][k + 1] + ky[0][k]) + (kx.y + kx.x)) * col[0][k + 1]
+                                 - (ky[0][k + 1] * col[0][k + 2] + ky[0][k] * col[0][k])
+                                 - (kx.y * col[1][k + 1] + kx.x * left);
+                float2 uu = *(const float2 *)(u + i), rr = *(const float2 *)(r + i);
+                uu.x += alpha * col[0][k + 1];
+                rv[0][k] = rr.x - alpha * w0;
+                diag[0][k] = 1.0f + (ky[0][k + 1] + ky[0][k]) + (kx.y + kx.x);
+                if (k < act[1]) {
+                    const float kxr = Kx[i + 2];
+                    const float right = rlane ? from_right : p[i + 2];
+                    const float w1 = (1.0f + (ky[1][k + 1] + ky[1][k]) + (kxr + kx.y)) * col[1][k + 1]
+                                     - (ky[1][k + 1] * col[1][k + 2] + ky[1][k] * col[1][k])
+                                     - (kxr * right + kx.y * col[0][k + 1]);
+                    uu.y += alpha * col[1][k + 1];
+                    rv[1][k] = rr.y - alpha * w1;
+                    diag[1][k] = 1.0f + (ky[1][k + 1] + ky[1][k]) + (kxr + kx.y);
+                    *(float2 *)(u + i) = uu;
+                    *(float2 *)(r + i) = make_float2(rv[0][k], rv[1][k]);
+                } else {
+                    u[i] = uu.x;
+                    r[i] = rv[0][k];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (act[q] > 0) strip_solve_col(rv[q], zv[q], act[q], diag[q], ky[q]);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            if (k < act[0]) {
+                const long i = i0 + (long)k * pitch;
+                float sdv[2];
+#pragma unroll
+                for (int q = 0; q < 2; q++) sdv[q] = zv[q][k] * theta_r;
+                const int nq = k < act[1] ? 2 : 1;
+                for (int q = 0; q < nq; q++) dd_add_prod(acc[0], (double)rv[q][k], (double)zv[q][k]);
+                if (nq == 2) {
+                    *(float2 *)(z + i) = make_float2(zv[0][k], zv[1][k]);
+                    *(float2 *)(sd + i) = make_float2(sdv[0], sdv[1]);
+                    if (rtemp) {   // null: rtemp = r and utemp = sd left to the first inner pass
+                        *(float2 *)(rtemp + i) = make_float2(rv[0][k], rv[1][k]);
+                        *(float2 *)(utemp + i) = make_float2(sdv[0], sdv[1]);
+                    }
+                } else {
+                    z[i] = zv[0][k];
                     sd[i] = sdv[0];
                     if (rtemp) {
                         rtemp[i] = rv[0][k];

@@ -98,14 +98,16 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
         assert a.values() == b.values(), (a, b)
 
 
-@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE"])
+@pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
+                                    "TEA_W_FROM_P"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
     """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0), the fused
     outer iteration with one inner step per pass (TEA_INNER_PASSES=0), the
-    inner passes on the tile kernels (TEA_INNER_MARCH=0), and the refinement
+    inner passes on the tile kernels (TEA_INNER_MARCH=0), the refinement
     loop without inner passes queued ahead of the replacement test
-    (TEA_SPECULATE=0), on the GPU give the same results as the oracle."""
+    (TEA_SPECULATE=0), and the FP32 fused pair with w written by begin and read
+    by ur (TEA_W_FROM_P=0), on the GPU give the same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)

@@ -1330,6 +1330,15 @@ constexpr int INNER_WAVES = 16;
 // their sum.
 constexpr double PASS_US_F32[MAX_PASS_STEPS] = {132, 140, 158, 166, 198};
 constexpr double PASS_US_F64[MAX_PASS_STEPS] = {230, 237, 269, 322, 766};
+// With jac_diag the five-step FP32 pass needs more than 256 registers (one
+// wave per SIMD), so plans leave it out.
+constexpr double PASS_US_F32_DIAG[MAX_PASS_STEPS] = {132, 140, 158, 166, 1e9};
+
+const double *pass_costs()
+{
+    if (!g.fp32) return PASS_US_F64;
+    return g.prec == PREC_DIAG ? PASS_US_F32_DIAG : PASS_US_F32;
+}
 
 // The streaming pass (k_ppcg_inner_march) is the faster kernel for every NS
 // in both precisions (measured against k_ppcg_inner_n[_x2]); it addresses
@@ -1637,7 +1646,7 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
     if (inner_steps > 0) {
         if (hipEventRecord(g.ev_scal, g.stream) != hipSuccess) { die("hipEventRecord failed"); return; }
         for (int step = 1; step <= inner_steps;) {
-            const int n = plan_first_pass(inner_steps - step + 1, g.fp32 ? PASS_US_F32 : PASS_US_F64);
+            const int n = plan_first_pass(inner_steps - step + 1, pass_costs());
             ppcg_inner_n_impl<T>(step - 1, n);
             step += n;
         }
@@ -1947,7 +1956,7 @@ EXPORT void tea_engine_ppcg_inner2(int step) { tea_engine_ppcg_inner_n(step, 2);
 // (the first pass of the cheapest plan for the current precision mode).
 EXPORT int tea_engine_ppcg_inner_plan(int remaining)
 {
-    return plan_first_pass(remaining, g.fp32 ? PASS_US_F32 : PASS_US_F64);
+    return plan_first_pass(remaining, pass_costs());
 }
 
 EXPORT void tea_engine_set_deck_path(const char *path) { g_deck_path = path ? path : "tea.in"; }

@@ -151,9 +151,10 @@ struct Chunk {
     // the field precision in bits): the next streaming inner pass reads r and
     // sd in their place; any other entry point writes them out first.
     int pending_rt_ut = 0;
-    // FP32 w = A p owed by the fused begin/ur pair (k_ppcg_ur_sd_x2_wp forms w
-    // from p instead of reading it); written out by settle_pending
-    bool pending_w = false;
+    // w = A p owed by the fused begin/ur pair (k_ppcg_ur_sd_x2_wp and
+    // k_ppcg_ur_sd_wp form w from p instead of reading it), 32 or 64 (the
+    // precision of the fields), 0 for none; written out by settle_pending
+    int pending_w = 0;
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -1211,11 +1212,14 @@ void settle_rt_ut()
 void settle_pending()
 {
     settle_rt_ut();
-    if (g.pending_w) {   // w = A p as cg_calc_w forms it (its p.w is not used)
-        g.pending_w = false;
+    const int wbits = g.pending_w;   // w = A p as cg_calc_w forms it (its p.w is not used)
+    g.pending_w = 0;
+    if (wbits == 32)
         launch("settle_w_f32", k_cg_calc_w<float>, interior(), g.pitch32, (const float *)g.fs32[F_P], g.fs32[X_W],
                (const float *)g.fs32[X_KX], (const float *)g.fs32[X_KY], red_ctx());
-    }
+    else if (wbits == 64)
+        launch("settle_w", k_cg_calc_w<double>, interior(), g.pitch, (const double *)g.fs[F_P], g.fs[X_W],
+               (const double *)g.fs[X_KX], (const double *)g.fs[X_KY], red_ctx());
 }
 
 #define PREC_SWITCH_T(T, PREC_VALUE, NAME, KERNEL, ...)                                           \
@@ -1524,15 +1528,15 @@ template <typename T> void ppcg_calc_p_impl(double beta)
 // Fused outer iteration (see k_ppcg_begin / k_ppcg_ur_sd / k_ppcg_end).
 // beta_num: null (beta given), or device memory whose value over beta is the
 // beta to apply (formed on the device as the host would form it).
-// FP32 block-Jacobi fused pair: begin leaves w = A p' unwritten and ur forms
-// it again from p' (k_ppcg_ur_sd_x2_wp), unless TEA_W_FROM_P=0.
+// Block-Jacobi fused pair: begin leaves w = A p' unwritten and ur forms it
+// again from p' (k_ppcg_ur_sd_x2_wp, k_ppcg_ur_sd_wp), unless TEA_W_FROM_P=0.
 template <typename T> bool w_from_p()
 {
     const char *s = getenv("TEA_W_FROM_P");
-    return std::is_same<T, float>::value && g.prec == PREC_BLOCK && !(s && s[0] == '0');
+    return g.prec == PREC_BLOCK && !(s && s[0] == '0');
 }
 
-// z_src: where z is read (null: the z field); w_later: w not stored (FP32).
+// z_src: where z is read (null: the z field); w_later: w not stored.
 template <typename T>
 bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr, const T *z_src = nullptr,
                        bool w_later = false)
@@ -1554,7 +1558,7 @@ bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr
                        (const T *)FT<T>(X_KY), (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     else
         ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
-                    FT<T>(X_P2), zf, FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
+                    FT<T>(X_P2), zf, w_later ? nullptr : FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
                     (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     if (!ok) return false;
     if (std::is_same<T, double>::value) {
@@ -1574,7 +1578,7 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
 // pw_rro: null (alpha given) or device memory holding p.w, r.z.
 // defer: leave rtemp = r and utemp = sd to the next streaming inner pass
 // (recorded in g.pending_rt_ut).
-// w_from_p: w formed from p (FP32 block Jacobi; begin left it unwritten).
+// w_from_p: w formed from p (block Jacobi; begin left it unwritten).
 template <typename T>
 bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer = false, bool w_from_p = false)
 {
@@ -1597,6 +1601,10 @@ bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer
         ok = g.prec == PREC_BLOCK ? go(k_ppcg_ur_sd_x2<PREC_BLOCK>)
              : g.prec == PREC_DIAG ? go(k_ppcg_ur_sd_x2<PREC_DIAG>)
                                    : go(k_ppcg_ur_sd_x2<PREC_NONE>);
+    } else if (w_from_p && g.prec == PREC_BLOCK) {
+        ok = launch("k_ppcg_ur_sd", k_ppcg_ur_sd_wp<T>, interior(), pitch_of<T>(), alpha, pw_rro, theta, FT<T>(F_U),
+                    (const T *)FT<T>(F_P), FT<T>(F_R), FT<T>(X_Z), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
+                    FT<T>(F_SD), rt, ut, red_ctx());
     } else {
         ok = PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, pw_rro, theta,
                            FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
@@ -1618,7 +1626,7 @@ template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double th
     const bool wp = w_from_p<T>();
     if (!ppcg_begin_launch<T>(with_p, beta, nullptr, nullptr, wp) || !finish(0, 2, g.d_scal)) return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
-    g.pending_w = wp;
+    g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
     if (collect(pw_rro, 0, 2)) *rrn = g.h_out[2];
 }
 
@@ -1642,7 +1650,7 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
     if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3, z_from_utemp ? ut : nullptr, wp) || !finish(0, 2, g.d_scal))
         return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
-    g.pending_w = wp;
+    g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
     if (inner_steps > 0) {
         if (hipEventRecord(g.ev_scal, g.stream) != hipSuccess) { die("hipEventRecord failed"); return; }
         for (int step = 1; step <= inner_steps;) {
@@ -1798,7 +1806,7 @@ static void drop_or_settle_pending()
 // written out.
 static void drop_w_settle_rest()
 {
-    g.pending_w = false;
+    g.pending_w = 0;
     settle_rt_ut();
 }
 
@@ -1925,7 +1933,8 @@ EXPORT void tea_engine_ppcg_end_begin_ur_inner(double rro_prev, double theta, in
 
 EXPORT void tea_engine_ppcg_end(double *rrn)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();   // reads neither p nor w: an owed w stays owed
+    settle_rt_ut();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     BY_PRECISION(ppcg_end_impl, rrn);
 }

@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int wit
                          - (Ky[i + pitch] * col[k + 2] + Ky[i] * col[k])
                          - (Kx[i + 1] * right + Kx[i] * left);
             pn[i] = pc;
-            w[i] = wv;
+            if (w) w[i] = wv;
             const T rv = r[i];
             rstore[i] = rv;
             dd_add_prod(acc[0], (double)pc, (double)wv);
@@ -1431,6 +1431,69 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2_wp(Ext e, long pitch, d
             }
         }
     }
+    block_reduce<1>(acc, red);
+}
+
+// k_ppcg_ur_sd<T, PREC_BLOCK> with w = A p formed here, by the expression of
+// k_ppcg_begin (which then leaves w unwritten), instead of read: p down the
+// lane's column (the strip and the rows above and below it), x neighbours
+// from the adjacent lanes, the edge lanes' from memory.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_wp(Ext e, long pitch, double alpha_in,
+                                                         const double *__restrict__ pw_rro, double theta_in,
+                                                         T *__restrict__ u, const T *__restrict__ p,
+                                                         T *__restrict__ r, T *__restrict__ z,
+                                                         const T *__restrict__ Kx, const T *__restrict__ Ky,
+                                                         T *__restrict__ sd, T *__restrict__ rtemp,
+                                                         T *__restrict__ utemp, Red red)
+{
+    const T alpha = (T)(pw_rro ? pw_rro[1] / pw_rro[0] : alpha_in);
+    const T theta_r = T(1.0) / (T)theta_in;
+    const int lane = threadIdx.x & (WAVE - 1);
+    DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
+    const int c = s.c;
+    T col[STRIP + 2];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; q++) {
+        col[q] = T(0.0);
+        if (s.len > 0 && q <= s.len + 1) col[q] = p[i0 + (long)(q - 1) * pitch];
+    }
+    const bool rlane = lane < WAVE - 1 && c + 1 <= e.x1;
+    T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        const T from_left = __shfl_up(col[k + 1], 1, WAVE);
+        const T from_right = __shfl_down(col[k + 1], 1, WAVE);
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const T left = lane > 0 ? from_left : p[i - 1];
+            const T right = rlane ? from_right : p[i + 1];
+            const T pc = col[k + 1];
+            const T wv = (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * pc
+                         - (Ky[i + pitch] * col[k + 2] + Ky[i] * col[k])
+                         - (Kx[i + 1] * right + Kx[i] * left);
+            u[i] += alpha * pc;
+            rv[k] = r[i] - alpha * wv;
+            r[i] = rv[k];
+        }
+    }
+    precond_strip<T, PREC_BLOCK>(rv, zv, s.len, i0, pitch, (const T *)nullptr, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            z[i] = zv[k];
+            dd_add_prod(acc[0], (double)rv[k], (double)zv[k]);
+            const T sdv = zv[k] * theta_r;
+            sd[i] = sdv;
+            if (rtemp) {   // null: rtemp = r and utemp = sd left to the first inner pass
+                rtemp[i] = rv[k];
+                utemp[i] = sdv;
+            }
+        }
+    }
+    STRIP_END
     block_reduce<1>(acc, red);
 }
 

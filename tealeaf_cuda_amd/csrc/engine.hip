@@ -352,20 +352,42 @@ bool launch(const char *name, K kernel, const Ext &e, A... args)
     return true;
 }
 
-// The same for kernels on tiles of 2*TILE_X x TILE_Y cells (FP32 kernels whose
-// lanes hold two columns).
+// The same with at most `cap` workgroups, each then looping over several
+// tiles (measured faster for a few of the streaming reductions).
 template <typename K, typename... A>
-bool launch_x2(const char *name, K kernel, const Ext &e, A... args)
+bool launch_capped(const char *name, int cap, K kernel, const Ext &e, A... args)
 {
-    const int nbx = (e.x1 - e.x0 + 2 * TILE_X) / (2 * TILE_X), nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
-    int nb = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : nbx * nby;
-    nb = nb < grid_max() ? nb : grid_max();
+    int nb = grid_for(e);
+    nb = nb < cap ? nb : cap;
     g.last_grid = nb;
     if (nb == 0) return true;
     klaunch(name, kernel, dim3(nb), dim3(BLOCK), e, args...);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
     return true;
+}
+
+// The same for kernels on tiles of 2*TILE_X x TILE_Y cells (FP32 kernels whose
+// lanes hold two columns); cap as for launch_capped.
+template <typename K, typename... A>
+bool launch_x2_capped(const char *name, int cap, K kernel, const Ext &e, A... args)
+{
+    const int nbx = (e.x1 - e.x0 + 2 * TILE_X) / (2 * TILE_X), nby = (e.y1 - e.y0 + TILE_Y) / TILE_Y;
+    int nb = (e.x1 < e.x0 || e.y1 < e.y0) ? 0 : nbx * nby;
+    nb = nb < grid_max() ? nb : grid_max();
+    nb = nb < cap ? nb : cap;
+    g.last_grid = nb;
+    if (nb == 0) return true;
+    klaunch(name, kernel, dim3(nb), dim3(BLOCK), e, args...);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
+    return true;
+}
+
+template <typename K, typename... A>
+bool launch_x2(const char *name, K kernel, const Ext &e, A... args)
+{
+    return launch_x2_capped(name, 1 << 30, kernel, e, args...);
 }
 
 // Combines the partials of the last reduction launch on the device into
@@ -1586,9 +1608,9 @@ bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer
     bool ok;
     if constexpr (std::is_same<T, float>::value) {
         if (w_from_p && g.prec == PREC_BLOCK) {
-            ok = launch_x2("k_ppcg_ur_sd_f32", k_ppcg_ur_sd_x2_wp, interior(), pitch_of<T>(), alpha, pw_rro, theta,
-                           FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), FT<T>(X_Z), (const T *)FT<T>(X_KX),
-                           (const T *)FT<T>(X_KY), FT<T>(F_SD), rt, ut, red_ctx());
+            ok = launch_x2_capped("k_ppcg_ur_sd_f32", 4096, k_ppcg_ur_sd_x2_wp, interior(), pitch_of<T>(), alpha,
+                                  pw_rro, theta, FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), FT<T>(X_Z),
+                                  (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD), rt, ut, red_ctx());
             if (ok && defer) g.pending_rt_ut = 32;
             return ok;
         }
@@ -1642,8 +1664,9 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
 {
     const bool z_from_utemp = g.prec != PREC_NONE;
     const T *ut = FT<T>(X_UTEMP);
-    if (!launch(nm<T>("ppcg_end"), k_ppcg_end<T>, interior(), pitch_of<T>(), ut, z_from_utemp ? nullptr : FT<T>(X_Z),
-                (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()) ||
+    if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), ut,
+                       z_from_utemp ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE),
+                       red_ctx()) ||
         !finish(3, 1, g.d_scal + 3))
         return;
     const bool wp = w_from_p<T>();

@@ -287,7 +287,8 @@ bool fused_inner()
     return v;
 }
 
-// One workgroup per tile measured fastest on MI355X for every kernel here;
+// One workgroup per tile measured fastest on MI355X for most kernels here (the
+// exceptions use launch_capped / launch_x2_capped);
 // TEA_GRID_MAX caps the grid (workgroups then loop over several tiles).
 int grid_max()
 {

@@ -914,6 +914,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
     const bool may_queue_inner = defer_end && b.ppcg_end_begin_ur_inner && b.ppcg_inner_n && b.ppcg_inner_plan &&
                                  d.tl_ppcg_inner_steps > 0 && !(passes_env && passes_env[0] == '0') &&
                                  !(spec_env && spec_env[0] == '0');
+    const bool queue_always = spec_env && spec_env[0] == '2';   // every time it may (tests the undo)
     int fields[NUM_FIELDS];
     std::fill(fields, fields + NUM_FIELDS, 0);
 
@@ -1024,7 +1025,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                 bool inner_queued = false;
                 if (end_pending) {   // the previous iteration's end, then begin with beta formed on the device
                     double rrn_end = 0.0;
-                    if (may_queue_inner && replacement_unlikely()) {
+                    if (may_queue_inner && (queue_always || replacement_unlikely())) {
                         b.ppcg_end_begin_ur_inner(rro_begin, theta, d.tl_ppcg_inner_steps, &rrn_end, two, &rrn);
                         inner_queued = true;
                     } else {

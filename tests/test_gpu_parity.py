@@ -120,6 +120,22 @@ def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracl
         assert a.values() == b.values(), (a, b)
 
 
+def test_deck_bitwise_inner_passes_always_queued(oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
+    """TEA_SPECULATE=2: the refinement loop queues the inner passes ahead of
+    every replacement test, so every replacement overwrites queued passes;
+    the GPU still gives the oracle's results."""
+    oracle_env("dot2")
+    monkeypatch.setenv("TEA_SPECULATE", "2")
+    print("[gpu parity] deck mixed_ppcg_block, inner passes always queued", flush=True)
+    path = _deck(**DECKS["mixed_ppcg_block"]).write(tmp_path / "tea.in")
+    gpu = run_deck(path)
+    cpu = run_deck(path, backend=oracle_so)
+    assert gpu.iterations == cpu.iterations and gpu.refinements == cpu.refinements
+    assert gpu.exact_error == cpu.exact_error
+    for a, b in zip(gpu.summaries, cpu.summaries):
+        assert a.values() == b.values(), (a, b)
+
+
 # ------------------------------------------------------------------ entry points
 
 def _prepare(eng, tmp_path, solver, prec, nx=70, ny=45, halo=2):

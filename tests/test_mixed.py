@@ -231,6 +231,34 @@ def test_reliable_keeps_the_krylov_process(oracle_so, oracle_env, tmp_path):
     assert all(k >= 1 for k in r64.refinements)
 
 
+@pytest.mark.parametrize("bits", ["fp32", "fp64"])
+def test_queued_inner_passes_change_nothing(bits, oracle_so, oracle_env, monkeypatch, tmp_path):
+    """The residual-replacement loop may queue an outer iteration's inner
+    passes before its replacement test; a replacement after all overwrites what
+    they wrote and runs them again.  Queued when no replacement is expected
+    (default), never (TEA_SPECULATE=0) and always (TEA_SPECULATE=2, so every
+    replacement undoes queued passes): the same iterations, replacements,
+    residuals and summaries."""
+    oracle_env("dot2")
+    base = dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10, inner_precision=bits)
+    runs = {}
+    for mode in ("default", "0", "2"):
+        if mode == "default":
+            monkeypatch.delenv("TEA_SPECULATE", raising=False)
+        else:
+            monkeypatch.setenv("TEA_SPECULATE", mode)
+        runs[mode] = _run(tmp_path, oracle_so, f"spec{mode}", **base)
+    ref = runs["0"]
+    assert sum(ref.refinements) >= 2 * len(ref.refinements)
+    for mode in ("default", "2"):
+        r = runs[mode]
+        assert r.iterations == ref.iterations and r.ppcg_inner == ref.ppcg_inner
+        assert r.refinements == ref.refinements
+        assert r.exact_error == ref.exact_error
+        for a, b in zip(r.summaries, ref.summaries):
+            assert a.values() == b.values()
+
+
 @pytest.mark.parametrize("kw,msg", [
     (dict(solver="jacobi", preconditioner="none"), "tl_inner_precision"),
     (dict(solver="ppcg", preconditioner="none", reflective_boundary=True), "reflective"),

@@ -1274,9 +1274,16 @@ Run::Solve Run::refine_reliable(double initial_residual_sq, double target_residu
 // sd, rtemp and utemp (tea_engine_mp_replace forms them itself).
 // Whether the next reliable_step can be expected to replace nothing: no
 // reference level yet, or the last two measures, extrapolated geometrically,
-// keep the drop and the predicted residual twice their thresholds.
+// keep the drop and the predicted residual a margin (1.5, or TEA_SPEC_MARGIN)
+// above their thresholds (1.25 and 1.5 measured best on the 4096^2 bench
+// deck, 0.5% ahead of 2).
 bool Run::replacement_unlikely() const
 {
+    static const double margin = [] {
+        const char *s = std::getenv("TEA_SPEC_MARGIN");
+        const double v = s ? std::atof(s) : 0.0;
+        return v >= 1.0 ? v : 1.5;
+    }();
     if (!rel.on) return false;
     if (!rel.have_q) return true;
     if (rel.nq < 2 || !(std::fabs(rel.q_prev) > 0.0) || !(std::fabs(rel.q_ref) > 0.0)) return false;
@@ -1284,7 +1291,7 @@ bool Run::replacement_unlikely() const
     const double q_next = std::fabs(rel.q_last) * (ratio < 1.0 ? ratio : 1.0);
     const double drop = std::sqrt(q_next / std::fabs(rel.q_ref));
     const double predicted = std::sqrt(std::fabs(rel.rr_ref)) * drop;
-    return drop >= 2.0 * d.reliable_delta && predicted >= 2.0 * rel.target;
+    return drop >= margin * d.reliable_delta && predicted >= margin * rel.target;
 }
 
 bool Run::reliable_step(double q, bool refresh_z, bool *replaced)

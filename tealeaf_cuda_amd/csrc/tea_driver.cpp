@@ -178,7 +178,7 @@ struct Deck {
     // read_input.f90 ignores these keys)
     int refine_bits = 0;         // tl_inner_precision fp32|fp64; 0: the reference's solve
     bool restart = false;        // tl_mixed_refinement restart|reliable
-    double reliable_delta = 0.1; // tl_reliable_delta: residual drop that triggers a replacement
+    double reliable_delta = 0.01; // tl_reliable_delta: residual drop that triggers a replacement
     double inner_eps = 1.0e-4;   // tl_inner_eps: restart scheme's inner tolerance (relative)
     int max_refine = 30;         // tl_max_refinements: restart scheme's limit
     std::string warm_start;      // tl_warm_start <weight file>: learned initial guess

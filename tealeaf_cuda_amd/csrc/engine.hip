@@ -1694,8 +1694,8 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
 
 template <typename T> void ppcg_end_impl(double *rrn)
 {
-    if (!launch(nm<T>("ppcg_end"), k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP), FT<T>(X_Z),
-                (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()))
+    if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP),
+                       FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()))
         return;
     fetch(rrn, 1);
 }

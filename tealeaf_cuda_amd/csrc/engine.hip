@@ -155,6 +155,9 @@ struct Chunk {
     // k_ppcg_ur_sd_wp form w from p instead of reading it), 32 or 64 (the
     // precision of the fields), 0 for none; written out by settle_pending
     int pending_w = 0;
+    // z = utemp owed by a separate ppcg_end with a preconditioner (the next
+    // fused begin reads utemp in its place and ur rewrites z), 32 or 64, or 0
+    int pending_z = 0;
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -1223,6 +1226,17 @@ template <typename T> void settle_fields()
     launch(nm<T>("settle_utemp"), k_copy<T>, in, pitch_of<T>(), FT<T>(X_UTEMP), (const T *)FT<T>(F_SD));
 }
 
+// z = utemp owed (only)
+void settle_z()
+{
+    const int bits = g.pending_z;
+    g.pending_z = 0;
+    if (bits == 32)
+        launch("settle_z_f32", k_copy<float>, interior(), g.pitch32, g.fs32[X_Z], (const float *)g.fs32[X_UTEMP]);
+    else if (bits == 64)
+        launch("settle_z", k_copy<double>, interior(), g.pitch, g.fs[X_Z], (const double *)g.fs[X_UTEMP]);
+}
+
 // rtemp and utemp owed (only)
 void settle_rt_ut()
 {
@@ -1234,6 +1248,7 @@ void settle_rt_ut()
 
 void settle_pending()
 {
+    settle_z();   // before the rtemp/utemp debt rewrites utemp
     settle_rt_ut();
     const int wbits = g.pending_w;   // w = A p as cg_calc_w forms it (its p.w is not used)
     g.pending_w = 0;
@@ -1644,12 +1659,15 @@ template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
 }
 
 // begin, then ur with alpha = rro / pw formed on the device: one wait.
+// z owed by a separate end (= utemp): begin reads utemp, ur rewrites z.
 template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double theta, double *pw_rro, double *rrn)
 {
     const bool wp = w_from_p<T>();
-    if (!ppcg_begin_launch<T>(with_p, beta, nullptr, nullptr, wp) || !finish(0, 2, g.d_scal)) return;
+    const T *z_src = g.pending_z == 8 * (int)sizeof(T) ? (const T *)FT<T>(X_UTEMP) : nullptr;
+    if (!ppcg_begin_launch<T>(with_p, beta, nullptr, z_src, wp) || !finish(0, 2, g.d_scal)) return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
     g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
+    g.pending_z = 0;
     if (collect(pw_rro, 0, 2)) *rrn = g.h_out[2];
 }
 
@@ -1675,6 +1693,7 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
         return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
     g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
+    g.pending_z = 0;   // end recomputed what z owed; ur rewrote z
     if (inner_steps > 0) {
         if (hipEventRecord(g.ev_scal, g.stream) != hipSuccess) { die("hipEventRecord failed"); return; }
         for (int step = 1; step <= inner_steps;) {
@@ -1692,11 +1711,15 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
     *rrn_end = g.h_out[3];
 }
 
+// With a preconditioner z = utemp is owed rather than stored (the fused
+// begin that follows reads utemp, and ur rewrites z).
 template <typename T> void ppcg_end_impl(double *rrn)
 {
+    const bool owe_z = g.prec != PREC_NONE;
     if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP),
-                       FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()))
+                       owe_z ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()))
         return;
+    g.pending_z = owe_z ? 8 * (int)sizeof(T) : 0;
     fetch(rrn, 1);
 }
 
@@ -1822,6 +1845,7 @@ EXPORT void tea_leaf_ppcg_init_kernel_cuda_(const int *step, double *rro)
 // Both overwrite sd, rtemp and utemp: rtemp and utemp still owed need no writing.
 static void drop_or_settle_pending()
 {
+    settle_z();   // init_sd_new reads z
     if (g.pending_rt_ut == (g.fp32 ? 32 : 64)) g.pending_rt_ut = 0;
     else settle_rt_ut();
 }
@@ -1969,6 +1993,7 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
 EXPORT void tea_engine_ppcg_inner_n(int step, int nsteps)
 {
     CHECK_INIT_KEEP_PENDING();   // a streaming pass takes rtemp and utemp still owed from r and sd
+    settle_z();                  // the pass rewrites utemp
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     const int k = step - 1;
     if (nsteps < 1 || nsteps > MAX_PASS_STEPS) {

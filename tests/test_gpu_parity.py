@@ -209,6 +209,10 @@ PPCG_CALLS = {
     "fused_ppcg_begin_ur_p": lambda e: e.ppcg_begin_ur(True, 0.45, 1.7),
     "fused_ppcg_begin_ur": lambda e: e.ppcg_begin_ur(False, 0.0, 1.7),
     "fused_ppcg_end_begin_ur": lambda e: e.ppcg_end_begin_ur(0.8, 1.7),
+    "fused_ppcg_end_then_begin_ur": lambda e: (e.ppcg_end(), e.ppcg_begin_ur(True, 0.45, 1.7)),
+    "fused_ppcg_end_then_inner_n2": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_end(), e.ppcg_inner_n(1, 2)),
     "fused_ppcg_end_begin_ur_inner6": lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_end_begin_ur_inner(0.8, 1.7, 6)),

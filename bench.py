@@ -30,9 +30,15 @@ inner-step passes, the dominant kernels, inside it for the roofline figure.
 Two further steps, outside the timed region, time every launch for the
 per-kernel breakdown.
 
-The CPU baseline runs the plain FP64 deck through the same driver over the CPU
-oracle (OpenMP, plain reductions) on a bounded sample of PPCG iterations and
-extrapolates to the GPU plain run's iteration counts.
+The CPU baseline runs the plain FP64 deck for three whole timesteps through
+the reference's own Fortran driver (compiled from its sources into
+oracle/_ref) over the CPU oracle (OpenMP, plain reductions) on the job's CPU
+cores, and reports ms per timestep over steps 2 and 3 from the driver's own
+wall clock.
+
+`roofline` is the dominant kernel's compulsory HBM bytes per launch over its
+average launch duration; `step_roofline` the compulsory bytes of every launch
+of a timestep over the time per timestep.
 """
 from __future__ import annotations
 
@@ -50,25 +56,43 @@ sys.path.insert(0, str(REPO))
 METRIC = "ms/timestep to FP64 rel-res 1e-10 at 4096²: learned+FP32 vs plain FP64 PPCG"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per interior cell of each kernel launch: every distinct
-# FP64 array the reference kernel reads or writes, once, 8 bytes each.
-ALGO_BYTES = {
-    # one whole inner step (the reference's update_r + calc_sd_new launches, jac_block)
+# Bytes per interior cell of each launch counted as the reference's separate
+# kernels would move them: every distinct FP64 array the reference kernel reads
+# or writes, once, 8 bytes each (4 in FP32), and a pass of n inner steps
+# charged n times.  Reported as `effective_gbs`; above the HBM peak for a
+# temporally blocked pass, so it is no roofline fraction.
+REF_STEP_BYTES = {
     "k_ppcg_inner_fused": 112,
     "k_ppcg_inner_fused_f32": 56,     # the same arrays in FP32
-    # n whole inner steps per launch (temporally blocked): n times the reference's bytes
     **{f"k_ppcg_inner{n}": 112 * n for n in range(1, 6)},
     **{f"k_ppcg_inner{n}_f32": 56 * n for n in range(1, 6)},
-    "ppcg_update_r": 40,              # rtemp, sd, Kx, Ky read; rtemp written
-    "k_ppcg_calc_sd": 72,             # jac_block: rtemp, sd, utemp, cp, bfp, Kx, Ky read; sd, utemp written
-    "cg_calc_w": 32,                  # p, Kx, Ky read; w written
-    "k_cg_calc_ur": 88,               # jac_block: u, p, r, w, cp, bfp, Kx, Ky read; u, r, z written
-    "ppcg_calc_rrn": 24,
-    "ppcg_calc_2norm": 16,
-    "ppcg_store_r": 16,
-    "ppcg_update_z": 16,
-    "ppcg_calc_p": 24,
-    "ppcg_init_sd_new": 40,           # r, z read; sd, rtemp, utemp written
+}
+
+# Compulsory HBM bytes per interior cell of one launch of each kernel as this
+# engine runs it with the block-Jacobi preconditioner: every distinct array the
+# launch must read or write, once (4 bytes per FP32 cell, 8 per FP64 cell).
+# The roofline fraction of a launch is these bytes over its duration; a
+# kernel absent here counts 0 bytes in the step-level figure (a lower bound).
+COMPULSORY_BYTES = {
+    # inner-step pass (any number of steps): sd, rtemp, utemp, Kx, Ky read; sd, rtemp, utemp written
+    **{f"k_ppcg_inner{n}_f32": 32 for n in range(1, 6)},
+    **{f"k_ppcg_inner{n}": 64 for n in range(1, 6)},
+    "k_ppcg_inner_fused_f32": 32, "k_ppcg_inner_fused": 64,
+    # fused begin: p, z (utemp), Kx, Ky, r read; p', r_store written (w formed again by ur)
+    "ppcg_begin_f32": 28, "ppcg_begin": 56,
+    # fused ur + init_sd_new: u, p', r, Kx, Ky read; u, r, z, sd written (rtemp, utemp left to the pass)
+    "k_ppcg_ur_sd_f32": 36, "k_ppcg_ur_sd": 72,
+    # fused end: utemp, r, r_store read (z owed to the next begin)
+    "ppcg_end_f32": 12, "ppcg_end": 24,
+    # CG iteration kernels: w = Ap (p, Kx, Ky read; w written); u/r/z update (u, p, r, w, Kx, Ky read;
+    # u, r, z written); p = z + beta p (z, p read; p written)
+    "cg_calc_w_f32": 16, "cg_calc_w": 32,
+    "k_cg_calc_ur_f32": 36, "k_cg_calc_ur": 72,
+    "cg_calc_p_f32": 12, "cg_calc_p": 24, "ppcg_calc_p_f32": 12, "ppcg_calc_p": 24,
+    # residual replacement in one pass: u, u0, Kx, Ky (FP64), u32, Kx32, Ky32 read;
+    # u, r (FP64), r32, u0_32, z32, sd32, u32 written
+    "mp_replace": 8 * 4 + 4 * 3 + 8 * 2 + 4 * 5,
+    "calc_residual": 40, "calc_2norm": 8, "calc_2norm_f32": 4,
 }
 
 
@@ -177,7 +201,8 @@ def measured_traffic(kernel):
 
 
 def roofline(profile, n):
-    """Dominant kernel (largest total time) with its algorithmic bandwidth."""
+    """Dominant kernel (largest total time): its compulsory bytes per launch
+    over its average launch duration, against the HBM peak."""
     best = None
     for name, (count, ms) in profile.items():
         if count and (best is None or ms > best[2]):
@@ -185,63 +210,148 @@ def roofline(profile, n):
     if best is None:
         return None, None
     name, count, ms = best
-    per_cell = ALGO_BYTES.get(name)
     avg_us = ms * 1e3 / count
     out = {"kernel": name, "launches": count, "avg_us": round(avg_us, 3)}
+    per_cell = COMPULSORY_BYTES.get(name)
     if per_cell is None:
         return out, None
     bytes_per_launch = per_cell * n * n
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    traffic = measured_traffic(name) if n == 4096 else None
     roof = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(name) if n == 4096 else None,
+        "traffic": traffic,
         "kernel": name,
         "bytes_per_launch": bytes_per_launch,
+        "compulsory_bytes_per_cell": per_cell,
         "avg_launch_us": round(avg_us, 3),
+        # HBM bytes the counters measured per launch, over the same duration
+        "traffic_gbs": round(traffic / (avg_us * 1e-6) / 1e9, 1) if traffic else None,
+        # the reference's separate-launch bytes of all the pass's steps (not a roofline fraction)
+        "effective_gbs": round(REF_STEP_BYTES[name] * n * n / (avg_us * 1e-6) / 1e9, 1)
+        if name in REF_STEP_BYTES else None,
     }
     return out, roof
 
 
-def cpu_baseline(n, gpu_res, timed, tmp):
-    """Oracle on the host cores: time two bounded first-step runs that differ
-    only in the number of PPCG iterations, take the per-iteration cost, and
-    scale it by the GPU run's outer iterations per timed step."""
+def step_roofline(breakdown, steps, ms_per_step, n):
+    """Compulsory HBM bytes of every launch of a timestep (launch counts from
+    the profiled steps) over the measured time per timestep: the step's
+    achieved fraction of the HBM peak (a lower bound: kernels without an
+    entry in COMPULSORY_BYTES count zero bytes)."""
+    total = 0.0
+    known_ms = 0.0
+    all_ms = 0.0
+    for name, (count, ms) in breakdown.items():
+        all_ms += ms
+        if name in COMPULSORY_BYTES:
+            total += COMPULSORY_BYTES[name] * n * n * count
+            known_ms += ms
+    per_step = total / steps
+    gbs = per_step / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_step": int(per_step), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "kernel_time_covered": round(known_ms / all_ms, 4) if all_ms else None}
+
+
+def cpu_share():
+    """CPU cores this job may use: the affinity mask, capped by a cgroup CPU
+    quota and by OMP_NUM_THREADS when the environment sets one (a shared GPU
+    host allots each job its share of the cores)."""
+    cores = len(os.sched_getaffinity(0))
+    info = {"os_cpu_count": os.cpu_count(), "affinity": cores}
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if quota != "max":
+            info["cgroup_quota"] = int(quota) / int(period)
+            cores = min(cores, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        info["omp_num_threads_env"] = int(env)
+        cores = min(cores, int(env))
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info["cpu_model"] = model
+    return cores, info
+
+
+def cpu_baseline(n, tmp, steps=3):
+    """The plain FP64 deck (PPCG, block Jacobi, the reference's stop at tl_eps
+    1e-10) through the reference's own Fortran driver compiled from its
+    sources (oracle/_ref, linked to the oracle's C restatement of the kernels
+    with OpenMP reductions) on the job's CPU cores, `steps` whole timesteps;
+    ms per timestep from the driver's own per-step wall clock in tea.out,
+    step 1 (CG pre-steps and eigenvalue estimate) excluded.  Without the
+    reference build, the same deck through this repository's host driver over
+    the same library, timed per step the same way."""
+    import subprocess
+
     import tealeaf_cuda_amd as tl
 
     oracle = REPO / "oracle" / "liboracle.so"
+    ref_exe = REPO / "oracle" / "_ref" / "tea_leaf_ref_oracle"
     if not oracle.exists():
         return None
-    cores = min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(cores)
-    os.environ["TEA_ORACLE_DOT"] = "plain"
-    # the sample switches to PPCG right after the 25 CG pre-steps (epslim
-    # lifted) so that the two runs differ only in PPCG iterations
-    presteps = 25
-    lo, hi = presteps + 2, presteps + 32
-    times = {}
-    for iters in (lo, hi):
-        deck = make_deck(Path(tmp) / f"cpu_{iters}.in", n, 1, max_iters=iters, epslim=1e30)
-        print(f"[bench] cpu baseline sample: {iters} iterations", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        tl.run_deck(deck, backend=oracle)
-        times[iters] = time.perf_counter() - t0
-    per_it = (times[hi] - times[lo]) / (hi - lo)
-    its = gpu_res.iterations[timed]
-    mean_its = sum(its) / len(its)
-    ms_step = per_it * mean_its * 1e3
+    cores, info = cpu_share()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores",
+               TEA_ORACLE_DOT="plain")
+    work = Path(tmp) / "cpu"
+    work.mkdir(exist_ok=True)
+    make_deck(work / "tea.in", n, steps)
+    print(f"[bench] cpu baseline: {steps} timesteps at {n}^2 on {cores} threads", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    walls = []
+    kind = "reference"
+    if ref_exe.exists():
+        proc = subprocess.run([str(ref_exe)], cwd=work, env=env, stdout=subprocess.DEVNULL,
+                              stderr=subprocess.PIPE, text=True, timeout=900)
+        text = (work / "tea.out").read_text() if (work / "tea.out").exists() else ""
+        walls = [float(l.split()[-1].replace("D", "E")) for l in text.splitlines()
+                 if l.strip().startswith("Wall clock") and len(l.split()) == 3]
+        if proc.returncode != 0 or len(walls) < steps:
+            print(f"[bench] reference driver failed ({proc.returncode}): {proc.stderr[-500:]}", file=sys.stderr)
+            walls = []
+    if not walls:
+        kind = "port"
+        saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES", "TEA_ORACLE_DOT")}
+        os.environ.update({k: env[k] for k in saved})
+        stamps = []
+        tl.run_deck(work / "tea.in", backend=oracle, on_step=lambda s: stamps.append(time.perf_counter()))
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+        walls = [t - t0 for t in stamps]
+    elapsed = time.perf_counter() - t0
+    per_step = [(walls[k] - walls[k - 1]) * 1e3 for k in range(1, steps)]
+    ms_step = sum(per_step) / len(per_step)
+    what = ("the reference Fortran driver (oracle/_ref/tea_leaf_ref_oracle)" if kind == "reference" else
+            "this repository's host driver")
     return {
         "value": round(ms_step, 1),
         "unit": "ms/timestep",
         "cores": cores,
-        "kind": "port",
-        "sample": (f"oracle (C restatement of the kernels, OpenMP {cores} threads, plain reductions) through the "
-                   f"same driver: first timestep at {n}^2 with tl_max_iters {lo} and {hi}; "
-                   f"{per_it * 1e3:.1f} ms per PPCG iteration x {mean_its:.1f} iterations per timed GPU step"),
-        "seconds": round(times[lo] + times[hi], 1),
+        "kind": kind,
+        "sample": (f"plain FP64 PPCG (block Jacobi, 10 inner steps, reference stop at tl_eps 1e-10), {n}^2 "
+                   f"random-rectangle layout seed {LAYOUT_SEED}: {what} over the oracle library (C restatement "
+                   f"of the kernels, OpenMP {cores} threads, OMP_PROC_BIND=close, OMP_PLACES=cores), {steps} whole "
+                   f"timesteps, ms per timestep over steps 2..{steps} from the driver's per-step wall clock"),
+        "per_step_ms": [round(x, 1) for x in per_step],
+        "seconds": round(elapsed, 1),
+        "host": info,
     }
 
 
@@ -280,8 +390,8 @@ def main():
         dominant, roof = roofline(profile, n)
         plain_roof = roofline(runs["fp64_reference"][3], n)[1] if "fp64_reference" in runs else None
         cpu = None
-        if not args.no_cpu_baseline and "fp64_reference" in runs:
-            cpu = cpu_baseline(n, runs["fp64_reference"][0], timed, tmp)
+        if not args.no_cpu_baseline:
+            cpu = cpu_baseline(n, tmp)
 
     def summary(variant):
         r, g_ms, w_ms, _, t, _ = runs[variant]
@@ -334,6 +444,10 @@ def main():
         "comparison": comparison,
         "roofline": roof,
         "roofline_fp64_reference": plain_roof,
+        "step_roofline": step_roofline(breakdown, PROFILED_STEPS, ms_per_step, n),
+        "step_roofline_fp64_reference": (step_roofline(runs["fp64_reference"][5], PROFILED_STEPS,
+                                                       runs["fp64_reference"][1] / args.steps, n)
+                                         if "fp64_reference" in runs else None),
         "dominant_kernel": dominant,
         "cpu_baseline": cpu,
         "kernels": {"steps": PROFILED_STEPS,

@@ -201,6 +201,10 @@ struct Chunk {
 };
 
 Chunk g;
+// The deck's profiler_on: per-kernel times are printed when the chunk is
+// released or the process ends (the reference prints them in the chunk's
+// destructor, cuda_errors.cu:91-106).
+bool g_profile_table = false;
 std::string g_deck_path = "tea.in";
 int g_error_mode = 0;
 char g_error[1024] = "";
@@ -410,6 +414,7 @@ bool collect(double *dst, int slot, int n)
 {
     hipError_t err = hipStreamSynchronize(g.stream);
     if (err != hipSuccess) return die("hipStreamSynchronize failed: %s", hipGetErrorString(err));
+    if (g_profile_table && g.prof.on) g.prof.drain(false);   // so the table at exit needs no HIP call
     for (int j = 0; j < n; j++) dst[j] = g.h_out[slot + j];
     return true;
 }
@@ -880,8 +885,34 @@ namespace {
 
 double *F(int id) { return g.fs[id]; }
 
+// The profiler_on table: total milliseconds per kernel, by name, in the
+// reference's format (cuda_errors.cu:95-105).  Folds in the completed launches
+// first when the stream still exists.
+void print_profile_table(bool hip_alive)
+{
+    if (!g_profile_table) return;
+    g_profile_table = false;
+    if (hip_alive && g.stream) {
+        (void)hipStreamSynchronize(g.stream);
+        g.prof.drain(false);
+    }
+    std::map<std::string, double> byname;
+    for (size_t k = 0; k < g.prof.names.size(); k++)
+        if (g.prof.counts[k] > 0) byname[g.prof.names[k]] += g.prof.ms[k];
+    fprintf(stdout, "@@@@@ PROFILING @@@@@\n");
+    for (const auto &kv : byname) fprintf(stdout, "%35s : %.3f\n", kv.first.c_str(), kv.second);
+    fflush(stdout);
+}
+
+// At process exit: the table of a chunk that was never released (no HIP calls
+// then; the launches were folded in at the last synchronisation).
+struct ProfileAtExit {
+    ~ProfileAtExit() { print_profile_table(false); }
+} g_profile_at_exit;
+
 void release_all()
 {
+    print_profile_table(true);
     if (g.stream) (void)hipStreamSynchronize(g.stream);
     for (auto &p : g.prof.pending) {
         (void)hipEventDestroy(p.a);
@@ -944,7 +975,10 @@ EXPORT void initialise_cuda_(int *x_min, int *x_max, int *y_min, int *y_max)
     if (prec.find("jac_diag") != std::string::npos) g.prec = PREC_DIAG;
     else if (prec.find("jac_block") != std::string::npos) g.prec = PREC_BLOCK;
     else g.prec = PREC_NONE;
-    if (param_enabled("profiler_on")) g.prof.on = true;
+    if (param_enabled("profiler_on")) {
+        g.prof.on = true;
+        g_profile_table = true;
+    }
 
     g.nx = *x_max - *x_min + 1;
     g.ny = *y_max - *y_min + 1;
@@ -1421,7 +1455,13 @@ template <typename T, int NS> void launch_inner_march(const Ext &e, long pt, con
     const int per_simd = sizeof(T) == 4 ? 2 : 1;
     int nchunks = cu_count() * 4 * per_simd / nbands;
     nchunks = nchunks < 1 ? 1 : (nchunks > nstrips ? nstrips : nchunks);
-    const int chunk = (nstrips + nchunks - 1) / nchunks;
+    int chunk = (nstrips + nchunks - 1) / nchunks;
+    // TEA_MARCH_CHUNK=<strips> sets the chunk length (tests run the
+    // production shape's multi-strip chunks on small meshes with it)
+    if (const char *s = getenv("TEA_MARCH_CHUNK")) {
+        const int c = atoi(s);
+        if (c > 0) chunk = c < nstrips ? c : nstrips;
+    }
     nchunks = (nstrips + chunk - 1) / chunk;
     const unsigned grid = (unsigned)((nbands * nchunks + MARCH_WAVES - 1) / MARCH_WAVES);
     // rtemp = r and utemp = sd still owed by tea_engine_ppcg_begin_ur: read r and sd instead

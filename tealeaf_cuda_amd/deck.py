@@ -122,6 +122,7 @@ class Deck:
     summary_frequency: int | None = None
     reflective_boundary: bool = False
     profiler_on: bool = False
+    verbose: bool = False               # verbose_on: the residual of every iteration in tea.out
     inverse_density: bool = False
     test_problem: int | None = None
     inner_precision: str | None = None  # "fp32" | "fp64": refinement on the FP64 true residual
@@ -165,6 +166,8 @@ class Deck:
             lines.append("reflective_boundary")
         if self.profiler_on:
             lines.append("profiler_on")
+        if self.verbose:
+            lines.append("verbose_on")
         if self.inverse_density:
             lines.append("tl_coefficient_inverse_density")
         if self.test_problem is not None:
@@ -196,6 +199,7 @@ class Deck:
 _SUMMARY = re.compile(r"^\s*step:\s*(\d+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s*$")
 _ITER = re.compile(r"^\s*Iteration count\s+(\d+)")
 _ERR = re.compile(r"^\s*Conduction error\s+(\S+)")
+_RES = re.compile(r"^\s*Residual\s+(\S+)\s*$")
 
 
 def _fortran_float(s: str) -> float:
@@ -207,8 +211,9 @@ def _fortran_float(s: str) -> float:
 
 def parse_tea_out(path) -> dict:
     """Field summaries [(step, (vol, mass, density, energy, u))], iteration
-    counts and conduction errors from a tea.out log."""
-    summaries, iterations, errors = [], [], []
+    counts, conduction errors and (verbose_on, tea_solve.f90:409-413) the
+    residual printed after every iteration from a tea.out log."""
+    summaries, iterations, errors, residuals = [], [], [], []
     with open(path) as f:
         for line in f:
             m = _SUMMARY.match(line)
@@ -222,4 +227,8 @@ def parse_tea_out(path) -> dict:
             m = _ERR.match(line)
             if m:
                 errors.append(_fortran_float(m.group(1)))
-    return {"summaries": summaries, "iterations": iterations, "conduction_error": errors}
+                continue
+            m = _RES.match(line)
+            if m:
+                residuals.append(_fortran_float(m.group(1)))
+    return {"summaries": summaries, "iterations": iterations, "conduction_error": errors, "residuals": residuals}

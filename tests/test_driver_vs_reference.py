@@ -35,6 +35,11 @@ DECKS = {
     "cg_inverse_density": dict(solver="cg", preconditioner="none", inverse_density=True),
     "ppcg_random_layout": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
                                states=random_layout(7, cells=48)),
+    # verbose_on: the residual after every iteration (tea_solve.f90:409-413)
+    "ppcg_block_verbose": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10, verbose=True),
+    "cg_diag_verbose": dict(solver="cg", preconditioner="jac_diag", eps=1e-12, verbose=True),
+    "ppcg_none_h2_verbose": dict(solver="ppcg", preconditioner="none", ppcg_inner_steps=6, halo_depth=2,
+                                 verbose=True),
 }
 
 
@@ -61,6 +66,8 @@ def test_same_output_as_reference_driver(name, oracle_so, driver_built, tmp_path
         assert s1 == s2
         assert v1 == v2  # 17 significant digits, bit-for-bit after printing
     assert our_out["conduction_error"] == ref_out["conduction_error"]
+    assert our_out["residuals"] == ref_out["residuals"]
+    assert (len(ref_out["residuals"]) > 0) == bool(deck.verbose)
 
 
 def test_reference_driver_chebyshev_aborts(oracle_so, tmp_path):

@@ -19,7 +19,7 @@ import os
 import numpy as np
 import pytest
 
-from tealeaf_cuda_amd import Deck, Engine, StateSpec, random_layout, run_deck
+from tealeaf_cuda_amd import Deck, Engine, StateSpec, parse_tea_out, random_layout, run_deck
 from tealeaf_cuda_amd import runtime as rt
 
 pytestmark = pytest.mark.gpu
@@ -71,6 +71,12 @@ DECKS = {
     "mixed_random_layout": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
                                 states=random_layout(12, cells=128), x_cells=128, y_cells=128,
                                 inner_precision="fp32"),
+    # verbose_on prints every iteration's residual, and keeps the refinement
+    # loop's outer iteration undeferred (the end is issued on its own)
+    "verbose_ppcg_block": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10, verbose=True),
+    "verbose_mixed_ppcg_block": dict(solver="ppcg", preconditioner="jac_block", ppcg_inner_steps=10,
+                                     inner_precision="fp32", verbose=True),
+    "verbose_cg_diag": dict(solver="cg", preconditioner="jac_diag", verbose=True),
 }
 
 
@@ -86,8 +92,10 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
     oracle_env("dot2")
     print(f"[gpu parity] deck {name}", flush=True)
     path = _deck(**DECKS[name]).write(tmp_path / "tea.in")
-    gpu = run_deck(path)
-    cpu = run_deck(path, backend=oracle_so)
+    gpu = run_deck(path, out_path=tmp_path / "gpu.out")
+    cpu = run_deck(path, backend=oracle_so, out_path=tmp_path / "cpu.out")
+    res_g, res_c = (parse_tea_out(tmp_path / f)["residuals"] for f in ("gpu.out", "cpu.out"))
+    assert res_g == res_c and (len(res_g) > 0) == bool(DECKS[name].get("verbose"))
     assert gpu.iterations == cpu.iterations
     assert gpu.ppcg_inner == cpu.ppcg_inner
     assert gpu.refinements == cpu.refinements

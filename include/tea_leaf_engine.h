@@ -285,6 +285,25 @@ void tea_engine_mp_replace(double theta, double *rr64, double *rr32);
 int tea_engine_get_field32(int field_id, float *host);
 int tea_engine_set_field32(int field_id, const float *host);
 
+/* ---- the 2-norm without allocation ------------------------------------------ *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the sum of a*b over the interior, as
+ * tea_leaf_calc_2norm_kernel_cuda_ forms it (the same compensated reduction,
+ * bit-identical results), on caller-owned FP64 device arrays of ny+2*halo rows
+ * of `pitch` elements (element [r*pitch + c] is cell (c, r), halo included;
+ * the interior is c in [halo, halo+nx), r in [halo, halo+ny)).  The result is
+ * written to dev_norm (device memory) by work queued on `stream` (a
+ * hipStream_t; NULL: the null stream); the call does not wait.  Allocates
+ * nothing: the reduction's partials live in a 64 KiB device array of the
+ * library that each GPU holds until the process ends, and every call on one
+ * GPU shares it, so calls that overlap on one GPU race.  Needs no chunk
+ * (initialise_cuda_ not required).  Limits: 1 <= nx, ny <= 65536, 0 <= halo <= 64,
+ * pitch >= nx + 2*halo.  Returns 0, or -1 with a message on stderr and in
+ * tea_engine_last_error for arguments beyond the limits (never aborts). */
+int tea_engine_calc_2norm_noalloc(const double *a, const double *b, int nx, int ny, int halo, long pitch,
+                                  double *dev_norm, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1232,6 +1232,87 @@ EXPORT void tea_leaf_calc_2norm_kernel_cuda_(int *norm_array, double *norm)
     if (ok) fetch(norm, 1);
 }
 
+// ---- the 2-norm without allocation ---------------------------------------------
+//
+// tea_engine_calc_2norm_noalloc: the dot product of tea_leaf_calc_2norm on
+// caller-owned device arrays, for callers that cannot allocate (code under
+// graph capture, serving loops).  Its workgroup partials live in `scratch`, a
+// file-scope device array of NOALLOC_BLOCKS entries (64 KiB) that each GPU
+// holds until the process ends; the grid is capped at that many workgroups,
+// which then loop over tiles, so any mesh size fits.  Every call on one GPU
+// shares that GPU's array: calls that overlap on one GPU race.
+namespace {
+constexpr int NOALLOC_BLOCKS = 4096;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+constexpr int NOALLOC_MAX_CELLS_SIDE = 1 << 16;
+static __device__ __align__(256) DD scratch[NOALLOC_BLOCKS];
+// The device address of each GPU's copy of `scratch` (the host-side name of a
+// __device__ array is no device address), looked up once per device.
+DD *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {nullptr};
+
+DD *scratch_on_current_device(char *err, size_t errlen)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipGetDevice failed: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        snprintf(err, errlen, "device %d beyond the %d devices supported", dev, NOALLOC_MAX_DEVICES);
+        return nullptr;
+    }
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            snprintf(err, errlen, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (DD *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Errors of the no-alloc entry point are returned, never fatal: the message
+// goes to stderr and to the sticky error (tea_engine_last_error).
+int noalloc_error(const char *msg)
+{
+    snprintf(g_error, sizeof g_error, "calc_2norm_noalloc: %s", msg);
+    fprintf(stderr, "tea_engine: %s\n", g_error);
+    fflush(stderr);
+    return -1;
+}
+}  // namespace
+
+EXPORT int tea_engine_calc_2norm_noalloc(const double *a, const double *b, int nx, int ny, int halo, long pitch,
+                                         double *dev_norm, void *stream)
+{
+    char err[256];
+    if (!a || !b || !dev_norm) return noalloc_error("null array or result pointer");
+    if (nx < 1 || ny < 1 || nx > NOALLOC_MAX_CELLS_SIDE || ny > NOALLOC_MAX_CELLS_SIDE || halo < 0 ||
+        halo > 64 || pitch < (long)nx + 2L * halo) {
+        snprintf(err, sizeof err, "mesh %d x %d, halo %d, pitch %ld: needs 1..%d cells per side, halo 0..64, "
+                 "pitch >= nx + 2 halo", nx, ny, halo, pitch, NOALLOC_MAX_CELLS_SIDE);
+        return noalloc_error(err);
+    }
+    DD *partials = scratch_on_current_device(err, sizeof err);
+    if (!partials) return noalloc_error(err);
+    const Ext e{halo, halo + nx - 1, halo, halo + ny - 1};
+    const int tiles = blocks_for(e);
+    const int grid = tiles < NOALLOC_BLOCKS ? tiles : NOALLOC_BLOCKS;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_dot<double>, dim3(grid), dim3(BLOCK), 0, s, e, pitch, a, b, Red{partials, NOALLOC_BLOCKS});
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, (const DD *)partials, NOALLOC_BLOCKS, grid, 1,
+                       dev_norm, (double *)nullptr);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        snprintf(err, sizeof err, "launch failed: %s", hipGetErrorString(le));
+        return noalloc_error(err);
+    }
+    return 0;
+}
+
 namespace {
 
 // Fields in the precision of the current solver mode: FP64 (the reference's

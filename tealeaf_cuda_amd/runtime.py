@@ -126,8 +126,15 @@ _EXT = {
     "tea_engine_set_field32": ([ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
 }
 
+# Entry points on caller-owned device memory: exported by the HIP library only.
+_DEVICE = {
+    "tea_engine_calc_2norm_noalloc": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+}
+
 ABI_SYMBOLS = tuple(_ABI)
 EXTENSION_SYMBOLS = tuple(_EXT)
+DEVICE_SYMBOLS = tuple(_DEVICE)
 
 
 def _ip(v: int):
@@ -161,6 +168,11 @@ class Engine:
             fn = getattr(self.lib, name)
             fn.argtypes = args
             fn.restype = ret
+        for name, (args, ret) in _DEVICE.items():
+            fn = getattr(self.lib, name, None)
+            if fn is not None:
+                fn.argtypes = args
+                fn.restype = ret
         self.lib.tea_engine_set_error_mode(1)
         self.refresh()
 
@@ -418,6 +430,15 @@ class Engine:
             raise ValueError(f"field shape {a.shape} != {self.shape()}")
         if self.lib.tea_engine_set_field32(int(field_id), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
             raise TeaError(f"set_field32({field_id}) failed")
+
+    def calc_2norm_noalloc(self, a_ptr: int, b_ptr: int, nx: int, ny: int, halo: int, pitch: int, out_ptr: int,
+                           stream: int | None = None) -> None:
+        """sum a*b over the interior of caller-owned device arrays into device
+        memory at out_ptr, queued on `stream`; allocates nothing (HIP library
+        only)."""
+        if self.lib.tea_engine_calc_2norm_noalloc(a_ptr, b_ptr, nx, ny, halo, pitch, out_ptr, stream) != 0:
+            self._check()
+            raise TeaError("calc_2norm_noalloc failed")
 
     def synchronize(self):
         self.lib.tea_engine_synchronize()

@@ -1482,19 +1482,30 @@ template <typename T> void ppcg_inner_impl(const Ext &e, double alpha, double be
 constexpr int INNER_WAVES = 16;
 
 // Microseconds per pass of NS = 1..MAX_PASS_STEPS inner steps at 4096^2 on
-// MI355X (block Jacobi; the ratios hold at 8192^2), with the kernel
-// use_march picks: the plan of passes for a run of inner steps minimises
+// MI355X (back-to-back passes over the same fields), with the kernel forms
+// inner_form picks: the plan of passes for a run of inner steps minimises
 // their sum.
-constexpr double PASS_US_F32[MAX_PASS_STEPS] = {132, 140, 158, 166, 198};
-constexpr double PASS_US_F64[MAX_PASS_STEPS] = {230, 237, 269, 322, 766};
-// With jac_diag the five-step FP32 pass needs more than 256 registers (one
-// wave per SIMD), so plans leave it out.
-constexpr double PASS_US_F32_DIAG[MAX_PASS_STEPS] = {132, 140, 158, 166, 1e9};
+constexpr double PASS_US_F32[MAX_PASS_STEPS] = {128, 138, 146, 150, 185};
+constexpr double PASS_US_F64[MAX_PASS_STEPS] = {239, 243, 255, 306, 766};
+constexpr double PASS_US_F32_DIAG[MAX_PASS_STEPS] = {144, 152, 161, 161, 168};
 
 const double *pass_costs()
 {
     if (!g.fp32) return PASS_US_F64;
     return g.prec == PREC_DIAG ? PASS_US_F32_DIAG : PASS_US_F32;
+}
+
+// Which form of the streaming pass runs (all give the same results): 0 the
+// shifting pipeline (k_ppcg_inner_march), 1 the ring (k_ppcg_inner_ring),
+// 2 the ring with the coefficients in LDS (k_ppcg_inner_ringl) -- the
+// fastest measured at 4096^2 for each field type, preconditioner and number
+// of steps; TEA_INNER_RING=0|1|2 forces one.
+template <typename T, int PREC, int NS> constexpr int inner_form()
+{
+    if (sizeof(T) == 4) return NS == 5 ? 2 : 1;
+    if (NS <= 2) return 0;
+    if (NS == 3) return 1;
+    return PREC == PREC_BLOCK ? 2 : 1;
 }
 
 // The streaming pass (k_ppcg_inner_march) is the faster kernel for every NS
@@ -1554,9 +1565,18 @@ template <typename T, int NS> void launch_inner_march(const Ext &e, long pt, con
                 FT<T>(X_SD2), rt_in, FT<T>(X_RT2), ut_in, FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
                 (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
     };
-    if (g.prec == PREC_BLOCK) go(k_ppcg_inner_march<T, PREC_BLOCK, NS>);
-    else if (g.prec == PREC_DIAG) go(k_ppcg_inner_march<T, PREC_DIAG, NS>);
-    else go(k_ppcg_inner_march<T, PREC_NONE, NS>);
+    const char *ring_env = getenv("TEA_INNER_RING");
+    const int forced = ring_env && ring_env[0] >= '0' && ring_env[0] <= '2' ? ring_env[0] - '0' : -1;
+    auto pick = [&](auto prec_c) {
+        constexpr int P = decltype(prec_c)::value;
+        const int form = forced >= 0 ? forced : inner_form<T, P, NS>();
+        if (form == 2) go(k_ppcg_inner_ringl<T, P, NS>);
+        else if (form == 1) go(k_ppcg_inner_ring<T, P, NS>);
+        else go(k_ppcg_inner_march<T, P, NS>);
+    };
+    if (g.prec == PREC_BLOCK) pick(std::integral_constant<int, PREC_BLOCK>{});
+    else if (g.prec == PREC_DIAG) pick(std::integral_constant<int, PREC_DIAG>{});
+    else pick(std::integral_constant<int, PREC_NONE>{});
 }
 
 int plan_first_pass(int remaining, const double *cost)

@@ -9,6 +9,8 @@
 #pragma once
 #include "engine_device.hpp"
 
+#include <type_traits>
+
 namespace tea {
 
 // Loop over this workgroup's tiles; the body sees the lane's strip s and i0,
@@ -1059,6 +1061,461 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_march(
         H[0] = cur;
 #pragma unroll
         for (int k = 0; k < STRIP; k++) sde0[k] = sde_cur[k];
+    }
+}
+
+// The value of lane 0 as a wave-uniform value (held in scalar registers).
+__device__ __forceinline__ float lane0_value(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ __forceinline__ double lane0_value(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// The streaming pass of k_ppcg_inner_march with its pipeline held as a ring
+// (no reference counterpart; the same results bit for bit).  Strip s lives
+// in ring slot (s - first) mod (NS+1) from the iteration it enters to the one
+// it leaves, and is advanced in place, so no strip moves between registers:
+// the loop body is instantiated NS+1 times, one per ring phase.  Row guards
+// are uniform per strip (its rows inside the extent) and branch, the column
+// guard is one select of sd per cell (a cell outside the extent keeps its sd;
+// its rtemp and utemp are never stored), and an advance a chunk's strips do
+// not need (the pipeline fill above the chunk: strip s needs steps up to
+// NS - its distance from the chunk) is skipped.
+template <typename T, int PREC, int NS>
+__global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
+    Ext e, long pitch, int chunk, const T *__restrict__ sd_in, T *__restrict__ sd_out, const T *__restrict__ rt_in,
+    T *__restrict__ rt_out, const T *ut_in, T *ut_out, const T *__restrict__ Mi, const T *__restrict__ Kx,
+    const T *__restrict__ Ky, StepCoefs cf)
+{
+    static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
+    constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS), RN = NS + 1;
+    static_assert(OUT_X == WAVE - HL - HR, "band layout");
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (e.x1 < e.x0 || e.y1 < e.y0) return;
+    const int nbands = (e.x1 - e.x0 + OUT_X) / OUT_X;
+    const int nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
+    const int nchunks = (nstrips + chunk - 1) / chunk;
+    const int w = blockIdx.x * MARCH_WAVES + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    if (w >= nbands * nchunks) return;
+    const int band = w % nbands, ck = w / nbands;
+    const int c = e.x0 + band * OUT_X - HL + lane;
+    const bool col_in = c >= e.x0 && c <= e.x1;
+    const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+    const bool out_lane = lane >= HL && lane <= WAVE - 1 - HR;
+    const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
+    const unsigned bytes = (unsigned)(e.y1 + 2) * row_b;
+    const FieldBuf<T> b_sd(sd_in, bytes), b_rt(rt_in, bytes), b_ut(ut_in, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
+        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes), b_uto(ut_out, bytes);
+    auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
+    const int s_first = ck * chunk;
+    const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
+
+    auto strip_len = [&](int r0) {   // rows of the strip inside the extent (uniform across the wave)
+        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
+        return len < 0 ? 0 : (len > STRIP ? STRIP : len);
+    };
+    // Issues the reads of strip s (the offsets of k_ppcg_inner_march).
+    auto fetch = [&](int s, MarchRaw<T> &m) {
+        const int r0 = e.y0 + s * STRIP, len = strip_len(r0), act = col_in ? len : 0;
+        const unsigned b0 = off(r0);
+        constexpr unsigned W = sizeof(T);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const int rr = r0 + k;
+            m.sd[k] = b_sd.load((col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) ? b0 + k * row_b : OOB);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = b_ky.load((k <= act && act > 0) ? b0 + k * row_b : OOB);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const bool a = k < act;
+            const unsigned bk = b0 + k * row_b;
+            m.kx[k] = b_kx.load(col_mem && k < len ? bk : OOB);
+            m.rt[k] = b_rt.load(a ? bk : OOB);
+            m.ut[k] = b_ut.load(a && out_lane ? bk : OOB);
+            m.mi[k] = PREC == PREC_DIAG ? b_mi.load(a ? bk : OOB) : T(0.0);
+            m.sde[k] = b_sd.load(a && lane == 0 ? bk - W : OOB);
+        }
+    };
+    // Forms ring slot m from strip s's reads; the block-Jacobi factors over the
+    // strip's rows inside the extent (a lane outside the extent reads zero
+    // coefficients, so its factors are finite and never stored).
+    auto prepare = [&](int s, const MarchRaw<T> &raw, MarchStrip<T> &m) {
+        m.r0 = e.y0 + s * STRIP;
+        m.len = strip_len(m.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            m.sd[k] = raw.sd[k];
+            m.rt[k] = raw.rt[k];
+            m.ut[k] = raw.ut[k];
+            m.kx[k] = raw.kx[k];
+            m.mi[k] = raw.mi[k];
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = raw.ky[k];
+        if (PREC == PREC_BLOCK && m.len > 0) {
+            T diag[STRIP];
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const T kxr = lane_from_above(m.kx[k]);
+                diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (kxr + m.kx[k]);
+            }
+            if (m.len == STRIP) strip_factor<T>(STRIP, diag, m.ky, m.cp, m.bfp);
+            else strip_factor<T>(m.len, diag, m.ky, m.cp, m.bfp);
+        }
+    };
+    // Moves strip m from step st to st+1 in place (lo, hi: sd of the rows
+    // below and above it at step st; sde: lane 0's left neighbour column at
+    // step 0).  The expressions of k_ppcg_inner_march's advance; LEN: the
+    // strip's rows inside the extent (STRIP, or 0 for a shorter strip, whose
+    // length m.len is then read at run time).
+    auto advance_len = [&](auto len_c, MarchStrip<T> &m, T lo, T hi, const T (&sde)[STRIP], int st) {
+        constexpr int LEN = decltype(len_c)::value;
+        const int len = LEN == STRIP ? STRIP : m.len;
+        const T a = (T)cf.a[st], b = (T)cf.b[st];
+        T rv[STRIP], zv[STRIP], diag[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const T from_left = lane_from_below(m.sd[k]);
+            const T from_right = lane_from_above(m.sd[k]);
+            const T kxr = lane_from_above(m.kx[k]);
+            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (kxr + m.kx[k]);
+            const T below = k == 0 ? lo : m.sd[k - 1];
+            const T above = k == STRIP - 1 ? hi : m.sd[k + 1];
+            const T left = (st > 0 || lane > 0) ? from_left : sde[k];
+            const T res = diag[k] * m.sd[k] - (m.ky[k + 1] * above + m.ky[k] * below)
+                          - (kxr * from_right + m.kx[k] * left);
+            rv[k] = m.rt[k] - res;
+        }
+        if (PREC == PREC_BLOCK) strip_apply<T>(rv, zv, len, diag[0], m.ky, m.cp, m.bfp);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            if (LEN == STRIP || k < len) {
+                T sdn;
+                if (PREC == PREC_BLOCK) sdn = a * m.sd[k] + b * zv[k];
+                else if (PREC == PREC_DIAG) sdn = a * m.sd[k] + b * rv[k] * m.mi[k];
+                else sdn = a * m.sd[k] + b * rv[k];
+                m.sd[k] = col_in ? sdn : m.sd[k];
+                m.rt[k] = rv[k];
+                m.ut[k] = m.ut[k] + sdn;
+            }
+        }
+    };
+    auto advance = [&](MarchStrip<T> &m, T lo, T hi, const T (&sde)[STRIP], int st) {
+        if (m.len == STRIP) advance_len(std::integral_constant<int, STRIP>{}, m, lo, hi, sde, st);
+        else if (m.len > 0) advance_len(std::integral_constant<int, 0>{}, m, lo, hi, sde, st);
+    };
+
+    MarchStrip<T> R[RN];   // R[j]: the strip entered at the iterations of phase j
+    T P[NS];               // P[m]: last row at step m of the strip behind the one at age m
+    T sde0[STRIP];         // lane 0's left neighbours of the strip at age 0 (uniform: scalar registers)
+#pragma unroll
+    for (int j = 0; j < RN; j++) {
+        R[j] = MarchStrip<T>{};
+        R[j].len = 0;
+        R[j].r0 = e.y0;
+    }
+#pragma unroll
+    for (int m = 0; m < NS; m++) P[m] = T(0.0);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) sde0[k] = T(0.0);
+    const int i0 = s_first - NS, i1 = s_last + NS;
+    MarchRaw<T> nxt;
+    fetch(i0, nxt);
+    // One iteration at ring phase PH = (i - i0) mod (NS+1): strip i enters
+    // slot PH, the strip of age m (i-1-m, slot PH-1-m) moves from step m to
+    // m+1, and strip i-NS (slot PH+1) leaves finished.
+    auto body = [&](auto ph, int i) {
+        constexpr int PH = decltype(ph)::value;
+        MarchStrip<T> &cur = R[PH];
+        prepare(i, nxt, cur);
+        T sde_cur[STRIP];   // lane 0's values, read out before the next fetch overwrites them
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde_cur[k] = lane0_value(nxt.sde[k]);
+        if (i < i1) fetch(i + 1, nxt);   // read one strip ahead
+        T hi = cur.sd[0];
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+            MarchStrip<T> &h = R[(PH - 1 - m + 2 * RN) % RN];
+            const T last = h.sd[STRIP - 1];
+            const int s = i - 1 - m;
+            const int dist = s < s_first ? s_first - s : (s > s_last ? s - s_last : 0);
+            if (dist <= NS - 1 - m) advance(h, P[m], hi, sde0, m);
+            P[m] = last;
+            hi = h.sd[0];
+        }
+        const MarchStrip<T> &f = R[(PH + 1) % RN];   // strip i-NS, finished
+        const int sf = i - NS;
+        const bool out = out_lane && sf >= s_first && sf <= s_last;
+        const unsigned b0 = off(f.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const unsigned bk = out && col_in && k < f.len ? b0 + k * row_b : OOB;
+            b_sdo.store(bk, f.sd[k]);
+            b_rto.store(bk, f.rt[k]);
+            b_uto.store(bk, f.ut[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde0[k] = sde_cur[k];
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    for (int i = i0;;) {
+        body(I0{}, i);
+        if (++i > i1) break;
+        body(I1{}, i);
+        if (++i > i1) break;
+        if constexpr (RN > 2) {
+            body(I2{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 3) {
+            body(I3{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 4) {
+            body(I4{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 5) {
+            body(I5{}, i);
+            if (++i > i1) break;
+        }
+    }
+}
+
+// k_ppcg_inner_ring with the face coefficients of the strips in flight in
+// LDS instead of registers (no reference counterpart; the same results bit
+// for bit).  A strip's Kx (4 rows) and Ky (5 rows) are written to the wave's
+// LDS ring slot when it enters and read back at each of its steps (the right
+// face's Kx from the next lane's column, lane 63 reading a zero column as the
+// lane shift gives it), which frees 9 registers per strip: longer passes keep
+// two waves per SIMD.  Each wave owns its part of LDS; its writes and reads
+// are in program order, so no barrier is needed.
+template <typename T, int PREC, int NS>
+__global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
+    Ext e, long pitch, int chunk, const T *__restrict__ sd_in, T *__restrict__ sd_out, const T *__restrict__ rt_in,
+    T *__restrict__ rt_out, const T *ut_in, T *ut_out, const T *__restrict__ Mi, const T *__restrict__ Kx,
+    const T *__restrict__ Ky, StepCoefs cf)
+{
+    static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
+    constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS), RN = NS + 1;
+    constexpr int NK = 2 * STRIP + 1, LW = WAVE + 1;   // LDS rows per slot: Kx 0..3, Ky 0..4
+    static_assert(OUT_X == WAVE - HL - HR, "band layout");
+    __shared__ T lds[MARCH_WAVES][RN][NK][LW];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    if (e.x1 < e.x0 || e.y1 < e.y0) return;
+    const int nbands = (e.x1 - e.x0 + OUT_X) / OUT_X;
+    const int nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
+    const int nchunks = (nstrips + chunk - 1) / chunk;
+    const int w = blockIdx.x * MARCH_WAVES + wv;
+    if (w >= nbands * nchunks) return;
+    T(*L)[NK][LW] = lds[wv];
+    if (lane == WAVE - 1) {   // the column past the last lane reads as zero
+#pragma unroll
+        for (int j = 0; j < RN; j++)
+#pragma unroll
+            for (int q = 0; q < NK; q++) L[j][q][WAVE] = T(0.0);
+    }
+    const int band = w % nbands, ck = w / nbands;
+    const int c = e.x0 + band * OUT_X - HL + lane;
+    const bool col_in = c >= e.x0 && c <= e.x1;
+    const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+    const bool out_lane = lane >= HL && lane <= WAVE - 1 - HR;
+    const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
+    const unsigned bytes = (unsigned)(e.y1 + 2) * row_b;
+    const FieldBuf<T> b_sd(sd_in, bytes), b_rt(rt_in, bytes), b_ut(ut_in, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
+        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes), b_uto(ut_out, bytes);
+    auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
+    const int s_first = ck * chunk;
+    const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
+
+    struct Slot {
+        T sd[STRIP], rt[STRIP], ut[STRIP], cp[STRIP], bfp[STRIP], mi[STRIP];
+        int r0, len;
+    };
+    auto strip_len = [&](int r0) {   // rows of the strip inside the extent (uniform across the wave)
+        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
+        return len < 0 ? 0 : (len > STRIP ? STRIP : len);
+    };
+    // Issues the reads of strip s (the offsets of k_ppcg_inner_march).
+    auto fetch = [&](int s, MarchRaw<T> &m) {
+        const int r0 = e.y0 + s * STRIP, len = strip_len(r0), act = col_in ? len : 0;
+        const unsigned b0 = off(r0);
+        constexpr unsigned W = sizeof(T);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const int rr = r0 + k;
+            m.sd[k] = b_sd.load((col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) ? b0 + k * row_b : OOB);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = b_ky.load((k <= act && act > 0) ? b0 + k * row_b : OOB);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const bool a = k < act;
+            const unsigned bk = b0 + k * row_b;
+            m.kx[k] = b_kx.load(col_mem && k < len ? bk : OOB);
+            m.rt[k] = b_rt.load(a ? bk : OOB);
+            m.ut[k] = b_ut.load(a && out_lane ? bk : OOB);
+            m.mi[k] = PREC == PREC_DIAG ? b_mi.load(a ? bk : OOB) : T(0.0);
+            m.sde[k] = b_sd.load(a && lane == 0 ? bk - W : OOB);
+        }
+    };
+    // Forms ring slot j from strip s's reads: the fields into registers, the
+    // coefficients into the slot's LDS rows, the block-Jacobi factors.
+    auto prepare = [&](int s, const MarchRaw<T> &raw, Slot &m, T (*Lj)[LW]) {
+        m.r0 = e.y0 + s * STRIP;
+        m.len = strip_len(m.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            m.sd[k] = raw.sd[k];
+            m.rt[k] = raw.rt[k];
+            m.ut[k] = raw.ut[k];
+            m.mi[k] = raw.mi[k];
+            Lj[k][lane] = raw.kx[k];
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) Lj[STRIP + k][lane] = raw.ky[k];
+        if (PREC == PREC_BLOCK && m.len > 0) {
+            T diag[STRIP];
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const T kxr = lane_from_above(raw.kx[k]);
+                diag[k] = T(1.0) + (raw.ky[k + 1] + raw.ky[k]) + (kxr + raw.kx[k]);
+            }
+            if (m.len == STRIP) strip_factor<T>(STRIP, diag, raw.ky, m.cp, m.bfp);
+            else strip_factor<T>(m.len, diag, raw.ky, m.cp, m.bfp);
+        }
+    };
+    // Moves the strip in slot m (coefficients in Lj) from step st to st+1 in
+    // place: the expressions of k_ppcg_inner_march's advance.
+    auto advance_len = [&](auto len_c, Slot &m, T (*Lj)[LW], T lo, T hi, const T (&sde)[STRIP], int st) {
+        constexpr int LEN = decltype(len_c)::value;
+        const int len = LEN == STRIP ? STRIP : m.len;
+        const T a = (T)cf.a[st], b = (T)cf.b[st];
+        T kx[STRIP], kxr[STRIP], ky[STRIP + 1];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            kx[k] = Lj[k][lane];
+            kxr[k] = Lj[k][lane + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) ky[k] = Lj[STRIP + k][lane];
+        T rv[STRIP], zv[STRIP], diag[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const T from_left = lane_from_below(m.sd[k]);
+            const T from_right = lane_from_above(m.sd[k]);
+            diag[k] = T(1.0) + (ky[k + 1] + ky[k]) + (kxr[k] + kx[k]);
+            const T below = k == 0 ? lo : m.sd[k - 1];
+            const T above = k == STRIP - 1 ? hi : m.sd[k + 1];
+            const T left = (st > 0 || lane > 0) ? from_left : sde[k];
+            const T res = diag[k] * m.sd[k] - (ky[k + 1] * above + ky[k] * below)
+                          - (kxr[k] * from_right + kx[k] * left);
+            rv[k] = m.rt[k] - res;
+        }
+        if (PREC == PREC_BLOCK) strip_apply<T>(rv, zv, len, diag[0], ky, m.cp, m.bfp);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            if (LEN == STRIP || k < len) {
+                T sdn;
+                if (PREC == PREC_BLOCK) sdn = a * m.sd[k] + b * zv[k];
+                else if (PREC == PREC_DIAG) sdn = a * m.sd[k] + b * rv[k] * m.mi[k];
+                else sdn = a * m.sd[k] + b * rv[k];
+                m.sd[k] = col_in ? sdn : m.sd[k];
+                m.rt[k] = rv[k];
+                m.ut[k] = m.ut[k] + sdn;
+            }
+        }
+    };
+    auto advance = [&](Slot &m, T (*Lj)[LW], T lo, T hi, const T (&sde)[STRIP], int st) {
+        if (m.len == STRIP) advance_len(std::integral_constant<int, STRIP>{}, m, Lj, lo, hi, sde, st);
+        else if (m.len > 0) advance_len(std::integral_constant<int, 0>{}, m, Lj, lo, hi, sde, st);
+    };
+
+    Slot R[RN];
+    T P[NS];
+    T sde0[STRIP];
+#pragma unroll
+    for (int j = 0; j < RN; j++) {
+        R[j] = Slot{};
+        R[j].len = 0;
+        R[j].r0 = e.y0;
+    }
+#pragma unroll
+    for (int m = 0; m < NS; m++) P[m] = T(0.0);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) sde0[k] = T(0.0);
+    const int i0 = s_first - NS, i1 = s_last + NS;
+    MarchRaw<T> nxt;
+    fetch(i0, nxt);
+    auto body = [&](auto ph, int i) {
+        constexpr int PH = decltype(ph)::value;
+        prepare(i, nxt, R[PH], L[PH]);
+        T sde_cur[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde_cur[k] = lane0_value(nxt.sde[k]);
+        T hi = R[PH].sd[0];
+        if (i < i1) fetch(i + 1, nxt);   // read one strip ahead
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+            const int j = (PH - 1 - m + 2 * RN) % RN;
+            Slot &h = R[j];
+            const T last = h.sd[STRIP - 1];
+            const int s = i - 1 - m;
+            const int dist = s < s_first ? s_first - s : (s > s_last ? s - s_last : 0);
+            if (dist <= NS - 1 - m) advance(h, L[j], P[m], hi, sde0, m);
+            P[m] = last;
+            hi = h.sd[0];
+        }
+        const Slot &f = R[(PH + 1) % RN];   // strip i-NS, finished
+        const int sf = i - NS;
+        const bool out = out_lane && sf >= s_first && sf <= s_last;
+        const unsigned b0 = off(f.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const unsigned bk = out && col_in && k < f.len ? b0 + k * row_b : OOB;
+            b_sdo.store(bk, f.sd[k]);
+            b_rto.store(bk, f.rt[k]);
+            b_uto.store(bk, f.ut[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde0[k] = sde_cur[k];
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    for (int i = i0;;) {
+        body(I0{}, i);
+        if (++i > i1) break;
+        body(I1{}, i);
+        if (++i > i1) break;
+        if constexpr (RN > 2) {
+            body(I2{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 3) {
+            body(I3{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 4) {
+            body(I4{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 5) {
+            body(I5{}, i);
+            if (++i > i1) break;
+        }
     }
 }
 

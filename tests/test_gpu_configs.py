@@ -115,6 +115,26 @@ def test_multi_strip_chunks_bitwise(name, chunk, oracle_so, driver_built, oracle
     _same(gpu, cpu)
 
 
+@pytest.mark.parametrize("form", ["0", "1", "2"])
+@pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "mixed_ppcg_diag",
+                                  "mixed_ppcg_none_ragged"])
+def test_pass_forms_bitwise(name, form, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
+    """Every form of the streaming inner-step pass (TEA_INNER_RING: 0 the
+    shifting pipeline, 1 the ring, 2 the ring with its coefficients in LDS)
+    with chunks of 3 strips gives the oracle's results."""
+    oracle_env("dot2")
+    monkeypatch.setenv("TEA_MARCH_CHUNK", "3")
+    monkeypatch.setenv("TEA_INNER_RING", form)
+    print(f"[gpu configs] deck {name}, pass form {form}", flush=True)
+    kw = dict(x_cells=96, y_cells=120, end_step=3, solver="ppcg", ppcg_inner_steps=10, eps=1e-10,
+              check_result=True, summary_frequency=1, states=LAYOUT)
+    kw.update(CHUNKED[name])
+    path = Deck(**kw).write(tmp_path / "tea.in")
+    gpu = run_deck(path)
+    cpu = run_deck(path, backend=oracle_so)
+    _same(gpu, cpu)
+
+
 def test_c4_heldout_warm_start(tmp_path):
     """C4 on held-out seeds 10000-10002, 5 steps each: iterations to the FP64
     true residual 1e-10 (of the u^n guess) for the three guesses.  Every run

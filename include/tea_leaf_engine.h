@@ -245,6 +245,16 @@ void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end
  * and runs them again. */
 void tea_engine_ppcg_end_begin_ur_inner(double rro_prev, double theta, int inner_steps, double *rrn_end,
                                         double *pw_rro, double *rrn);
+/* ---- fused CG iteration ------------------------------------------------------ *
+ * No counterpart in the reference: [tea_leaf_cg_calc_p_kernel_cuda_(beta) if
+ * with_p], tea_leaf_cg_calc_w_kernel_cuda_ (pw), then
+ * tea_leaf_cg_calc_ur_kernel_cuda_(rro / pw) (rrn), with identical results, in
+ * two passes over HBM and one host wait (the division by pw is done on the
+ * device as the host does it).  For CG-family decks without reflective
+ * boundaries (the reference driver's update_halo of p between calc_p and
+ * calc_w is a no-op there). */
+void tea_engine_cg_begin_ur(int with_p, double beta, double rro, double *pw, double *rrn);
+
 /* Inner steps `step` .. `step + nsteps - 1` (1-based; nsteps 1..5) in one
  * pass over HBM (temporal blocking): the same results as
  * tea_leaf_ppcg_inner_kernel_cuda_ called for each of the steps with

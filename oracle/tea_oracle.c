@@ -843,6 +843,16 @@ EXPORT void tea_engine_ppcg_begin_ur(int with_p, double beta, double theta, doub
     tea_engine_ppcg_ur(pw_rro[1] / pw_rro[0], theta, rrn);
 }
 
+/* The fused CG iteration of the engine, as the composition it names:
+ * [cg_calc_p(beta)], cg_calc_w (pw), cg_calc_ur(rro / pw) (rrn). */
+EXPORT void tea_engine_cg_begin_ur(int with_p, double beta, double rro, double *pw, double *rrn)
+{
+    if (with_p) tea_leaf_cg_calc_p_kernel_cuda_(&beta);
+    tea_leaf_cg_calc_w_kernel_cuda_(pw);
+    double alpha = rro / *pw;
+    tea_leaf_cg_calc_ur_kernel_cuda_(&alpha, rrn);
+}
+
 EXPORT void tea_engine_ppcg_end(double *rrn);
 EXPORT void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end, double *pw_rro,
                                          double *rrn)

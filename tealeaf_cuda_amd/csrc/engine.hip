@@ -1898,6 +1898,66 @@ EXPORT void tea_leaf_cg_calc_p_kernel_cuda_(double *beta)
     BY_PRECISION(cg_calc_p_impl, *beta);
 }
 
+// ---- fused CG iteration (no reference counterpart) ---------------------------------
+//
+// tea_engine_cg_begin_ur: [cg_calc_p(beta) if with_p], cg_calc_w (pw),
+// cg_calc_ur(alpha = rro / pw) (rrn), with identical results: calc_p is
+// folded into calc_w (p' formed on the fly into the spare p buffer, w = A p'
+// not stored), alpha is divided on the device, and ur forms w = A p' again
+// from p' (one host wait, two passes instead of three).  w is then owed
+// (pending_w): the next fused call rewrites it, any other entry point writes
+// it out first.  FP32 mode runs the separate calls.
+namespace {
+bool cg_begin_ur_f64(int with_p, double beta, double rro, double *pw, double *rrn)
+{
+    const long pt = g.pitch;
+    const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+    if (g.p_halo_dirty[0]) {   // the spare's halo ring takes p's (calc_p leaves the halo as it is)
+        Timed t("p_halo_ring");
+        hipLaunchKernelGGL(k_copy_ring<double>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream,
+                           F(X_P2), (const double *)F(F_P), pt, g.h, g.nx, g.ny);
+        g.p_halo_dirty[0] = false;
+    }
+    const double *src = g.prec != PREC_NONE ? (const double *)F(X_Z) : (const double *)F(F_R);
+    if (!launch("cg_begin", k_ppcg_begin<double>, interior(), pt, with_p, beta, (const double *)nullptr,
+                (const double *)F(F_P), F(X_P2), src, (double *)nullptr, (const double *)F(X_KX),
+                (const double *)F(X_KY), (const double *)nullptr, (double *)nullptr, red_ctx()))
+        return false;
+    std::swap(g.f[F_P], g.f[X_P2]);
+    std::swap(g.fs[F_P], g.fs[X_P2]);
+    if (!finish(0, 1, g.d_scal)) return false;
+    auto go = [&](auto kernel) {
+        return launch("cg_ur", kernel, interior(), pt, rro, (const double *)g.d_scal, F(F_U), (const double *)F(F_P),
+                      F(F_R), F(X_Z), (const double *)F(X_MI), (const double *)F(X_KX), (const double *)F(X_KY),
+                      red_ctx());
+    };
+    const bool ok = g.prec == PREC_BLOCK ? go(k_cg_ur_wp<double, PREC_BLOCK>)
+                    : g.prec == PREC_DIAG ? go(k_cg_ur_wp<double, PREC_DIAG>)
+                                          : go(k_cg_ur_wp<double, PREC_NONE>);
+    if (!ok || !finish(2, 1, nullptr)) return false;
+    g.pending_w = 64;
+    if (!collect(pw, 0, 1)) return false;
+    *rrn = g.h_out[2];
+    return true;
+}
+}  // namespace
+
+EXPORT void tea_engine_cg_begin_ur(int with_p, double beta, double rro, double *pw, double *rrn)
+{
+    CHECK_INIT_KEEP_PENDING();
+    REQUIRE_SOLVER(S_CG_FAMILY);
+    if (g.fp32) {   // the composition, call by call
+        settle_pending();
+        if (with_p) cg_calc_p_impl<float>(beta);
+        cg_calc_w_impl<float>(pw);
+        cg_calc_ur_impl<float>(rro / *pw, rrn);
+        return;
+    }
+    g.pending_w = 0;   // rewritten below
+    settle_pending();
+    cg_begin_ur_f64(with_p, beta, rro, pw, rrn);
+}
+
 namespace {
 void upload_coefs(const double *a, const double *b, int n)
 {

@@ -1584,10 +1584,12 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_begin(Ext e, long pitch, int wit
                          - (Kx[i + 1] * right + Kx[i] * left);
             pn[i] = pc;
             if (w) w[i] = wv;
-            const T rv = r[i];
-            rstore[i] = rv;
             dd_add_prod(acc[0], (double)pc, (double)wv);
-            dd_add_prod(acc[1], (double)rv, (double)z[i]);
+            if (rstore) {   // null: CG's begin (no store_r, no r.z)
+                const T rv = r[i];
+                rstore[i] = rv;
+                dd_add_prod(acc[1], (double)rv, (double)z[i]);
+            }
         }
     }
     STRIP_END
@@ -1947,6 +1949,63 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_wp(Ext e, long pitch, doub
             if (rtemp) {   // null: rtemp = r and utemp = sd left to the first inner pass
                 rtemp[i] = rv[k];
                 utemp[i] = sdv;
+            }
+        }
+    }
+    STRIP_END
+    block_reduce<1>(acc, red);
+}
+
+// CG's calc_ur (tea_leaf_cg.cuknl:72-137) with w = A p formed again from p
+// (no reference counterpart as a kernel: the fused CG iteration's begin
+// leaves w unwritten), alpha = rro / pw formed on the device from begin's
+// p.w: u += alpha p; r -= alpha w; z = M^-1 r; sum r*z (r*r).  The same
+// expressions as cg_calc_w and cg_calc_ur, so the same values.
+template <typename T, int PREC>
+__global__ __launch_bounds__(BLOCK) void k_cg_ur_wp(Ext e, long pitch, double rro, const double *__restrict__ pw,
+                                                    T *__restrict__ u, const T *__restrict__ p, T *__restrict__ r,
+                                                    T *__restrict__ z, const T *__restrict__ Mi,
+                                                    const T *__restrict__ Kx, const T *__restrict__ Ky, Red red)
+{
+    const T alpha = (T)(rro / pw[0]);
+    const int lane = threadIdx.x & (WAVE - 1);
+    DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
+    const int c = s.c;
+    T col[STRIP + 2];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; q++) {
+        col[q] = T(0.0);
+        if (s.len > 0 && q <= s.len + 1) col[q] = p[i0 + (long)(q - 1) * pitch];
+    }
+    const bool rlane = lane < WAVE - 1 && c + 1 <= e.x1;
+    T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        const T from_left = __shfl_up(col[k + 1], 1, WAVE);
+        const T from_right = __shfl_down(col[k + 1], 1, WAVE);
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const T left = lane > 0 ? from_left : p[i - 1];
+            const T right = rlane ? from_right : p[i + 1];
+            const T pc = col[k + 1];
+            const T wv = (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * pc
+                         - (Ky[i + pitch] * col[k + 2] + Ky[i] * col[k])
+                         - (Kx[i + 1] * right + Kx[i] * left);
+            u[i] += alpha * pc;
+            rv[k] = r[i] - alpha * wv;
+            r[i] = rv[k];
+        }
+    }
+    if (PREC != PREC_NONE) precond_strip<T, PREC>(rv, zv, s.len, i0, pitch, Mi, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            if (PREC != PREC_NONE) {
+                z[i0 + k * pitch] = zv[k];
+                dd_add_prod(acc[0], (double)rv[k], (double)zv[k]);
+            } else {
+                dd_add_prod(acc[0], (double)rv[k], (double)rv[k]);
             }
         }
     }

@@ -97,6 +97,7 @@ struct Backend {
     void (*ppcg_end_begin_ur)(double, double, double *, double *, double *) = nullptr;
     void (*ppcg_end_begin_ur_inner)(double, double, int, double *, double *, double *) = nullptr;
     int (*ppcg_inner_plan)(int) = nullptr;
+    void (*cg_begin_ur)(int, double, double, double *, double *) = nullptr;
 
     void check() const
     {
@@ -141,6 +142,8 @@ Backend load_backend(const char *path)
         dlsym(b.handle, "tea_engine_ppcg_end_begin_ur"));
     b.ppcg_end_begin_ur_inner = reinterpret_cast<void (*)(double, double, int, double *, double *, double *)>(
         dlsym(b.handle, "tea_engine_ppcg_end_begin_ur_inner"));
+    b.cg_begin_ur =
+        reinterpret_cast<void (*)(int, double, double, double *, double *)>(dlsym(b.handle, "tea_engine_cg_begin_ur"));
     return b;
 }
 
@@ -937,6 +940,22 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
         fields[FIELD_U - 1] = 1;
     }
 
+    // fused CG iterations (tea_engine_cg_begin_ur): the calc_p of each
+    // iteration is applied by the next call, or by the separate entry point
+    // when the CG branch is left (not under residual replacement, whose
+    // updates fall between calc_ur and calc_p's successor)
+    const char *fused_cg_env = std::getenv("TEA_FUSED_CG");
+    const bool fused_cg = cg_family && b.cg_begin_ur && !d.reflective_boundary && !rel.on &&
+                          !(fused_cg_env && fused_cg_env[0] == '0');
+    bool cg_p_pending = false;
+    double cg_beta_pending = 0.0;
+    auto flush_cg_p = [&]() {
+        if (!cg_p_pending) return;
+        b.tea_leaf_cg_calc_p_kernel_cuda_(&cg_beta_pending);
+        b.check();
+        cg_p_pending = false;
+    };
+
     for (n = 1; n <= max_iters; n++) {
         if (!ch_switch_check) {
             if (cheby_calc_steps > 0) ch_switch_check = true;
@@ -944,6 +963,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
             else if (std::fabs(old_error) <= d.tl_ch_cg_epslim && n >= d.tl_ch_cg_presteps) ch_switch_check = true;
             else ch_switch_check = false;
         }
+        if ((d.tl_use_chebyshev || d.tl_use_ppcg) && ch_switch_check) flush_cg_p();
 
         if ((d.tl_use_chebyshev || d.tl_use_ppcg) && ch_switch_check) {
             if (cheby_calc_steps == 0) {
@@ -1103,6 +1123,19 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                 rro = rrn;
             }
             cheby_calc_steps = cheby_calc_steps + 1;
+        } else if (cg_family && fused_cg) {
+            // the calls below in one: [calc_p(beta) of the previous iteration], calc_w, calc_ur
+            fields[FIELD_P - 1] = 1;
+            b.cg_begin_ur(cg_p_pending ? 1 : 0, cg_beta_pending, rro, &pw, &rrn);
+            b.check();
+            alpha = rro / pw;
+            cg_alphas[n] = alpha;
+            beta = rrn / rro;
+            cg_betas[n] = beta;
+            cg_p_pending = true;
+            cg_beta_pending = beta;
+            error = rrn;
+            rro = rrn;
         } else if (cg_family) {
             fields[FIELD_P - 1] = 1;
             b.tea_leaf_cg_calc_w_kernel_cuda_(&pw);
@@ -1132,6 +1165,7 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
         if (std::fabs(error) < eps * initial_residual) break;
         old_error = error;
     }
+    flush_cg_p();
     // a DO loop that runs to completion leaves n = max_iters + 1
     if (end_pending) {   // stopped by tl_max_iters: complete the last outer iteration
         b.ppcg_end(&rrn);

@@ -117,6 +117,8 @@ _EXT = {
                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_ppcg_inner_plan": ([ctypes.c_int], ctypes.c_int),
+    "tea_engine_cg_begin_ur": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
     "tea_engine_mp_begin": ([_D], None),
@@ -388,6 +390,12 @@ class Engine:
         n = self.lib.tea_engine_ppcg_inner_plan(int(remaining))
         self._check()
         return int(n)
+
+    def cg_begin_ur(self, with_p: bool, beta: float, rro: float):
+        pw, rrn = ctypes.c_double(), ctypes.c_double()
+        self._call("tea_engine_cg_begin_ur", int(bool(with_p)), float(beta), float(rro), ctypes.byref(pw),
+                   ctypes.byref(rrn))
+        return pw.value, rrn.value
 
     def ppcg_end(self) -> float:
         out = ctypes.c_double()

@@ -107,15 +107,18 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 
 
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
-                                    "TEA_W_FROM_P"])
-@pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg"])
+                                    "TEA_W_FROM_P", "TEA_FUSED_CG"])
+@pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
+                                  "cg_block_ragged"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
     """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0), the fused
     outer iteration with one inner step per pass (TEA_INNER_PASSES=0), the
     inner passes on the tile kernels (TEA_INNER_MARCH=0), the refinement
     loop without inner passes queued ahead of the replacement test
-    (TEA_SPECULATE=0), and the FP32 fused pair with w written by begin and read
-    by ur (TEA_W_FROM_P=0), on the GPU give the same results as the oracle."""
+    (TEA_SPECULATE=0), the FP32 fused pair with w written by begin and read
+    by ur (TEA_W_FROM_P=0), and CG iterations through the separate calls
+    instead of the fused one (TEA_FUSED_CG=0), on the GPU give the same
+    results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -185,6 +188,12 @@ CALLS = {
     "cg_calc_ur": lambda e: e.cg_calc_ur(0.37),
     "cg_calc_p": lambda e: e.cg_calc_p(0.61),
     "field_summary": lambda e: e.field_summary(),
+    "fused_cg_begin_ur_p": lambda e: e.cg_begin_ur(True, 0.61, 0.8),
+    "fused_cg_begin_ur": lambda e: e.cg_begin_ur(False, 0.0, 0.8),
+    # the w owed by the fused iteration is written out before the next call reads it
+    "fused_cg_begin_ur_then_norm_then_w": lambda e: (e.cg_begin_ur(True, 0.61, 0.8), e.calc_2norm(1),
+                                                     e.cg_calc_w()),
+    "fused_cg_begin_ur_twice": lambda e: (e.cg_begin_ur(False, 0.0, 0.8), e.cg_begin_ur(True, 0.3, 1.1)),
 }
 def _tile_kernels(fn):
     """Runs fn with the engine's inner-step passes on the tile kernels

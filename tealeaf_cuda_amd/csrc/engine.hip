@@ -1715,6 +1715,33 @@ template <typename T> bool w_from_p()
     return g.prec == PREC_BLOCK && !(s && s[0] == '0');
 }
 
+// The streaming form of the fused begin (k_begin_march) unless
+// TEA_BEGIN_MARCH=0: bands of 62 columns, runs of rows sized for about eight
+// waves per SIMD.  Returns false (nothing launched) when it does not apply.
+template <typename T, bool STORE_R>
+bool begin_march_launch(const char *name, int with_p, double beta, const double *beta_num, const T *p, T *pn,
+                        const T *z, T *w, const T *r, T *rstore)
+{
+    const char *env = getenv("TEA_BEGIN_MARCH");
+    if (env && env[0] == '0') return false;
+    const Ext e = interior();
+    const int nbands = (e.x1 - e.x0 + BEGIN_OUT_X) / BEGIN_OUT_X, ny = e.y1 - e.y0 + 1;
+    const int target = cu_count() * 4 * 8;
+    int runs = target / nbands;
+    runs = runs < 1 ? 1 : runs;
+    int rows = (ny + runs - 1) / runs;
+    rows = rows < 8 ? 8 : rows;
+    runs = (ny + rows - 1) / rows;
+    const int grid = (nbands * runs + MARCH_WAVES - 1) / MARCH_WAVES;
+    if (grid > g.max_blocks) return false;
+    g.last_grid = grid;
+    klaunch(name, k_begin_march<T, STORE_R>, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pitch_of<T>(), rows, with_p,
+            beta, beta_num, p, pn, z, w, (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), r, rstore, red_ctx());
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
+    return true;
+}
+
 // z_src: where z is read (null: the z field); w_later: w not stored.
 template <typename T>
 bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr, const T *z_src = nullptr,
@@ -1732,11 +1759,15 @@ bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr
     }
     bool ok;
     if constexpr (std::is_same<T, float>::value)
-        ok = launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
+        ok = begin_march_launch<T, true>("ppcg_begin_f32", with_p, beta, beta_num, (const T *)FT<T>(F_P), FT<T>(X_P2),
+                                         zf, w_later ? nullptr : FT<T>(X_W), (const T *)FT<T>(F_R), FT<T>(X_RSTORE)) ||
+             launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
                        FT<T>(X_P2), zf, w_later ? nullptr : FT<T>(X_W), (const T *)FT<T>(X_KX),
                        (const T *)FT<T>(X_KY), (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     else
-        ok = launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
+        ok = begin_march_launch<T, true>(nm<T>("ppcg_begin"), with_p, beta, beta_num, (const T *)FT<T>(F_P), FT<T>(X_P2),
+                                         zf, w_later ? nullptr : FT<T>(X_W), (const T *)FT<T>(F_R), FT<T>(X_RSTORE)) ||
+             launch(nm<T>("ppcg_begin"), k_ppcg_begin<T>, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
                     FT<T>(X_P2), zf, w_later ? nullptr : FT<T>(X_W), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
                     (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx());
     if (!ok) return false;
@@ -1919,7 +1950,9 @@ bool cg_begin_ur_f64(int with_p, double beta, double rro, double *pw, double *rr
         g.p_halo_dirty[0] = false;
     }
     const double *src = g.prec != PREC_NONE ? (const double *)F(X_Z) : (const double *)F(F_R);
-    if (!launch("cg_begin", k_ppcg_begin<double>, interior(), pt, with_p, beta, (const double *)nullptr,
+    if (!begin_march_launch<double, false>("cg_begin", with_p, beta, nullptr, (const double *)F(F_P), F(X_P2), src,
+                                           nullptr, nullptr, nullptr) &&
+        !launch("cg_begin", k_ppcg_begin<double>, interior(), pt, with_p, beta, (const double *)nullptr,
                 (const double *)F(F_P), F(X_P2), src, (double *)nullptr, (const double *)F(X_KX),
                 (const double *)F(X_KY), (const double *)nullptr, (double *)nullptr, red_ctx()))
         return false;

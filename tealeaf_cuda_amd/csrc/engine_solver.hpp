@@ -1519,6 +1519,78 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     }
 }
 
+// The fused begin (k_ppcg_begin) as a streaming pass (no reference
+// counterpart; the same values): [p' = z + beta p] on the fly, w = A p'
+// (stored if w != null), sum p'.w, and with STORE_R rstore = r and sum r.z.
+// Each wave owns a band of WAVE columns (lanes 1..WAVE-2 written; lanes 0 and
+// WAVE-1 form p' for the band's neighbour columns) and a run of `rows` rows,
+// and walks down it keeping p' of the rows above and below in registers, so
+// every field row is read once per band (k_ppcg_begin forms p' of a strip's
+// rows above and below from memory again).  Columns come from lane shifts.
+constexpr int BEGIN_OUT_X = WAVE - 2;
+template <typename T, bool STORE_R>
+__global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long pitch, int rows, int with_p,
+                                                                   double beta_in, const double *__restrict__ beta_num,
+                                                                   const T *__restrict__ p, T *__restrict__ pn,
+                                                                   const T *__restrict__ z, T *__restrict__ w,
+                                                                   const T *__restrict__ Kx, const T *__restrict__ Ky,
+                                                                   const T *__restrict__ r, T *__restrict__ rstore,
+                                                                   Red red)
+{
+    const T beta = (T)(beta_num ? beta_num[0] / beta_in : beta_in);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nbands = (e.x1 - e.x0 + BEGIN_OUT_X) / BEGIN_OUT_X;
+    const int ny = e.y1 - e.y0 + 1;
+    const int nruns = (ny + rows - 1) / rows;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int wid = blockIdx.x * MARCH_WAVES + wv;
+    DD acc[2] = {{0, 0}, {0, 0}};
+    if (wid < nbands * nruns) {
+        const int band = wid % nbands, run = wid / nbands;
+        const int c = e.x0 + band * BEGIN_OUT_X - 1 + lane;
+        const bool col_in = c >= e.x0 && c <= e.x1;
+        const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+        const bool out = col_in && lane >= 1 && lane <= WAVE - 2;
+        const int ya = e.y0 + run * rows;
+        const int yb = (ya + rows - 1 < e.y1 ? ya + rows - 1 : e.y1);
+        // p' at (c, row): z + beta p inside the extent (with_p), p elsewhere
+        auto pval = [&](int row) -> T {
+            if (!col_mem || row < e.y0 - 1 || row > e.y1 + 1) return T(0.0);
+            const long i = (long)c + (long)row * pitch;
+            const bool in = col_in && row >= e.y0 && row <= e.y1;
+            return (with_p && in) ? z[i] + beta * p[i] : p[i];
+        };
+        auto kyv = [&](int row) -> T {   // Ky at (c, row) (rows y0 .. y1+1 are faces of the extent)
+            return (col_in && row >= e.y0 && row <= e.y1 + 1) ? Ky[(long)c + (long)row * pitch] : T(0.0);
+        };
+        T pm = pval(ya - 1), pc = pval(ya);
+        T ky0 = kyv(ya);
+        for (int row = ya; row <= yb; row++) {
+            const long i = (long)c + (long)row * pitch;
+            const T pp = pval(row + 1);
+            const T ky1 = kyv(row + 1);
+            const T kx0 = col_mem ? Kx[i] : T(0.0);
+            const T kx1 = lane_from_above(kx0);   // Kx on the right face: the next column's
+            const T left = lane_from_below(pc), right = lane_from_above(pc);
+            const T wval = (T(1.0) + (ky1 + ky0) + (kx1 + kx0)) * pc - (ky1 * pp + ky0 * pm) - (kx1 * right + kx0 * left);
+            if (out) {
+                pn[i] = pc;
+                if (w) w[i] = wval;
+                dd_add_prod(acc[0], (double)pc, (double)wval);
+                if (STORE_R) {
+                    const T rv = r[i];
+                    rstore[i] = rv;
+                    dd_add_prod(acc[1], (double)rv, (double)z[i]);
+                }
+            }
+            pm = pc;
+            pc = pp;
+            ky0 = ky1;
+        }
+    }
+    block_reduce<2>(acc, red);
+}
+
 // tea_leaf_ppcg.cuknl:279-297: sum (r - r_store)*z
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const T *__restrict__ rstore,

@@ -158,6 +158,13 @@ struct Chunk {
     // z = utemp owed by a separate ppcg_end with a preconditioner (the next
     // fused begin reads utemp in its place and ur rewrites z), 32 or 64, or 0
     int pending_z = 0;
+    // w and r of the u held in the spare u buffer owed by a fused Chebyshev
+    // iteration (32 or 64: the precision of the fields, 0: none)
+    int pending_cheby = 0;
+    // u's halo ring may differ from the spare u buffer's (set by every entry
+    // point but the fused Chebyshev iteration, which copies the ring across
+    // when it is set)
+    bool u_ring_dirty = true;
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -235,7 +242,7 @@ void settle_pending();
 #define CHECK_INIT_KEEP_PENDING()                                                                 \
     do { if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return; } } while (0)
 #define CHECK_INIT()                                                                              \
-    do { CHECK_INIT_KEEP_PENDING(); settle_pending(); } while (0)
+    do { CHECK_INIT_KEEP_PENDING(); g.u_ring_dirty = true; settle_pending(); } while (0)
 // Kernels are registered per solver in the reference (init_cuda.cu:251-282) and
 // launching an unregistered one throws there; the same calls are errors here.
 #define REQUIRE_SOLVER(mask)                                                                      \
@@ -1168,6 +1175,7 @@ EXPORT void cuda_pack_buffers_(int *fields, int *offsets, int *depth, int *face,
 EXPORT void cuda_unpack_buffers_(int *fields, int *offsets, int *depth, int *face, double *host_buffer)
 {
     if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
+    g.u_ring_dirty = true;                 // halo cells of u may change
     pack_unpack(fields, offsets, *depth, *face, 0, host_buffer);
 }
 
@@ -1215,11 +1223,26 @@ EXPORT void tea_leaf_common_finalise_kernel_cuda_(void)
 
 EXPORT void tea_leaf_calc_2norm_kernel_cuda_(int *norm_array, double *norm)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();
+    g.u_ring_dirty = true;
     if (*norm_array < 0 || *norm_array > 2) {
         die("Invalid value '%d' for norm_array passed, should be [0, 1, 2]", *norm_array);
         return;
     }
+    if (*norm_array == 1 && g.pending_cheby == (g.fp32 ? 32 : 64)) {
+        // r.r of the r a fused Chebyshev iteration owes, formed on the fly (it stays owed)
+        bool ok;
+        if (g.fp32)
+            ok = launch("calc_2norm_cheby_f32", k_cheby_rr<float>, interior(), g.pitch32,
+                        (const float *)g.fs32[X_U2], (const float *)g.fs32[X_U0], (const float *)g.fs32[X_KX],
+                        (const float *)g.fs32[X_KY], red_ctx());
+        else
+            ok = launch("calc_2norm_cheby", k_cheby_rr<double>, interior(), g.pitch, (const double *)F(X_U2),
+                        (const double *)F(X_U0), (const double *)F(X_KX), (const double *)F(X_KY), red_ctx());
+        if (ok) fetch(norm, 1);
+        return;
+    }
+    settle_pending();
     const int u0 = X_U0, r = F_R, z = g.prec != PREC_NONE ? X_Z : F_R;
     const int ia = *norm_array == 0 ? u0 : r, ib = *norm_array == 0 ? u0 : (*norm_array == 1 ? r : z);
     bool ok;
@@ -1365,6 +1388,16 @@ void settle_pending()
 {
     settle_z();   // before the rtemp/utemp debt rewrites utemp
     settle_rt_ut();
+    const int cbits = g.pending_cheby;
+    g.pending_cheby = 0;
+    if (cbits == 32)
+        launch("settle_cheby_wr_f32", k_cheby_wr<float>, interior(), g.pitch32, (const float *)g.fs32[X_U2],
+               (const float *)g.fs32[X_U0], g.fs32[F_R], g.fs32[X_W], (const float *)g.fs32[X_KX],
+               (const float *)g.fs32[X_KY]);
+    else if (cbits == 64)
+        launch("settle_cheby_wr", k_cheby_wr<double>, interior(), g.pitch, (const double *)g.fs[X_U2],
+               (const double *)g.fs[X_U0], g.fs[F_R], g.fs[X_W], (const double *)g.fs[X_KX],
+               (const double *)g.fs[X_KY]);
     const int wbits = g.pending_w;   // w = A p as cg_calc_w forms it (its p.w is not used)
     g.pending_w = 0;
     if (wbits == 32)
@@ -2020,8 +2053,49 @@ template <typename T> void cheby_init_impl(double theta)
     launch(nm<T>("cheby_calc_u"), k_add<T>, interior(), pitch_of<T>(), FT<T>(F_U), (const T *)FT<T>(F_P));
 }
 
+// One Chebyshev iteration.  Without block Jacobi (and unless
+// TEA_FUSED_CHEBY=0) in one streaming pass (k_cheby_march): u + p goes to the
+// spare u buffer, which becomes u, and w, r of the previous u (kept in the
+// spare) are owed (pending_cheby): the next iteration recomputes them, any
+// other entry point writes them out first.
 template <typename T> void cheby_iterate_impl(double alpha, double beta)
 {
+    const char *env = getenv("TEA_FUSED_CHEBY");
+    if (g.prec != PREC_BLOCK && !(env && env[0] == '0')) {
+        const Ext e = interior();
+        const long pt = pitch_of<T>();
+        const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+        if (g.u_ring_dirty) {   // the spare takes u's halo, which u + p keeps
+            Timed t("u_halo_ring");
+            hipLaunchKernelGGL(k_copy_ring<T>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream,
+                               FT<T>(X_U2), (const T *)FT<T>(F_U), pt, g.h, g.nx, g.ny);
+            g.u_ring_dirty = false;
+        }
+        const int nbands = (e.x1 - e.x0 + BEGIN_OUT_X) / BEGIN_OUT_X, ny = e.y1 - e.y0 + 1;
+        int runs = cu_count() * 4 * 8 / nbands;
+        runs = runs < 1 ? 1 : runs;
+        int rows = (ny + runs - 1) / runs;
+        rows = rows < 8 ? 8 : rows;
+        runs = (ny + rows - 1) / rows;
+        const unsigned grid = (unsigned)((nbands * runs + MARCH_WAVES - 1) / MARCH_WAVES);
+        auto go = [&](auto kernel) {
+            klaunch(nm<T>("cheby_iterate"), kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, rows,
+                    (const T *)FT<T>(F_U), FT<T>(X_U2), (const T *)FT<T>(X_U0), FT<T>(F_P), (const T *)FT<T>(X_MI),
+                    (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), alpha, beta);
+        };
+        if (g.prec == PREC_DIAG) go(k_cheby_march<T, PREC_DIAG>);
+        else go(k_cheby_march<T, PREC_NONE>);
+        const hipError_t err = hipGetLastError();
+        if (err != hipSuccess) { die("launch of k_cheby_march failed: %s", hipGetErrorString(err)); return; }
+        if (std::is_same<T, double>::value) {
+            std::swap(g.f[F_U], g.f[X_U2]);
+            std::swap(g.fs[F_U], g.fs[X_U2]);
+        } else {
+            std::swap(g.fs32[F_U], g.fs32[X_U2]);
+        }
+        g.pending_cheby = 8 * (int)sizeof(T);
+        return;
+    }
     if (!cheby_p<T, 0>(0.0, alpha, beta)) return;
     launch(nm<T>("cheby_calc_u"), k_add<T>, interior(), pitch_of<T>(), FT<T>(F_U), (const T *)FT<T>(F_P));
 }
@@ -2038,7 +2112,9 @@ EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const doub
 
 EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
 {
-    CHECK_INIT();
+    CHECK_INIT_KEEP_PENDING();
+    g.pending_cheby = 0;   // this iteration forms w and r again
+    settle_pending();
     REQUIRE_SOLVER(1 << SOLVER_CHEBY);
     const int k = *cheby_calc_step - 1;
     if (k < 0 || k >= (int)g.ch_a.size()) {
@@ -2286,6 +2362,7 @@ EXPORT int tea_engine_set_field(int id, const double *host)
     if (id == F_SD) g.sd_halo_dirty = true;
     if (id == F_P) g.p_halo_dirty[0] = true;
     if (id == X_RTEMP) g.rt_halo_dirty[0] = true;
+    g.u_ring_dirty = true;
     const size_t w = (size_t)g.nx + 2 * g.h;
     HIP_OK_R(hipMemcpy2DAsync(g.f[id] + (g.lead - g.h), g.pitch * sizeof(double), host, w * sizeof(double),
                               w * sizeof(double), (size_t)g.rows, hipMemcpyHostToDevice, g.stream), -1);
@@ -2436,6 +2513,7 @@ EXPORT int tea_engine_get_field32(int id, float *host)
 EXPORT int tea_engine_set_field32(int id, const float *host)
 {
     if (g.initialised) settle_pending();   // rtemp and utemp owed by tea_engine_ppcg_begin_ur
+    g.u_ring_dirty = true;
     if (!g.initialised || !g.slab32 || id < 1 || id >= X_COUNT) return -1;
     const size_t w = (size_t)g.nx + 2 * g.h;
     HIP_OK_R(hipMemcpy2DAsync(g.fs32[id], g.pitch32 * sizeof(float), host, w * sizeof(float), w * sizeof(float),

@@ -1591,6 +1591,109 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long p
     block_reduce<2>(acc, red);
 }
 
+// One Chebyshev iteration (tea_leaf_cheby.cuknl:84-152: w = A u, r = u0 - w,
+// p = alpha p + beta M^-1 r; then calc_u: u += p) as one streaming pass (no
+// reference counterpart as a kernel; the same values): u + p goes to u_out
+// (neighbouring bands read u), w and r are not stored (the caller owes them:
+// both follow from the u read here, which the caller keeps).  Bands of 62
+// columns walk down runs of rows with the rows of u above and below in
+// registers.  No preconditioner or the diagonal one (block Jacobi needs whole
+// strips and keeps the separate kernels).
+template <typename T, int PREC>
+__global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_cheby_march(Ext e, long pitch, int rows,
+                                                                   const T *__restrict__ u, T *__restrict__ u_out,
+                                                                   const T *__restrict__ u0, T *__restrict__ p,
+                                                                   const T *__restrict__ Mi, const T *__restrict__ Kx,
+                                                                   const T *__restrict__ Ky, double alpha_in,
+                                                                   double beta_in)
+{
+    static_assert(PREC != PREC_BLOCK, "block Jacobi keeps the separate kernels");
+    const T alpha = (T)alpha_in, beta = (T)beta_in;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nbands = (e.x1 - e.x0 + BEGIN_OUT_X) / BEGIN_OUT_X;
+    const int ny = e.y1 - e.y0 + 1;
+    const int nruns = (ny + rows - 1) / rows;
+    const int wid = blockIdx.x * MARCH_WAVES + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    if (wid >= nbands * nruns) return;
+    const int band = wid % nbands, run = wid / nbands;
+    const int c = e.x0 + band * BEGIN_OUT_X - 1 + lane;
+    const bool col_in = c >= e.x0 && c <= e.x1;
+    const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+    const bool out = col_in && lane >= 1 && lane <= WAVE - 2;
+    const int ya = e.y0 + run * rows;
+    const int yb = (ya + rows - 1 < e.y1 ? ya + rows - 1 : e.y1);
+    auto uval = [&](int row) -> T {
+        return (col_mem && row >= e.y0 - 1 && row <= e.y1 + 1) ? u[(long)c + (long)row * pitch] : T(0.0);
+    };
+    auto kyv = [&](int row) -> T {
+        return (col_in && row >= e.y0 && row <= e.y1 + 1) ? Ky[(long)c + (long)row * pitch] : T(0.0);
+    };
+    T um = uval(ya - 1), uc = uval(ya);
+    T ky0 = kyv(ya);
+    for (int row = ya; row <= yb; row++) {
+        const long i = (long)c + (long)row * pitch;
+        const T up = uval(row + 1);
+        const T ky1 = kyv(row + 1);
+        const T kx0 = col_mem ? Kx[i] : T(0.0);
+        const T kx1 = lane_from_above(kx0);
+        const T left = lane_from_below(uc), right = lane_from_above(uc);
+        const T wv = (T(1.0) + (ky1 + ky0) + (kx1 + kx0)) * uc - (ky1 * up + ky0 * um) - (kx1 * right + kx0 * left);
+        if (out) {
+            const T rv = u0[i] - wv;
+            T pv;
+            if (PREC == PREC_DIAG) pv = alpha * p[i] + beta * Mi[i] * rv;
+            else pv = alpha * p[i] + beta * rv;
+            p[i] = pv;
+            u_out[i] = uc + pv;
+        }
+        um = uc;
+        uc = up;
+        ky0 = ky1;
+    }
+}
+
+// w = A u and r = u0 - w on the interior, for u the iterate a fused
+// Chebyshev iteration read (the w and r it left owed): the expressions of
+// k_cheby_p.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_cheby_wr(Ext e, long pitch, const T *__restrict__ u,
+                                                    const T *__restrict__ u0, T *__restrict__ r, T *__restrict__ w,
+                                                    const T *__restrict__ Kx, const T *__restrict__ Ky)
+{
+    STRIP_BEGIN
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const T wv = apply_A(u, Kx, Ky, i, pitch);
+            w[i] = wv;
+            r[i] = u0[i] - wv;
+        }
+    }
+    STRIP_END
+}
+
+// sum r*r with r = u0 - A u formed on the fly (the r a fused Chebyshev
+// iteration left owed): calc_2norm(1) of the r k_cheby_wr would write.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_cheby_rr(Ext e, long pitch, const T *__restrict__ u,
+                                                    const T *__restrict__ u0, const T *__restrict__ Kx,
+                                                    const T *__restrict__ Ky, Red red)
+{
+    DD acc[1] = {{0, 0}};
+    STRIP_BEGIN
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const T rv = u0[i] - apply_A(u, Kx, Ky, i, pitch);
+            dd_add_prod(acc[0], (double)rv, (double)rv);
+        }
+    }
+    STRIP_END
+    block_reduce<1>(acc, red);
+}
+
 // tea_leaf_ppcg.cuknl:279-297: sum (r - r_store)*z
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_calc_rrn(Ext e, long pitch, const T *__restrict__ rstore,

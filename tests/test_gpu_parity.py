@@ -107,17 +107,19 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 
 
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
-                                    "TEA_W_FROM_P", "TEA_FUSED_CG"])
+                                    "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
-                                  "cg_block_ragged"])
+                                  "cg_block_ragged", "cheby_none", "mixed_cheby_diag"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
     """The driver's separate-call PPCG path (TEA_FUSED_OUTER=0), the fused
     outer iteration with one inner step per pass (TEA_INNER_PASSES=0), the
     inner passes on the tile kernels (TEA_INNER_MARCH=0), the refinement
     loop without inner passes queued ahead of the replacement test
     (TEA_SPECULATE=0), the FP32 fused pair with w written by begin and read
-    by ur (TEA_W_FROM_P=0), and CG iterations through the separate calls
-    instead of the fused one (TEA_FUSED_CG=0), on the GPU give the same
+    by ur (TEA_W_FROM_P=0), CG iterations through the separate calls
+    instead of the fused one (TEA_FUSED_CG=0), the fused begins on the tile
+    kernels (TEA_BEGIN_MARCH=0), and Chebyshev iterations on the separate
+    calc_p / calc_u kernels (TEA_FUSED_CHEBY=0), on the GPU give the same
     results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
@@ -256,6 +258,13 @@ PPCG_CALLS = {
 CHEBY_CALLS = {
     "cheby_init": lambda e: e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3),
     "cheby_iterate": lambda e: (e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3), e.cheby_iterate(3)),
+    # the fused iteration leaves w and r owed: iterations in a row, a norm between them, a halo update
+    "cheby_iterate_three": lambda e: (e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3), e.cheby_iterate(1),
+                                      e.cheby_iterate(2), e.cheby_iterate(3)),
+    "cheby_iterate_norm_iterate": lambda e: (e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3), e.cheby_iterate(1),
+                                             e.calc_2norm(1), e.cheby_iterate(2)),
+    "cheby_iterate_halo_iterate": lambda e: (e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3), e.cheby_iterate(1),
+                                             e.update_halo([0, 0, 0, 1, 0, 0, 0], 2), e.cheby_iterate(2)),
 }
 
 

@@ -18,7 +18,9 @@ sys.path.insert(0, str(REPO))
 
 GOLDEN = REPO / "tests" / "golden"
 ORACLE_DIR = REPO / "oracle"
-ORACLE_SO = ORACLE_DIR / "liboracle.so"
+# TEA_ORACLE_LIB names another build of the oracle (the sanitizer build of
+# oracle/Makefile's `sanitize` target is run this way)
+ORACLE_SO = Path(os.environ["TEA_ORACLE_LIB"]) if os.environ.get("TEA_ORACLE_LIB") else ORACLE_DIR / "liboracle.so"
 REF_ORACLE = ORACLE_DIR / "_ref" / "tea_leaf_ref_oracle"
 REF_HIP = ORACLE_DIR / "_ref" / "tea_leaf_ref_hip"
 

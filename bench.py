@@ -110,6 +110,13 @@ def variants(n):
     return {
         "headline": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n)))),
         "fp32_extrapolate": dict(inner_precision="fp32", warm_start="extrapolate"),
+        # the same with the differences of the last 4 steps in the Galerkin span (tl_warm_start_history)
+        "headline_h4": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n))), warm_start_history=4),
+        "fp32_extrapolate_h4": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=4),
+        "fp32_extrapolate_h3": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=3),
+        "fp32_extrapolate_h5": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=5),
+        "fp32_extrapolate_h6": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=6),
+        "fp32_extrapolate_h8": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=8),
         "fp32": dict(inner_precision="fp32"),
         "fp64_true": dict(inner_precision="fp64"),
         "fp64_reference": dict(),

@@ -12,8 +12,10 @@
  *     f2 = u^n / s_u,             s_u = RMS over the mesh of u^n
  * (trained to output (u^{n+1} - u^n - (u^n - u^{n-1})) / s).  The guess
  * handed to the solver is
- *     u = u^n + c0 (u^n - u^{n-1}) + c1 (u^{n-1} - u^{n-2}) + c2 s net,
- * with the coefficients that minimise the error's energy norm
+ *     u = u^n + c0 (u^n - u^{n-1}) + c1 (u^{n-1} - u^{n-2}) + ... + cN s net,
+ * over the differences of the last H steps (the deck key
+ * tl_warm_start_history H, 1..8, default 2), with the coefficients that
+ * minimise the error's energy norm
  * ||u* - u||_A over that span (a Galerkin step: G c = g with
  * G_ab = v_a . A v_b, g_a = v_a . (u0 - A u^n)), so the guess is never worse
  * in that norm than u^n, the linear extrapolation or the network's own guess
@@ -56,7 +58,8 @@ int tea_engine_warm_start_load(const float *params, int n);
 void tea_engine_warm_start_apply(int *applied);
 /* Forgets the previous steps (the next apply only records u^n). */
 void tea_engine_warm_start_reset(void);
-/* c0, c1, c2 of the last guess (all 0 when it only recorded u^n). */
+/* The last guess's coefficients of the first two history differences and of
+ * the network's direction (c[0], c[1], c[2]; all 0 when it only recorded u^n). */
 void tea_engine_warm_start_coefficients(double *c);
 /* Copies the interior of field field_id (TEA_FIELD_* / TEA_XFIELD_*) into a
  * device array of ny x nx doubles (row-major, x fastest).  Synchronous. */

@@ -52,6 +52,11 @@ enum FieldId {
     N_SLOTS
 };
 enum { SOLVER_JACOBI = 0, SOLVER_CG = 1, SOLVER_CHEBY = 2, SOLVER_PPCG = 3 };
+// most previous-step differences the warm start combines (tl_warm_start_history)
+constexpr int WS_HMAX = 8;
+struct WsCoefs {
+    double c[WS_HMAX + 1];
+};
 constexpr int S_CG_FAMILY = (1 << SOLVER_CG) | (1 << SOLVER_CHEBY) | (1 << SOLVER_PPCG);
 
 struct Profiler {
@@ -200,9 +205,12 @@ struct Chunk {
     ws::Acts ws_acts;
     float *ws_planes = nullptr;   // 3 x ny x nx features
     double *ws_slab = nullptr;
-    double *ws_up1 = nullptr, *ws_up2 = nullptr;
-    int ws_nprev = 0;             // previous u0 fields held (0..2)
-    double ws_coef[3] = {0, 0, 0};  // coefficients of the last guess
+    // previous u0 fields, most recent first (a ring: the oldest buffer takes
+    // the next u0), ws_hist of them in use
+    double *ws_prev[WS_HMAX] = {nullptr};
+    int ws_hist = 2;              // previous-step differences in the guess (tl_warm_start_history)
+    int ws_nprev = 0;             // previous u0 fields held (0..ws_hist)
+    double ws_coef[WS_HMAX + 1] = {0};  // coefficients of the last guess (history..., network)
 
     Profiler prof;
 };
@@ -710,15 +718,18 @@ __global__ __launch_bounds__(BLOCK) void k_ws_features(Ext e, long pitch, int h,
     STRIP_END
 }
 
-// Search directions of the warm start at interior cell i:
-//   v0 = u0 - up1, v1 = up1 - up2 (when nv > 1 previous steps), v2 = scale * net
-// (when net != nullptr); the network plane is dense, its cells outside the
-// interior count as 0 (their faces carry K = 0 under the zero-flux boundary).
+// Search directions of the warm start at interior cell i: the differences of
+// the previous steps v_j = up_j - up_{j+1} (up_0 = u0, up_1 = the previous
+// step's u0, ...) for j < nh, and v_HN = scale * net (when net != nullptr);
+// the network plane is dense, its cells outside the interior count as 0
+// (their faces carry K = 0 under the zero-flux boundary).
 struct WsDirs {
-    const double *u0, *up1, *up2;
+    const double *u0;
+    const double *up[WS_HMAX];
     const float *net;
     double scale;
     int h, nx, ny;
+    int nh;   // history differences available (0..HN)
 };
 
 __device__ __forceinline__ double ws_net_at(const WsDirs &d, int c, int r)
@@ -727,72 +738,107 @@ __device__ __forceinline__ double ws_net_at(const WsDirs &d, int c, int r)
     return d.scale * (double)d.net[(long)(r - d.h) * d.nx + (c - d.h)];
 }
 
-// Gram matrix and right-hand side of the energy-minimising combination:
-// G[a][b] = v_a . A v_b and g[a] = v_a . r0 with r0 = u0 - A u0 (the residual
-// of the u = u^n guess), packed as G00 G01 G02 G11 G12 G22 g0 g1 g2.
-__global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
-                                                   const double *__restrict__ Ky, int two, Red red)
+// Combines per-lane sums over the workgroup in compensated arithmetic (each
+// lane's sum covers the few cells of its strips) and publishes the
+// workgroup's partials; one value's wave reduction at a time, so the lanes
+// keep only their plain sums.
+template <int NV>
+__device__ __forceinline__ void block_reduce_lanes(const double (&v)[NV], const Red red)
 {
-    DD acc[9];
+    __shared__ DD lds[NV][WAVES];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
 #pragma unroll
-    for (int j = 0; j < 9; j++) acc[j] = DD{0.0, 0.0};
+    for (int j = 0; j < NV; j++) {
+        const DD t = wave_reduce(DD{v[j], 0.0});
+        if (lane == 0) lds[j][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < NV; j++) {
+            DD t = lds[j][0];
+            for (int w = 1; w < WAVES; w++) dd_merge(t, lds[j][w]);
+            red.partials[j * red.max_blocks + blockIdx.x] = t;
+        }
+    }
+}
+
+// Gram matrix and right-hand side of the energy-minimising combination of
+// HN history directions and the network's: G[a][b] = v_a . A v_b (a <= b, row
+// by row) then g[a] = v_a . r0, r0 = u0 - A u0 (the residual of the u = u^n
+// guess); directions not present count 0.
+template <int HN>
+__global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
+                                                   const double *__restrict__ Ky, Red red)
+{
+    constexpr int D = HN + 1, NG = D * (D + 1) / 2;
+    double acc[NG + D];
+#pragma unroll
+    for (int j = 0; j < NG + D; j++) acc[j] = 0.0;
     STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         const int c = s.c, r = s.r0 + k;
         const double a_u0 = apply_A(d.u0, Kx, Ky, i, pitch);
-        const double a_p1 = apply_A(d.up1, Kx, Ky, i, pitch);
-        double v[3], av[3];
-        v[0] = d.u0[i] - d.up1[i];
-        av[0] = a_u0 - a_p1;
-        v[1] = 0.0;
-        av[1] = 0.0;
-        if (two) {
-            v[1] = d.up1[i] - d.up2[i];
-            av[1] = a_p1 - apply_A(d.up2, Kx, Ky, i, pitch);
+        double v[D], av[D];
+        double up_prev = d.u0[i], a_prev = a_u0;
+#pragma unroll
+        for (int j = 0; j < HN; j++) {
+            v[j] = 0.0;
+            av[j] = 0.0;
+            if (j < d.nh) {
+                const double a_j = apply_A(d.up[j], Kx, Ky, i, pitch);
+                v[j] = up_prev - d.up[j][i];
+                av[j] = a_prev - a_j;
+                up_prev = d.up[j][i];
+                a_prev = a_j;
+            }
         }
-        v[2] = 0.0;
-        av[2] = 0.0;
+        v[HN] = 0.0;
+        av[HN] = 0.0;
         if (d.net) {
             const double q = ws_net_at(d, c, r);
-            v[2] = q;
-            av[2] = (1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * q -
-                    (Ky[i + pitch] * ws_net_at(d, c, r + 1) + Ky[i] * ws_net_at(d, c, r - 1)) -
-                    (Kx[i + 1] * ws_net_at(d, c + 1, r) + Kx[i] * ws_net_at(d, c - 1, r));
+            v[HN] = q;
+            av[HN] = (1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * q -
+                     (Ky[i + pitch] * ws_net_at(d, c, r + 1) + Ky[i] * ws_net_at(d, c, r - 1)) -
+                     (Kx[i + 1] * ws_net_at(d, c + 1, r) + Kx[i] * ws_net_at(d, c - 1, r));
         }
         const double r0 = d.u0[i] - a_u0;
-        dd_add_prod(acc[0], v[0], av[0]);
-        dd_add_prod(acc[1], v[0], av[1]);
-        dd_add_prod(acc[2], v[0], av[2]);
-        dd_add_prod(acc[3], v[1], av[1]);
-        dd_add_prod(acc[4], v[1], av[2]);
-        dd_add_prod(acc[5], v[2], av[2]);
-        dd_add_prod(acc[6], v[0], r0);
-        dd_add_prod(acc[7], v[1], r0);
-        dd_add_prod(acc[8], v[2], r0);
+        int q = 0;
+#pragma unroll
+        for (int x = 0; x < D; x++)
+#pragma unroll
+            for (int y = x; y < D; y++) acc[q++] += v[x] * av[y];
+#pragma unroll
+        for (int x = 0; x < D; x++) acc[NG + x] += v[x] * r0;
     }
     STRIP_END
-    block_reduce<9>(acc, red);
+    block_reduce_lanes<NG + D>(acc, red);
 }
 
-// u = u0 + c0 v0 + c1 v1 + c2 v2; then up2 = up1, up1 = u0 (apply == 0: only
-// the shift of the previous fields)
+// u = u0 + sum_j c[j] v_j + c[HN] v_HN (apply == 0: no guess); and u0 goes to
+// `keep`, the buffer of the oldest previous field read here (the host then
+// rotates the ring: keep becomes up_1).
+template <int HN>
 __global__ __launch_bounds__(BLOCK) void k_ws_apply(Ext e, long pitch, double *__restrict__ u, WsDirs d,
-                                                    double *__restrict__ up1, double *__restrict__ up2, double c0,
-                                                    double c1, double c2, int apply)
+                                                    double *__restrict__ keep, WsCoefs cf, int apply)
 {
     STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
-        const double u0 = d.u0[i], p1 = up1[i];
+        const double u0 = d.u0[i];
         if (apply) {
-            double v = u0 + c0 * (u0 - p1);
-            if (c1 != 0.0) v += c1 * (p1 - up2[i]);
-            if (c2 != 0.0) v += c2 * ws_net_at(d, s.c, s.r0 + k);
+            double v = u0 + cf.c[0] * (u0 - d.up[0][i]);
+            double up_prev = d.up[0][i];
+#pragma unroll
+            for (int j = 1; j < HN; j++) {
+                if (j < d.nh && cf.c[j] != 0.0) v += cf.c[j] * (up_prev - d.up[j][i]);
+                if (j < d.nh) up_prev = d.up[j][i];
+            }
+            if (cf.c[HN] != 0.0) v += cf.c[HN] * ws_net_at(d, s.c, s.r0 + k);
             u[i] = v;
         }
-        up2[i] = p1;
-        up1[i] = u0;
+        keep[i] = u0;
     }
     STRIP_END
 }
@@ -2542,11 +2588,16 @@ EXPORT int tea_engine_warm_start_load(const float *params, int n)
         return -1;
     }
     if (!g.ws_slab) {
+        const int hist = read_int("tl_warm_start_history");
+        if (hist != -1 && (hist < 1 || hist > WS_HMAX)) {
+            die("tl_warm_start_history %d outside 1..%d", hist, WS_HMAX);
+            return -1;
+        }
+        g.ws_hist = hist == -1 ? 2 : hist;
         const size_t elems = (size_t)g.pitch * (size_t)g.rows + 16;
-        HIP_OK_R(hipMalloc(&g.ws_slab, 2 * elems * sizeof(double)), -1);
-        HIP_OK_R(hipMemset(g.ws_slab, 0, 2 * elems * sizeof(double)), -1);
-        g.ws_up1 = g.ws_slab + (g.fs[F_U] - g.f[F_U]);
-        g.ws_up2 = g.ws_up1 + elems;
+        HIP_OK_R(hipMalloc(&g.ws_slab, (size_t)g.ws_hist * elems * sizeof(double)), -1);
+        HIP_OK_R(hipMemset(g.ws_slab, 0, (size_t)g.ws_hist * elems * sizeof(double)), -1);
+        for (int j = 0; j < g.ws_hist; j++) g.ws_prev[j] = g.ws_slab + (g.fs[F_U] - g.f[F_U]) + (size_t)j * elems;
     }
     if (params) {
         if (!g.ws_net.params) {
@@ -2568,10 +2619,11 @@ namespace {
 // Solves the k x k system G c = b (G symmetric positive semi-definite) by
 // Cholesky, dropping a direction whose pivot falls below 1e-12 of its
 // diagonal (linearly dependent on the earlier ones); c = 0 for dropped ones.
-void ws_solve(int k, double G[3][3], const double *b, double *c)
+constexpr int WS_DMAX = WS_HMAX + 1;
+void ws_solve(int k, double G[WS_DMAX][WS_DMAX], const double *b, double *c)
 {
-    double L[3][3] = {{0}};
-    bool use[3] = {false, false, false};
+    double L[WS_DMAX][WS_DMAX] = {{0}};
+    bool use[WS_DMAX] = {false};
     for (int j = 0; j < k; j++) {
         double dj = G[j][j];
         for (int m = 0; m < j; m++) dj -= L[j][m] * L[j][m];
@@ -2584,7 +2636,7 @@ void ws_solve(int k, double G[3][3], const double *b, double *c)
             L[i][j] = v / L[j][j];
         }
     }
-    double y[3] = {0, 0, 0};
+    double y[WS_DMAX] = {0};
     for (int i = 0; i < k; i++) {
         if (!use[i]) continue;
         double v = b[i];
@@ -2601,18 +2653,39 @@ void ws_solve(int k, double G[3][3], const double *b, double *c)
 }
 }  // namespace
 
+namespace {
+template <int HN> bool ws_gram_launch(const WsDirs &d)
+{
+    return launch("ws_gram", k_ws_gram<HN>, interior(), g.pitch, d, (const double *)F(X_KX), (const double *)F(X_KY),
+                  red_ctx());
+}
+template <int HN> bool ws_apply_launch(const WsDirs &d, double *keep, const WsCoefs &cf, int apply)
+{
+    return launch("ws_apply", k_ws_apply<HN>, interior(), g.pitch, F(F_U), d, keep, cf, apply);
+}
+}  // namespace
+
 EXPORT void tea_engine_warm_start_apply(int *applied)
 {
     CHECK_INIT();
     *applied = 0;
     if (!g.ws_on) { die("tea_engine_warm_start_apply: tea_engine_warm_start_load was not called"); return; }
     const Ext in = interior();
-    WsDirs d{(const double *)F(X_U0), g.ws_up1, g.ws_up2, nullptr, 0.0, g.h, g.nx, g.ny};
-    double c[3] = {0.0, 0.0, 0.0};
+    const int HN = g.ws_hist;
+    WsDirs d{};
+    d.u0 = (const double *)F(X_U0);
+    for (int j = 0; j < HN; j++) d.up[j] = g.ws_prev[j];
+    d.net = nullptr;
+    d.scale = 0.0;
+    d.h = g.h;
+    d.nx = g.nx;
+    d.ny = g.ny;
+    d.nh = g.ws_nprev < HN ? g.ws_nprev : HN;
+    WsCoefs cf{};
     int apply = 0;
     if (g.ws_nprev >= 1) {
         if (g.ws_use_net) {
-            if (!launch("ws_stats", k_ws_stats, in, g.pitch, d.u0, (const double *)g.ws_up1, red_ctx())) return;
+            if (!launch("ws_stats", k_ws_stats, in, g.pitch, d.u0, (const double *)g.ws_prev[0], red_ctx())) return;
             double sums[2];
             if (!fetch(sums, 2)) return;
             const double cells = (double)g.nx * (double)g.ny;
@@ -2620,7 +2693,7 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
             if (sc > 0.0 && su > 0.0) {
                 const long n = (long)g.nx * g.ny;
                 if (!launch("ws_features", k_ws_features, in, g.pitch, g.h, g.nx, n, (const double *)F(F_DENSITY),
-                            d.u0, (const double *)g.ws_up1, 1.0 / sc, 1.0 / su, g.ws_planes))
+                            d.u0, (const double *)g.ws_prev[0], 1.0 / sc, 1.0 / su, g.ws_planes))
                     return;
                 {
                     Timed t("ws_network");
@@ -2632,37 +2705,67 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
                 d.scale = sc;
             }
         }
-        const int two = g.ws_nprev >= 2 ? 1 : 0;
-        if (!launch("ws_gram", k_ws_gram, in, g.pitch, d, (const double *)F(X_KX), (const double *)F(X_KY), two,
-                    red_ctx()))
-            return;
-        double r[9];
-        if (!fetch(r, 9)) return;
-        double G[3][3] = {{r[0], r[1], r[2]}, {r[1], r[3], r[4]}, {r[2], r[4], r[5]}};
-        double b[3] = {r[6], r[7], r[8]};
-        // active directions in order: v0, [v1], [v2]
-        int idx[3], k = 0;
-        idx[k++] = 0;
-        if (two) idx[k++] = 1;
-        if (d.net) idx[k++] = 2;
-        double Gs[3][3], bs[3], cs[3];
+        bool ok;
+        switch (HN) {
+        case 1: ok = ws_gram_launch<1>(d); break;
+        case 2: ok = ws_gram_launch<2>(d); break;
+        case 3: ok = ws_gram_launch<3>(d); break;
+        case 4: ok = ws_gram_launch<4>(d); break;
+        case 5: ok = ws_gram_launch<5>(d); break;
+        case 6: ok = ws_gram_launch<6>(d); break;
+        case 7: ok = ws_gram_launch<7>(d); break;
+        default: ok = ws_gram_launch<8>(d); break;
+        }
+        if (!ok) return;
+        const int D = HN + 1, NG = D * (D + 1) / 2;
+        double r[MAX_RED];
+        if (!fetch(r, NG + D)) return;
+        double G[WS_DMAX][WS_DMAX], b[WS_DMAX];
+        for (int x = 0, q = 0; x < D; x++)
+            for (int y = x; y < D; y++, q++) G[x][y] = G[y][x] = r[q];
+        for (int x = 0; x < D; x++) b[x] = r[NG + x];
+        // active directions in order: the history differences held, [the network]
+        int idx[WS_DMAX], k = 0;
+        for (int j = 0; j < d.nh; j++) idx[k++] = j;
+        if (d.net) idx[k++] = HN;
+        double Gs[WS_DMAX][WS_DMAX], bs[WS_DMAX], cs[WS_DMAX];
         for (int a = 0; a < k; a++) {
             bs[a] = b[idx[a]];
             for (int q = 0; q < k; q++) Gs[a][q] = G[idx[a]][idx[q]];
         }
         ws_solve(k, Gs, bs, cs);
-        for (int a = 0; a < k; a++) c[idx[a]] = cs[a];
+        for (int a = 0; a < k; a++) cf.c[idx[a]] = cs[a];
         apply = 1;
     }
-    for (int a = 0; a < 3; a++) g.ws_coef[a] = c[a];
-    if (!launch("ws_apply", k_ws_apply, in, g.pitch, F(F_U), d, g.ws_up1, g.ws_up2, c[0], c[1], c[2], apply)) return;
-    g.ws_nprev = g.ws_nprev < 2 ? g.ws_nprev + 1 : 2;
+    for (int a = 0; a <= WS_HMAX; a++) g.ws_coef[a] = 0.0;
+    for (int a = 0; a <= HN; a++) g.ws_coef[a] = cf.c[a];
+    // u0 goes to the oldest buffer of the ring, which then becomes up_1
+    double *keep = g.ws_prev[HN - 1];
+    bool ok;
+    switch (HN) {
+    case 1: ok = ws_apply_launch<1>(d, keep, cf, apply); break;
+    case 2: ok = ws_apply_launch<2>(d, keep, cf, apply); break;
+    case 3: ok = ws_apply_launch<3>(d, keep, cf, apply); break;
+    case 4: ok = ws_apply_launch<4>(d, keep, cf, apply); break;
+    case 5: ok = ws_apply_launch<5>(d, keep, cf, apply); break;
+    case 6: ok = ws_apply_launch<6>(d, keep, cf, apply); break;
+    case 7: ok = ws_apply_launch<7>(d, keep, cf, apply); break;
+    default: ok = ws_apply_launch<8>(d, keep, cf, apply); break;
+    }
+    if (!ok) return;
+    for (int j = HN - 1; j > 0; j--) g.ws_prev[j] = g.ws_prev[j - 1];
+    g.ws_prev[0] = keep;
+    g.ws_nprev = g.ws_nprev < HN ? g.ws_nprev + 1 : HN;
     *applied = apply;
 }
 
+// The coefficients of the last guess: c[0], c[1] the first two history
+// differences, c[2] the network's direction.
 EXPORT void tea_engine_warm_start_coefficients(double *c)
 {
-    for (int a = 0; a < 3; a++) c[a] = g.ws_coef[a];
+    c[0] = g.ws_coef[0];
+    c[1] = g.ws_hist > 1 ? g.ws_coef[1] : 0.0;
+    c[2] = g.ws_coef[g.ws_hist];
 }
 
 EXPORT int tea_engine_copy_field(int id, double *dst)

@@ -27,7 +27,7 @@ constexpr int WAVES = 4;
 constexpr int TILE_Y = STRIP * WAVES;
 constexpr int BLOCK = WAVE * WAVES;
 constexpr int NUM_XCD = 8;
-constexpr int MAX_RED = 9;        // values reduced by one launch at most
+constexpr int MAX_RED = 54;       // values reduced by one launch at most
 
 enum { PREC_NONE = 1, PREC_DIAG = 2, PREC_BLOCK = 3 };
 

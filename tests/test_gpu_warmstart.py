@@ -126,21 +126,25 @@ def _operator(eng, h, nx, ny):
     return A
 
 
-@pytest.mark.parametrize("with_net", [True, False])
-def test_engine_apply_matches_oracle(with_net, tmp_path):
+@pytest.mark.parametrize("with_net,history", [(True, None), (False, None), (True, 4), (False, 3), (False, 1), (False, 6), (True, 8)])
+def test_engine_apply_matches_oracle(with_net, history, tmp_path):
     """tea_engine_warm_start_apply on engine fields: the guess u0 + sum c_a v_a
-    with the energy-minimising coefficients, computed here from the oracle's
-    network output and a dense restatement of the operator."""
+    over the differences of the last `history` steps (default 2) and the
+    network's direction, with the energy-minimising coefficients, computed
+    here from the oracle's network output and a dense restatement of the
+    operator."""
     import ctypes
 
-    print(f"[gpu warmstart] engine apply (network {with_net})", flush=True)
+    print(f"[gpu warmstart] engine apply (network {with_net}, history {history})", flush=True)
     from test_oracle_math import setup_chunk
 
     lib = wsm.library()
     eng = Engine()
+    hist = history or 2
     try:
         nx, ny = 64, 8
-        setup_chunk(eng, tmp_path, nx=nx, ny=ny, solver="cg", halo=1)
+        setup_chunk(eng, tmp_path, nx=nx, ny=ny, solver="cg", halo=1,
+                    extra=[f"tl_warm_start_history={history}"] if history else ())
         p = _params(7, w6_scale=0.2)
         if with_net:
             assert lib.tea_engine_warm_start_load(p.ctypes.data_as(wsm._F), p.size) == 0
@@ -159,7 +163,7 @@ def test_engine_apply_matches_oracle(with_net, tmp_path):
         rho = 10.0 ** rng.uniform(-1, 2, (ny, nx))
         put(rt.FIELD_DENSITY, rho)
         us = [rng.uniform(1.0, 3.0, (ny, nx))]
-        for _ in range(2):
+        for _ in range(hist + 1):
             us.append(us[-1] + rng.standard_normal((ny, nx)) * 0.1)
         applied = ctypes.c_int(-1)
         for k, u0 in enumerate(us):
@@ -169,9 +173,7 @@ def test_engine_apply_matches_oracle(with_net, tmp_path):
             assert applied.value == (1 if k else 0)
             if k == 0:
                 continue
-            dirs = [u0 - us[k - 1]]
-            if k >= 2:
-                dirs.append(us[k - 1] - us[k - 2])
+            dirs = [us[k - j] - us[k - j - 1] for j in range(min(k, hist))]
             if with_net:
                 d = dirs[0]
                 sc, su = np.sqrt(np.mean(d * d)), np.sqrt(np.mean(u0 * u0))

@@ -16,7 +16,7 @@ the residual of the u = u^n guess:
 * headline: learned warm start + FP32 fields with FP64 dot accumulators and
   FP64 residual replacement (the network's weights were trained on the GPU
   from plain FP64 PPCG solves at this mesh size);
-* fp32_extrapolate: the same without the network (the energy-minimising
+* fp32_extrapolate: the same without the network (the residual-minimising
   combination of the previous steps' differences only);
 * fp32: FP32 refinement from the u = u^n guess;
 * fp64_true: plain FP64 PPCG with the same true-residual stop;
@@ -110,13 +110,13 @@ def variants(n):
     return {
         "headline": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n)))),
         "fp32_extrapolate": dict(inner_precision="fp32", warm_start="extrapolate"),
-        # the same with the differences of the last 4 steps in the Galerkin span (tl_warm_start_history)
-        "headline_h4": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n))), warm_start_history=4),
+        # the Galerkin span over the differences of fewer previous steps (tl_warm_start_history, default 6)
+        "headline_h2": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n))), warm_start_history=2),
+        "fp32_extrapolate_h2": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=2),
         "fp32_extrapolate_h4": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=4),
-        "fp32_extrapolate_h3": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=3),
-        "fp32_extrapolate_h5": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=5),
-        "fp32_extrapolate_h6": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=6),
         "fp32_extrapolate_h8": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=8),
+        # the coefficients that minimise the error's energy norm instead of the residual's 2-norm
+        "fp32_extrapolate_energy": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_norm="energy"),
         "fp32": dict(inner_precision="fp32"),
         "fp64_true": dict(inner_precision="fp64"),
         "fp64_reference": dict(),

@@ -14,14 +14,15 @@
  * handed to the solver is
  *     u = u^n + c0 (u^n - u^{n-1}) + c1 (u^{n-1} - u^{n-2}) + ... + cN s net,
  * over the differences of the last H steps (the deck key
- * tl_warm_start_history H, 1..8, default 2), with the coefficients that
- * minimise the error's energy norm
- * ||u* - u||_A over that span (a Galerkin step: G c = g with
- * G_ab = v_a . A v_b, g_a = v_a . (u0 - A u^n)), so the guess is never worse
- * in that norm than u^n, the linear extrapolation or the network's own guess
- * (c = 1, 0, 1).  The solver still runs to the same FP64 residual target,
- * measured from the u = u^n guess, so the guess can only change the
- * iteration count.
+ * tl_warm_start_history H, 1..8, default 6), with the coefficients that
+ * minimise the 2-norm of the residual ||u0 - A u||_2 over that span (least
+ * squares: G c = g with G_ab = A v_a . A v_b, g_a = A v_a . (u0 - A u^n)), or
+ * with tl_warm_start_norm=energy the error's energy norm ||u* - u||_A (a
+ * Galerkin step: G_ab = v_a . A v_b, g_a = v_a . (u0 - A u^n)).  Either way
+ * the guess is never worse in that norm than u^n, the linear extrapolation
+ * or the network's own guess (c = 1, 0, 1).  The solver still runs to the
+ * same FP64 residual target, measured from the u = u^n guess, so the guess
+ * can only change the iteration count.
  *
  * The parameter vector (tea_ws_param_count() floats) is, in order:
  *     W1[27][32] (k = tap*3 + feature, tap = (dy+1)*3 + (dx+1)), b1[32],

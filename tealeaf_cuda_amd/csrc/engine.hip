@@ -208,7 +208,9 @@ struct Chunk {
     // previous u0 fields, most recent first (a ring: the oldest buffer takes
     // the next u0), ws_hist of them in use
     double *ws_prev[WS_HMAX] = {nullptr};
-    int ws_hist = 2;              // previous-step differences in the guess (tl_warm_start_history)
+    int ws_hist = 6;              // previous-step differences in the guess (tl_warm_start_history)
+    bool ws_minres = true;        // minimise the residual's 2-norm (tl_warm_start_norm residual) or the
+                                  // error's energy norm (energy)
     int ws_nprev = 0;             // previous u0 fields held (0..ws_hist)
     double ws_coef[WS_HMAX + 1] = {0};  // coefficients of the last guess (history..., network)
 
@@ -763,11 +765,13 @@ __device__ __forceinline__ void block_reduce_lanes(const double (&v)[NV], const 
     }
 }
 
-// Gram matrix and right-hand side of the energy-minimising combination of
-// HN history directions and the network's: G[a][b] = v_a . A v_b (a <= b, row
-// by row) then g[a] = v_a . r0, r0 = u0 - A u0 (the residual of the u = u^n
-// guess); directions not present count 0.
-template <int HN>
+// Gram matrix and right-hand side of the combination of HN history
+// directions and the network's that minimises the energy norm of the error
+// (MINRES false: G[a][b] = v_a . A v_b (a <= b, row by row) then g[a] = v_a .
+// r0) or the 2-norm of the residual (MINRES true: G[a][b] = A v_a . A v_b,
+// g[a] = A v_a . r0); r0 = u0 - A u0 is the residual of the u = u^n guess, and
+// directions not present count 0.
+template <int HN, bool MINRES>
 __global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
                                                    const double *__restrict__ Ky, Red red)
 {
@@ -808,9 +812,9 @@ __global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, 
 #pragma unroll
         for (int x = 0; x < D; x++)
 #pragma unroll
-            for (int y = x; y < D; y++) acc[q++] += v[x] * av[y];
+            for (int y = x; y < D; y++) acc[q++] += (MINRES ? av[x] : v[x]) * av[y];
 #pragma unroll
-        for (int x = 0; x < D; x++) acc[NG + x] += v[x] * r0;
+        for (int x = 0; x < D; x++) acc[NG + x] += (MINRES ? av[x] : v[x]) * r0;
     }
     STRIP_END
     block_reduce_lanes<NG + D>(acc, red);
@@ -2593,7 +2597,13 @@ EXPORT int tea_engine_warm_start_load(const float *params, int n)
             die("tl_warm_start_history %d outside 1..%d", hist, WS_HMAX);
             return -1;
         }
-        g.ws_hist = hist == -1 ? 2 : hist;
+        g.ws_hist = hist == -1 ? 6 : hist;
+        const std::string norm = read_string("tl_warm_start_norm");
+        if (!norm.empty() && norm != "residual" && norm != "energy") {
+            die("tl_warm_start_norm %s: expected residual or energy", norm.c_str());
+            return -1;
+        }
+        g.ws_minres = norm != "energy";
         const size_t elems = (size_t)g.pitch * (size_t)g.rows + 16;
         HIP_OK_R(hipMalloc(&g.ws_slab, (size_t)g.ws_hist * elems * sizeof(double)), -1);
         HIP_OK_R(hipMemset(g.ws_slab, 0, (size_t)g.ws_hist * elems * sizeof(double)), -1);
@@ -2656,8 +2666,11 @@ void ws_solve(int k, double G[WS_DMAX][WS_DMAX], const double *b, double *c)
 namespace {
 template <int HN> bool ws_gram_launch(const WsDirs &d)
 {
-    return launch("ws_gram", k_ws_gram<HN>, interior(), g.pitch, d, (const double *)F(X_KX), (const double *)F(X_KY),
-                  red_ctx());
+    if (g.ws_minres)
+        return launch("ws_gram", k_ws_gram<HN, true>, interior(), g.pitch, d, (const double *)F(X_KX),
+                      (const double *)F(X_KY), red_ctx());
+    return launch("ws_gram", k_ws_gram<HN, false>, interior(), g.pitch, d, (const double *)F(X_KX),
+                  (const double *)F(X_KY), red_ctx());
 }
 template <int HN> bool ws_apply_launch(const WsDirs &d, double *keep, const WsCoefs &cf, int apply)
 {

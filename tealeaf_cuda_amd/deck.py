@@ -131,7 +131,8 @@ class Deck:
     inner_eps: float | None = None
     max_refinements: int | None = None
     warm_start: str | None = None        # weight file of the learned initial guess
-    warm_start_history: int | None = None  # previous-step differences in the guess (1..8, default 2)
+    warm_start_history: int | None = None  # previous-step differences in the guess (1..8, default 6)
+    warm_start_norm: str | None = None     # norm the guess minimises: "residual" (default) or "energy"
     states: list = field(default_factory=standard_states)
     extra: list = field(default_factory=list)
 
@@ -187,6 +188,8 @@ class Deck:
             lines.append(f"tl_warm_start {self.warm_start}")
         if self.warm_start_history is not None:
             lines.append(f"tl_warm_start_history={self.warm_start_history}")
+        if self.warm_start_norm is not None:
+            lines.append(f"tl_warm_start_norm={self.warm_start_norm}")
         lines += list(self.extra)
         lines.append("*endtea")
         return "\n".join(lines) + "\n"

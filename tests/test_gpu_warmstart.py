@@ -126,25 +126,29 @@ def _operator(eng, h, nx, ny):
     return A
 
 
-@pytest.mark.parametrize("with_net,history", [(True, None), (False, None), (True, 4), (False, 3), (False, 1), (False, 6), (True, 8)])
-def test_engine_apply_matches_oracle(with_net, history, tmp_path):
+@pytest.mark.parametrize("with_net,history,norm", [(True, None, None), (False, None, None), (True, 4, "energy"),
+                                                  (False, 3, None), (False, 1, "energy"), (False, 2, "residual"),
+                                                  (True, 8, None), (False, 6, "energy")])
+def test_engine_apply_matches_oracle(with_net, history, norm, tmp_path):
     """tea_engine_warm_start_apply on engine fields: the guess u0 + sum c_a v_a
-    over the differences of the last `history` steps (default 2) and the
-    network's direction, with the energy-minimising coefficients, computed
-    here from the oracle's network output and a dense restatement of the
-    operator."""
+    over the differences of the last `history` steps (default 6) and the
+    network's direction, with the coefficients that minimise the residual's
+    2-norm (default) or the error's energy norm, computed here from the
+    oracle's network output and a dense restatement of the operator; the
+    guess's residual (energy) is never above that of u = u0."""
     import ctypes
 
-    print(f"[gpu warmstart] engine apply (network {with_net}, history {history})", flush=True)
+    print(f"[gpu warmstart] engine apply (network {with_net}, history {history}, norm {norm})", flush=True)
     from test_oracle_math import setup_chunk
 
     lib = wsm.library()
     eng = Engine()
-    hist = history or 2
+    hist = history or 6
     try:
         nx, ny = 64, 8
         setup_chunk(eng, tmp_path, nx=nx, ny=ny, solver="cg", halo=1,
-                    extra=[f"tl_warm_start_history={history}"] if history else ())
+                    extra=([f"tl_warm_start_history={history}"] if history else []) +
+                    ([f"tl_warm_start_norm={norm}"] if norm else []))
         p = _params(7, w6_scale=0.2)
         if with_net:
             assert lib.tea_engine_warm_start_load(p.ctypes.data_as(wsm._F), p.size) == 0
@@ -182,12 +186,19 @@ def test_engine_apply_matches_oracle(with_net, history, tmp_path):
                 dirs.append(sc * net[0])
             AV = [A(v) for v in dirs]
             r0 = u0 - A(u0)
-            G = np.array([[np.sum(a * b) for b in AV] for a in dirs])
-            c = np.linalg.solve(G, np.array([np.sum(v * r0) for v in dirs]))
+            left = dirs if norm == "energy" else AV
+            G = np.array([[np.sum(a * b) for b in AV] for a in left])
+            c = np.linalg.solve(G, np.array([np.sum(v * r0) for v in left]))
             expect = u0 + sum(ci * v for ci, v in zip(c, dirs))
             u = eng.get_field(rt.FIELD_U)[h:h + ny, h:h + nx]
             tol = (2e-2 if with_net else 1e-10) * np.abs(expect - u0).max()
             assert np.abs(u - expect).max() <= tol, (k, np.abs(u - expect).max(), tol)
+            r = u0 - A(u)
+            if norm == "energy":   # ||e||_A^2 = r . A^-1 r; compare through e0 - e = u - u0
+                d = u - u0
+                assert np.sum(d * A(d)) - 2 * np.sum(d * r0) <= 1e-9 * np.sum(r0 * r0)
+            else:
+                assert np.sum(r * r) <= np.sum(r0 * r0) * (1 + 1e-12)
     finally:
         eng.finalize()
 

@@ -183,7 +183,12 @@ def run_gpu(n, warmup, steps, tmp, variant):
             eng.profile_reset()
             eng.profile_only("")
             eng.profile_enable(True)
+            ev["pstart"] = torch.cuda.Event(enable_timing=True)
+            ev["pstart"].record(stream)
         elif step == warmup + steps + PROFILED_STEPS:
+            ev["pstop"] = torch.cuda.Event(enable_timing=True)
+            ev["pstop"].record(stream)
+            ev["pstop"].synchronize()
             eng.profile_enable(False)
             ev["breakdown"] = eng.profile()
 
@@ -193,7 +198,9 @@ def run_gpu(n, warmup, steps, tmp, variant):
     timed = slice(warmup, warmup + steps)
     print(f"[bench] {variant}: {gpu_ms / steps:.1f} ms/step, iterations {res.iterations[timed]}, "
           f"true residual {res.exact_error[timed]}", file=sys.stderr, flush=True)
-    return res, gpu_ms, wall_ms, ev["profile"], timed, ev["breakdown"]
+    # the profiled steps' own time (every launch timed: a little slower than untimed steps)
+    profiled_ms = ev["pstart"].elapsed_time(ev["pstop"]) / PROFILED_STEPS
+    return res, gpu_ms, wall_ms, ev["profile"], timed, (ev["breakdown"], profiled_ms)
 
 
 def measured_traffic(kernel):
@@ -245,11 +252,12 @@ def roofline(profile, n):
     return out, roof
 
 
-def step_roofline(breakdown, steps, ms_per_step, n):
-    """Compulsory HBM bytes of every launch of a timestep (launch counts from
-    the profiled steps) over the measured time per timestep: the step's
-    achieved fraction of the HBM peak (a lower bound: kernels without an
-    entry in COMPULSORY_BYTES count zero bytes)."""
+def step_roofline(profiled, steps, n):
+    """Compulsory HBM bytes of every launch of the profiled timesteps over
+    those steps' own GPU time: the step's achieved fraction of the HBM peak
+    (a lower bound: kernels without an entry in COMPULSORY_BYTES count zero
+    bytes, and timing every launch slows the steps a little)."""
+    breakdown, ms_per_step = profiled
     total = 0.0
     known_ms = 0.0
     all_ms = 0.0
@@ -260,7 +268,8 @@ def step_roofline(breakdown, steps, ms_per_step, n):
             known_ms += ms
     per_step = total / steps
     gbs = per_step / (ms_per_step * 1e-3) / 1e9
-    return {"bytes_per_step": int(per_step), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    return {"bytes_per_step": int(per_step), "ms_per_step": round(ms_per_step, 3), "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
             "kernel_time_covered": round(known_ms / all_ms, 4) if all_ms else None}
 
@@ -451,15 +460,15 @@ def main():
         "comparison": comparison,
         "roofline": roof,
         "roofline_fp64_reference": plain_roof,
-        "step_roofline": step_roofline(breakdown, PROFILED_STEPS, ms_per_step, n),
-        "step_roofline_fp64_reference": (step_roofline(runs["fp64_reference"][5], PROFILED_STEPS,
-                                                       runs["fp64_reference"][1] / args.steps, n)
+        "step_roofline": step_roofline(breakdown, PROFILED_STEPS, n),
+        "step_roofline_fp64_reference": (step_roofline(runs["fp64_reference"][5], PROFILED_STEPS, n)
                                          if "fp64_reference" in runs else None),
         "dominant_kernel": dominant,
         "cpu_baseline": cpu,
         "kernels": {"steps": PROFILED_STEPS,
                     "note": f"every launch timed over {PROFILED_STEPS} steps after the timed region",
-                    **{k: {"launches": c, "total_ms": round(ms, 3)} for k, (c, ms) in sorted(breakdown.items())}},
+                    "ms_per_step": round(breakdown[1], 3),
+                    **{k: {"launches": c, "total_ms": round(ms, 3)} for k, (c, ms) in sorted(breakdown[0].items())}},
     }
     print(json.dumps(line), flush=True)
 

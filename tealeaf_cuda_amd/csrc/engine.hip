@@ -728,6 +728,7 @@ __global__ __launch_bounds__(BLOCK) void k_ws_features(Ext e, long pitch, int h,
 struct WsDirs {
     const double *u0;
     const double *up[WS_HMAX];
+    const double *la, *lb;   // the two fields of the last history direction (U_{nd-1}, U_nd)
     const float *net;
     double scale;
     int h, nx, ny;
@@ -765,17 +766,23 @@ __device__ __forceinline__ void block_reduce_lanes(const double (&v)[NV], const 
     }
 }
 
-// Gram matrix and right-hand side of the combination of HN history
-// directions and the network's that minimises the energy norm of the error
-// (MINRES false: G[a][b] = v_a . A v_b (a <= b, row by row) then g[a] = v_a .
-// r0) or the 2-norm of the residual (MINRES true: G[a][b] = A v_a . A v_b,
-// g[a] = A v_a . r0); r0 = u0 - A u0 is the residual of the u = u^n guess, and
-// directions not present count 0.
+// Gram matrix and right-hand side of the combination of the history
+// directions v_a = U_a - U_{a+1} (a < nd = d.nh; U_0 = u0, U_{a+1} = up[a],
+// the earlier steps' u0, newest first) and the network's q that minimises
+// the 2-norm of the residual (MINRES true: G[a][b] = A v_a . A v_b (a <= b,
+// row by row) then g[a] = A v_a . r0) or the energy norm of the error (MINRES
+// false: G[a][b] = v_a . A v_b, g[a] = v_a . r0); directions not present
+// count 0.  Each earlier step's solve left A U_a = U_{a+1} to its tolerance
+// (U_a was that step's solution and U_{a+1} its right-hand side), so
+// A v_a = v_{a+1} for a + 1 < nd and the residual of the u = u0 guess is
+// r0 = u0 - A u0 = v_0, both to that tolerance: only A v_{nd-1} (on d.la -
+// d.lb) and A q take the stencil.
 template <int HN, bool MINRES>
 __global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
                                                    const double *__restrict__ Ky, Red red)
 {
     constexpr int D = HN + 1, NG = D * (D + 1) / 2;
+    const int nd = d.nh;
     double acc[NG + D];
 #pragma unroll
     for (int j = 0; j < NG + D; j++) acc[j] = 0.0;
@@ -783,31 +790,34 @@ __global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, 
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         const int c = s.c, r = s.r0 + k;
-        const double a_u0 = apply_A(d.u0, Kx, Ky, i, pitch);
         double v[D], av[D];
-        double up_prev = d.u0[i], a_prev = a_u0;
+        double prev = d.u0[i];
+        v[HN] = 0.0;
+        av[HN] = 0.0;
 #pragma unroll
         for (int j = 0; j < HN; j++) {
             v[j] = 0.0;
-            av[j] = 0.0;
-            if (j < d.nh) {
-                const double a_j = apply_A(d.up[j], Kx, Ky, i, pitch);
-                v[j] = up_prev - d.up[j][i];
-                av[j] = a_prev - a_j;
-                up_prev = d.up[j][i];
-                a_prev = a_j;
+            if (j < nd) {
+                const double uj = d.up[j][i];
+                v[j] = prev - uj;
+                prev = uj;
             }
         }
-        v[HN] = 0.0;
-        av[HN] = 0.0;
+        // A v_{nd-1}: the stencil on the difference of its two fields
+        const double ky1 = Ky[i + pitch], ky0 = Ky[i], kx1 = Kx[i + 1], kx0 = Kx[i];
+        const double last = (1.0 + (ky1 + ky0) + (kx1 + kx0)) * (d.la[i] - d.lb[i]) -
+                            (ky1 * (d.la[i + pitch] - d.lb[i + pitch]) + ky0 * (d.la[i - pitch] - d.lb[i - pitch])) -
+                            (kx1 * (d.la[i + 1] - d.lb[i + 1]) + kx0 * (d.la[i - 1] - d.lb[i - 1]));
+#pragma unroll
+        for (int j = 0; j < HN; j++) av[j] = j + 1 < nd ? v[j + 1] : j + 1 == nd ? last : 0.0;
         if (d.net) {
             const double q = ws_net_at(d, c, r);
             v[HN] = q;
-            av[HN] = (1.0 + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * q -
-                     (Ky[i + pitch] * ws_net_at(d, c, r + 1) + Ky[i] * ws_net_at(d, c, r - 1)) -
-                     (Kx[i + 1] * ws_net_at(d, c + 1, r) + Kx[i] * ws_net_at(d, c - 1, r));
+            av[HN] = (1.0 + (ky1 + ky0) + (kx1 + kx0)) * q -
+                     (ky1 * ws_net_at(d, c, r + 1) + ky0 * ws_net_at(d, c, r - 1)) -
+                     (kx1 * ws_net_at(d, c + 1, r) + kx0 * ws_net_at(d, c - 1, r));
         }
-        const double r0 = d.u0[i] - a_u0;
+        const double r0 = v[0];
         int q = 0;
 #pragma unroll
         for (int x = 0; x < D; x++)
@@ -2664,13 +2674,16 @@ void ws_solve(int k, double G[WS_DMAX][WS_DMAX], const double *b, double *c)
 }  // namespace
 
 namespace {
+// At most WS_GRAM_GRID workgroups, each over many tiles: the workgroup's
+// reduction of up to MAX_RED values then costs little beside its cells.
+constexpr int WS_GRAM_GRID = 1024;
 template <int HN> bool ws_gram_launch(const WsDirs &d)
 {
     if (g.ws_minres)
-        return launch("ws_gram", k_ws_gram<HN, true>, interior(), g.pitch, d, (const double *)F(X_KX),
-                      (const double *)F(X_KY), red_ctx());
-    return launch("ws_gram", k_ws_gram<HN, false>, interior(), g.pitch, d, (const double *)F(X_KX),
-                  (const double *)F(X_KY), red_ctx());
+        return launch_capped("ws_gram", WS_GRAM_GRID, k_ws_gram<HN, true>, interior(), g.pitch, d,
+                             (const double *)F(X_KX), (const double *)F(X_KY), red_ctx());
+    return launch_capped("ws_gram", WS_GRAM_GRID, k_ws_gram<HN, false>, interior(), g.pitch, d,
+                         (const double *)F(X_KX), (const double *)F(X_KY), red_ctx());
 }
 template <int HN> bool ws_apply_launch(const WsDirs &d, double *keep, const WsCoefs &cf, int apply)
 {
@@ -2694,6 +2707,8 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
     d.nx = g.nx;
     d.ny = g.ny;
     d.nh = g.ws_nprev < HN ? g.ws_nprev : HN;
+    d.la = d.nh > 1 ? d.up[d.nh - 2] : d.u0;
+    d.lb = d.nh > 0 ? d.up[d.nh - 1] : d.u0;
     WsCoefs cf{};
     int apply = 0;
     if (g.ws_nprev >= 1) {

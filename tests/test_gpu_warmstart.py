@@ -130,12 +130,14 @@ def _operator(eng, h, nx, ny):
                                                   (False, 3, None), (False, 1, "energy"), (False, 2, "residual"),
                                                   (True, 8, None), (False, 6, "energy")])
 def test_engine_apply_matches_oracle(with_net, history, norm, tmp_path):
-    """tea_engine_warm_start_apply on engine fields: the guess u0 + sum c_a v_a
-    over the differences of the last `history` steps (default 6) and the
-    network's direction, with the coefficients that minimise the residual's
-    2-norm (default) or the error's energy norm, computed here from the
-    oracle's network output and a dense restatement of the operator; the
-    guess's residual (energy) is never above that of u = u0."""
+    """tea_engine_warm_start_apply on engine fields over a history of solves
+    (each u0 the exact solution of A u = the previous u0, as the timesteps
+    leave them): the guess u0 + sum c_a v_a over the differences of the last
+    `history` steps (default 6) and the network's direction, with the
+    coefficients that minimise the residual's 2-norm (default) or the error's
+    energy norm, computed here by least squares from the oracle's network
+    output and a dense restatement of the operator; the guess's residual
+    (energy) is never above that of u = u0."""
     import ctypes
 
     print(f"[gpu warmstart] engine apply (network {with_net}, history {history}, norm {norm})", flush=True)
@@ -158,6 +160,8 @@ def test_engine_apply_matches_oracle(with_net, history, norm, tmp_path):
         h = eng.h
         shape = eng.shape()
         A = _operator(eng, h, nx, ny)
+        eye = np.eye(nx * ny)
+        Am = np.stack([A(eye[j].reshape(ny, nx)).ravel() for j in range(nx * ny)], axis=1)
 
         def put(fid, vals):
             a = np.zeros(shape)
@@ -167,8 +171,8 @@ def test_engine_apply_matches_oracle(with_net, history, norm, tmp_path):
         rho = 10.0 ** rng.uniform(-1, 2, (ny, nx))
         put(rt.FIELD_DENSITY, rho)
         us = [rng.uniform(1.0, 3.0, (ny, nx))]
-        for _ in range(hist + 1):
-            us.append(us[-1] + rng.standard_normal((ny, nx)) * 0.1)
+        for _ in range(hist + 2):
+            us.append(np.linalg.solve(Am, us[-1].ravel()).reshape(ny, nx))
         applied = ctypes.c_int(-1)
         for k, u0 in enumerate(us):
             put(rt.XFIELD_U0, u0)
@@ -184,15 +188,18 @@ def test_engine_apply_matches_oracle(with_net, history, norm, tmp_path):
                 planes = np.stack([(np.log10(rho) - 0.5) / 1.5, d / sc, u0 / su]).astype(np.float32)[None]
                 net, _ = co.forward(p, planes, mode="bf16")
                 dirs.append(sc * net[0])
-            AV = [A(v) for v in dirs]
-            r0 = u0 - A(u0)
-            left = dirs if norm == "energy" else AV
-            G = np.array([[np.sum(a * b) for b in AV] for a in left])
-            c = np.linalg.solve(G, np.array([np.sum(v * r0) for v in left]))
-            expect = u0 + sum(ci * v for ci, v in zip(c, dirs))
+            V = np.stack([v.ravel() for v in dirs], axis=1)
+            AV = Am @ V
+            r0 = (u0 - A(u0)).ravel()
+            if norm == "energy":
+                c = np.linalg.lstsq(V.T @ AV, V.T @ r0, rcond=1e-14)[0]
+            else:
+                c = np.linalg.lstsq(AV, r0, rcond=1e-14)[0]
+            expect = u0 + (V @ c).reshape(ny, nx)
             u = eng.get_field(rt.FIELD_U)[h:h + ny, h:h + nx]
-            tol = (2e-2 if with_net else 1e-10) * np.abs(expect - u0).max()
+            tol = (2e-2 if with_net else 1e-6) * np.abs(expect - u0).max()
             assert np.abs(u - expect).max() <= tol, (k, np.abs(u - expect).max(), tol)
+            r0 = r0.reshape(ny, nx)
             r = u0 - A(u)
             if norm == "energy":   # ||e||_A^2 = r . A^-1 r; compare through e0 - e = u - u0
                 d = u - u0

@@ -198,9 +198,12 @@ int tea_engine_set_field(int field_id, const double *host);
  * back count and total milliseconds for one kernel name.  Returns 0 if the
  * name was seen.  tea_engine_profile_names writes newline-separated names.
  * tea_engine_profile_only(prefix) times only launches whose name starts with
- * prefix (NULL or "": all); a timed launch costs the run a few microseconds. */
+ * prefix (NULL or "": all); a timed launch costs the run a few microseconds
+ * of idle GPU around it, so tea_engine_profile_sample(every) times one in
+ * `every` of each name's launches (counts are of the timed ones). */
 void tea_engine_profile_enable(int on);
 void tea_engine_profile_only(const char *prefix);
+void tea_engine_profile_sample(int every);
 void tea_engine_profile_reset(void);
 int tea_engine_profile_get(const char *kernel, long *count, double *total_ms);
 int tea_engine_profile_names(char *buf, int buflen);
@@ -245,6 +248,13 @@ void tea_engine_ppcg_end_begin_ur(double rro_prev, double theta, double *rrn_end
  * and runs them again. */
 void tea_engine_ppcg_end_begin_ur_inner(double rro_prev, double theta, int inner_steps, double *rrn_end,
                                         double *pw_rro, double *rrn);
+/* One whole outer iteration, as the reference's own PPCG branch runs it
+ * (tea_solve.f90:324-340 without halo exchanges): tea_engine_ppcg_begin_ur
+ * (pw_rro, rrn_ur), inner steps 1..inner_steps as tea_engine_ppcg_inner_plan
+ * splits them, then tea_engine_ppcg_end (rrn_end), queued back to back with
+ * one host wait, at the end, instead of two. */
+void tea_engine_ppcg_iteration(int with_p, double beta, double theta, int inner_steps, double *pw_rro,
+                               double *rrn_ur, double *rrn_end);
 /* ---- fused CG iteration ------------------------------------------------------ *
  * No counterpart in the reference: [tea_leaf_cg_calc_p_kernel_cuda_(beta) if
  * with_p], tea_leaf_cg_calc_w_kernel_cuda_ (pw), then

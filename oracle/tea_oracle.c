@@ -813,6 +813,7 @@ EXPORT int tea_engine_set_field(int id, const double *host)
 }
 EXPORT void tea_engine_profile_enable(int on) { (void)on; }
 EXPORT void tea_engine_profile_only(const char *prefix) { (void)prefix; }
+EXPORT void tea_engine_profile_sample(int every) { (void)every; }
 EXPORT void tea_engine_profile_reset(void) {}
 EXPORT int tea_engine_profile_get(const char *k, long *count, double *ms) { (void)k; *count = 0; *ms = 0; return -1; }
 EXPORT int tea_engine_profile_names(char *buf, int n) { if (buf && n > 0) buf[0] = 0; return 0; }
@@ -873,6 +874,15 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
 {
     tea_leaf_ppcg_update_z_kernel_cuda_();
     tea_leaf_ppcg_calc_rrn_kernel_cuda_(rrn);
+}
+
+/* The engine's whole outer iteration, as the composition it names. */
+EXPORT void tea_engine_ppcg_iteration(int with_p, double beta, double theta, int inner_steps, double *pw_rro,
+                                      double *rrn_ur, double *rrn_end)
+{
+    tea_engine_ppcg_begin_ur(with_p, beta, theta, pw_rro, rrn_ur);
+    if (inner_steps > 0) tea_engine_ppcg_inner_n(1, inner_steps);
+    tea_engine_ppcg_end(rrn_end);
 }
 
 EXPORT void tea_engine_ppcg_inner_n(int step, int nsteps)

@@ -62,6 +62,7 @@ constexpr int S_CG_FAMILY = (1 << SOLVER_CG) | (1 << SOLVER_CHEBY) | (1 << SOLVE
 struct Profiler {
     bool on = false;
     std::string only;   // name prefix of the launches to time (empty: all)
+    int every = 1;      // time one in `every` of each name's launches
     struct Pending {
         int name;
         hipEvent_t a, b;
@@ -69,6 +70,7 @@ struct Profiler {
     std::vector<std::string> names;
     std::map<std::string, int> ids;
     std::vector<long> counts;
+    std::vector<long> seen;   // launches of each name while wanted
     std::vector<double> ms;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -81,6 +83,7 @@ struct Profiler {
         names.push_back(n);
         ids[n] = k;
         counts.push_back(0);
+        seen.push_back(0);
         ms.push_back(0.0);
         return k;
     }
@@ -360,6 +363,10 @@ void klaunch(const char *name, K kernel, dim3 grid, dim3 block, A... args)
         return;
     }
     const int id = g.prof.id(name);
+    if (g.prof.seen[id]++ % g.prof.every != 0) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, g.stream, args...);
+        return;
+    }
     hipEvent_t a = g.prof.get(), b = g.prof.get();
     hipExtLaunchKernelGGL(kernel, grid, block, 0, g.stream, a, b, 0u, args...);
     g.prof.pending.push_back({id, a, b});
@@ -1988,6 +1995,35 @@ template <typename T> void ppcg_end_impl(double *rrn)
     fetch(rrn, 1);
 }
 
+// begin_ur, the inner steps in passes and end queued back to back with one
+// host wait at the end: for callers that need only the end's r.z to go on.
+template <typename T>
+void ppcg_iteration_impl(int with_p, double beta, double theta, int inner_steps, double *pw_rro, double *rrn_ur,
+                         double *rrn_end)
+{
+    const bool wp = w_from_p<T>();
+    const T *z_src = g.pending_z == 8 * (int)sizeof(T) ? (const T *)FT<T>(X_UTEMP) : nullptr;
+    if (!ppcg_begin_launch<T>(with_p, beta, nullptr, z_src, wp) || !finish(0, 2, g.d_scal)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
+    g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
+    g.pending_z = 0;
+    for (int step = 1; step <= inner_steps;) {
+        const int n = plan_first_pass(inner_steps - step + 1, pass_costs());
+        ppcg_inner_n_impl<T>(step - 1, n);
+        step += n;
+    }
+    settle_rt_ut();
+    const bool owe_z = g.prec != PREC_NONE;
+    if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP),
+                       owe_z ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()) ||
+        !finish(3, 1, nullptr))
+        return;
+    g.pending_z = owe_z ? 8 * (int)sizeof(T) : 0;
+    if (!collect(pw_rro, 0, 2)) return;
+    *rrn_ur = g.h_out[2];
+    *rrn_end = g.h_out[3];
+}
+
 }  // namespace
 
 #define BY_PRECISION(fn, ...) (g.fp32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
@@ -2357,6 +2393,19 @@ EXPORT void tea_engine_ppcg_end(double *rrn)
     BY_PRECISION(ppcg_end_impl, rrn);
 }
 
+EXPORT void tea_engine_ppcg_iteration(int with_p, double beta, double theta, int inner_steps, double *pw_rro,
+                                      double *rrn_ur, double *rrn_end)
+{
+    CHECK_INIT_KEEP_PENDING();
+    drop_w_settle_rest();
+    REQUIRE_SOLVER(1 << SOLVER_PPCG);
+    if (inner_steps < 0 || inner_steps > (int)g.ch_a.size()) {
+        die("ppcg_iteration: %d inner steps, %d coefficients uploaded", inner_steps, (int)g.ch_a.size());
+        return;
+    }
+    BY_PRECISION(ppcg_iteration_impl, with_p, beta, theta, inner_steps, pw_rro, rrn_ur, rrn_end);
+}
+
 // Inner steps `step` .. `step + nsteps - 1` (1-based, nsteps 1..5) over the
 // interior in one pass: tea_leaf_ppcg_inner_kernel_cuda_ once per step with
 // bounds_extra 0 and every face external (one chunk).
@@ -2442,11 +2491,14 @@ EXPORT void tea_engine_profile_enable(int on)
 // Times only the launches whose name starts with `prefix` (NULL or "": all);
 // each timed launch costs the run a few microseconds.
 EXPORT void tea_engine_profile_only(const char *prefix) { g.prof.only = prefix ? prefix : ""; }
+// Times one in `every` (>= 1) of each name's wanted launches, from the first.
+EXPORT void tea_engine_profile_sample(int every) { g.prof.every = every > 1 ? every : 1; }
 EXPORT void tea_engine_profile_reset(void)
 {
     if (g.stream) (void)hipStreamSynchronize(g.stream);
     g.prof.drain(true);
     for (auto &c : g.prof.counts) c = 0;
+    for (auto &c : g.prof.seen) c = 0;
     for (auto &m : g.prof.ms) m = 0.0;
 }
 EXPORT int tea_engine_profile_get(const char *kernel, long *count, double *total_ms)

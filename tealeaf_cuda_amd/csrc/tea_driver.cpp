@@ -96,6 +96,7 @@ struct Backend {
     void (*ppcg_begin_ur)(int, double, double, double *, double *) = nullptr;
     void (*ppcg_end_begin_ur)(double, double, double *, double *, double *) = nullptr;
     void (*ppcg_end_begin_ur_inner)(double, double, int, double *, double *, double *) = nullptr;
+    void (*ppcg_iteration)(int, double, double, int, double *, double *, double *) = nullptr;
     int (*ppcg_inner_plan)(int) = nullptr;
     void (*cg_begin_ur)(int, double, double, double *, double *) = nullptr;
 
@@ -142,6 +143,8 @@ Backend load_backend(const char *path)
         dlsym(b.handle, "tea_engine_ppcg_end_begin_ur"));
     b.ppcg_end_begin_ur_inner = reinterpret_cast<void (*)(double, double, int, double *, double *, double *)>(
         dlsym(b.handle, "tea_engine_ppcg_end_begin_ur_inner"));
+    b.ppcg_iteration = reinterpret_cast<void (*)(int, double, double, int, double *, double *, double *)>(
+        dlsym(b.handle, "tea_engine_ppcg_iteration"));
     b.cg_begin_ur =
         reinterpret_cast<void (*)(int, double, double, double *, double *)>(dlsym(b.handle, "tea_engine_cg_begin_ur"));
     return b;
@@ -918,6 +921,12 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                                  d.tl_ppcg_inner_steps > 0 && !(passes_env && passes_env[0] == '0') &&
                                  !(spec_env && spec_env[0] == '0');
     const bool queue_always = spec_env && spec_env[0] == '2';   // every time it may (tests the undo)
+    // end not deferred and no residual replacement: the whole outer iteration
+    // (begin_ur, the inner passes, end) in one call with one host wait
+    const char *iter_env = std::getenv("TEA_PPCG_ITERATION");
+    const bool whole_iteration = fused && !defer_end && !rel.on && b.ppcg_iteration && b.ppcg_inner_n &&
+                                 b.ppcg_inner_plan && d.tl_ppcg_inner_steps >= 0 &&
+                                 !(passes_env && passes_env[0] == '0') && !(iter_env && iter_env[0] == '0');
     int fields[NUM_FIELDS];
     std::fill(fields, fields + NUM_FIELDS, 0);
 
@@ -1038,6 +1047,20 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                         b.check();
                     }
                 }
+            } else if (d.tl_use_ppcg && whole_iteration) {
+                // the calls of the branch below in one: begin_ur, the inner
+                // steps, end (rrn)
+                double two[2], rrn_ur = 0.0;
+                b.ppcg_iteration(p_pending ? 1 : 0, beta_pending, theta, d.tl_ppcg_inner_steps, two, &rrn_ur, &rrn);
+                b.check();
+                pw = two[0];
+                rro = two[1];
+                ppcg_inner_iters += d.tl_ppcg_inner_steps;
+                beta = rrn / rro;
+                p_pending = true;
+                beta_pending = beta;
+                error = rrn;
+                rro = rrn;
             } else if (d.tl_use_ppcg && fused) {
                 // the same calls as below, fused; ppcg_calc_p of the previous
                 // iteration is folded into this iteration's first pass

@@ -98,6 +98,7 @@ _EXT = {
     "tea_engine_set_field": ([ctypes.c_int, _D], ctypes.c_int),
     "tea_engine_profile_enable": ([ctypes.c_int], None),
     "tea_engine_profile_only": ([ctypes.c_char_p], None),
+    "tea_engine_profile_sample": ([ctypes.c_int], None),
     "tea_engine_profile_reset": ([], None),
     "tea_engine_profile_get": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), _D], ctypes.c_int),
     "tea_engine_profile_names": ([ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
@@ -116,6 +117,9 @@ _EXT = {
     "tea_engine_ppcg_end_begin_ur_inner": ([ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)], None),
+    "tea_engine_ppcg_iteration": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_ppcg_inner_plan": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_cg_begin_ur": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)], None),
@@ -338,6 +342,9 @@ class Engine:
     def profile_only(self, prefix: str = ""):
         self.lib.tea_engine_profile_only(prefix.encode())
 
+    def profile_sample(self, every: int = 1):
+        self.lib.tea_engine_profile_sample(int(every))
+
     def profile_reset(self):
         self.lib.tea_engine_profile_reset()
 
@@ -374,6 +381,13 @@ class Engine:
         self._call("tea_engine_ppcg_end_begin_ur_inner", float(rro_prev), float(theta), int(inner_steps),
                    ctypes.byref(rrn_end), two, ctypes.byref(rrn))
         return rrn_end.value, two[0], two[1], rrn.value
+
+    def ppcg_iteration(self, with_p: bool, beta: float, theta: float, inner_steps: int):
+        """Returns (pw, rro, rrn_ur, rrn_end)."""
+        two, rrn_ur, rrn_end = (ctypes.c_double * 2)(), ctypes.c_double(), ctypes.c_double()
+        self._call("tea_engine_ppcg_iteration", 1 if with_p else 0, float(beta), float(theta), int(inner_steps), two,
+                   ctypes.byref(rrn_ur), ctypes.byref(rrn_end))
+        return two[0], two[1], rrn_ur.value, rrn_end.value
 
     def ppcg_ur(self, alpha: float, theta: float) -> float:
         out = ctypes.c_double()

@@ -107,7 +107,8 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 
 
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
-                                    "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY"])
+                                    "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY",
+                                    "TEA_PPCG_ITERATION"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
                                   "cg_block_ragged", "cheby_none", "mixed_cheby_diag"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
@@ -118,9 +119,11 @@ def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracl
     (TEA_SPECULATE=0), the FP32 fused pair with w written by begin and read
     by ur (TEA_W_FROM_P=0), CG iterations through the separate calls
     instead of the fused one (TEA_FUSED_CG=0), the fused begins on the tile
-    kernels (TEA_BEGIN_MARCH=0), and Chebyshev iterations on the separate
-    calc_p / calc_u kernels (TEA_FUSED_CHEBY=0), on the GPU give the same
-    results as the oracle."""
+    kernels (TEA_BEGIN_MARCH=0), Chebyshev iterations on the separate
+    calc_p / calc_u kernels (TEA_FUSED_CHEBY=0), and PPCG outer iterations
+    through begin_ur, the passes and end as separate calls instead of one
+    (TEA_PPCG_ITERATION=0), on the GPU give the same results as the
+    oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -239,6 +242,11 @@ PPCG_CALLS = {
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_begin_ur(False, 0.0, 1.7), e.ppcg_inner_n(1, 4), e.ppcg_end_begin_ur(0.8, 1.7)),
     "fused_ppcg_inner2": lambda e: (e.ppcg_init_constants([0.3, 0.5, 0.7], [1.1, 0.9, 0.8]), e.ppcg_inner2(2)),
+    # a whole outer iteration in one call, twice (the second with p owed), and with no inner steps
+    "fused_ppcg_iteration_twice": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_iteration(False, 0.0, 1.7, 6), e.ppcg_iteration(True, 0.45, 1.7, 6)),
+    "fused_ppcg_end_then_iteration0": lambda e: (e.ppcg_end(), e.ppcg_iteration(True, 0.45, 1.7, 0)),
     **{f"fused_ppcg_inner_n{n}": (lambda n: lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_inner_n(7 - n, n)))(n) for n in range(1, 6)},

@@ -212,8 +212,10 @@ def generate(seeds, n: int, steps: int, crops_per_step: int, store: SampleStore,
     """Runs the solver over `seeds` and fills `store` with crops.  The target
     is the part of each step that a guess misses: residual_of "extrapolation"
     (u^n + (u^n - u^{n-1})) or "galerkin" (the engine's energy-minimising
-    combination of the previous-step differences, which the solves then also
-    start from; the solution is the same to the solver tolerance)."""
+    combination of the last two previous-step differences, which the solves
+    then also start from -- tl_warm_start_history 2 and tl_warm_start_norm
+    energy, the guess the samples' two coefficients restate; the solution is
+    the same to the solver tolerance)."""
     lib = library()
     rng = rng or np.random.default_rng(0)
     st = GenStats()
@@ -221,7 +223,8 @@ def generate(seeds, n: int, steps: int, crops_per_step: int, store: SampleStore,
     bufs = {k: torch.empty((n, n), dtype=torch.float64, device=dev)
             for k in ("rho", "prev2", "prev", "u0", "u", "kx", "ky")}
     coef = (ctypes.c_double * 3)()
-    extra = {"warm_start": "extrapolate"} if residual_of == "galerkin" else {}
+    extra = ({"warm_start": "extrapolate", "warm_start_history": 2, "warm_start_norm": "energy"}
+             if residual_of == "galerkin" else {})
     origins = torch.empty(2 * crops_per_step, dtype=torch.int32, device=dev)
     scales = (ctypes.c_double * 2)()
     crop = store.crop

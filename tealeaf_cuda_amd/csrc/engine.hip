@@ -173,6 +173,13 @@ struct Chunk {
     // point but the fused Chebyshev iteration, which copies the ring across
     // when it is set)
     bool u_ring_dirty = true;
+    // entry points called so far; the end of an outer iteration (update_z,
+    // calc_rrn) computed by the last inner pass of a fused call is taken by
+    // the next call only when no entry point came in between (end_call: the
+    // call that queued it, end_slot: its h_out / d_scal slot, -1: none)
+    long calls = 0;
+    long end_call = -1;
+    int end_slot = -1;
     int max_blocks = 0;              // largest grid of any launch
     int last_grid = 0;               // grid of the latest launch
 
@@ -253,7 +260,7 @@ bool die(const char *fmt, ...)
     } while (0)
 void settle_pending();
 #define CHECK_INIT_KEEP_PENDING()                                                                 \
-    do { if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return; } } while (0)
+    do { if (!g.initialised) { die("%s called before initialise_cuda_", __func__); return; } g.calls++; } while (0)
 #define CHECK_INIT()                                                                              \
     do { CHECK_INIT_KEEP_PENDING(); g.u_ring_dirty = true; settle_pending(); } while (0)
 // Kernels are registered per solver in the reference (init_cuda.cu:251-282) and
@@ -303,6 +310,20 @@ int blocks_for(const Ext &e)
 }
 
 Red red_ctx() { return Red{g.d_partials, g.max_blocks}; }
+
+// The end of a fused FP64 outer iteration (update_z, calc_rrn) rides in its
+// last inner pass unless TEA_FUSED_END=0 (FP64 plain PPCG at 4096^2:
+// 99.0 -> 97.5 ms per timestep).  Not in FP32: that pass is bound by VALU
+// issue, and the compensated FP64 sum it would add cost as much as the
+// separate end (45 us) it saves.
+template <typename T> bool fused_end()
+{
+    static bool v = [] {
+        const char *s = getenv("TEA_FUSED_END");
+        return !(s && s[0] == '0');
+    }();
+    return v && sizeof(T) == 8;
+}
 
 // The PPCG inner step runs as one fused launch unless TEA_FUSED_INNER=0.
 bool fused_inner()
@@ -1639,11 +1660,16 @@ int cu_count()
 // FP32 (its kernels fit 256 registers up to five steps), one in FP64 (the
 // longer chunks that leaves measured faster than two waves per SIMD even
 // where the registers would allow two), chunks as long as that leaves.
-template <typename T, int NS> void launch_inner_march(const Ext &e, long pt, const StepCoefs &cf, const char *name)
+// With r_end (and rs_end) given and a ring form with a preconditioner
+// running, the pass also forms k_ppcg_end's (r - rstore).z over the utemp it
+// writes (z owed, not stored) into the reduction partials, and returns true.
+template <typename T, int NS>
+bool launch_inner_march(const Ext &e, long pt, const StepCoefs &cf, const char *name, const T *r_end = nullptr,
+                        const T *rs_end = nullptr)
 {
     constexpr int OX = march_out_x(NS);
     const int nbands = (e.x1 - e.x0 + OX) / OX, nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
-    if (nbands < 1 || nstrips < 1) return;
+    if (nbands < 1 || nstrips < 1) return false;
     const int per_simd = sizeof(T) == 4 ? 2 : 1;
     int nchunks = cu_count() * 4 * per_simd / nbands;
     nchunks = nchunks < 1 ? 1 : (nchunks > nstrips ? nstrips : nchunks);
@@ -1665,18 +1691,39 @@ template <typename T, int NS> void launch_inner_march(const Ext &e, long pt, con
                 FT<T>(X_SD2), rt_in, FT<T>(X_RT2), ut_in, FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
                 (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
     };
+    auto go_ring = [&](auto kernel, const T *re, const T *rse) {
+        g.last_grid = (int)grid;
+        klaunch(name, kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk, (const T *)FT<T>(F_SD),
+                FT<T>(X_SD2), rt_in, FT<T>(X_RT2), ut_in, FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
+                (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf, re, rse, red_ctx());
+    };
     const char *ring_env = getenv("TEA_INNER_RING");
     const int forced = ring_env && ring_env[0] >= '0' && ring_env[0] <= '2' ? ring_env[0] - '0' : -1;
+    const bool want_end = r_end && rs_end && (int)grid <= g.max_blocks;
+    bool end_done = false;
     auto pick = [&](auto prec_c) {
         constexpr int P = decltype(prec_c)::value;
         const int form = forced >= 0 ? forced : inner_form<T, P, NS>();
-        if (form == 2) go(k_ppcg_inner_ringl<T, P, NS>);
-        else if (form == 1) go(k_ppcg_inner_ring<T, P, NS>);
+        if constexpr (P != PREC_NONE) {
+            if (want_end && form == 2) {
+                go_ring(k_ppcg_inner_ringl<T, P, NS, true>, r_end, rs_end);
+                end_done = true;
+                return;
+            }
+            if (want_end && form == 1) {
+                go_ring(k_ppcg_inner_ring<T, P, NS, true>, r_end, rs_end);
+                end_done = true;
+                return;
+            }
+        }
+        if (form == 2) go_ring(k_ppcg_inner_ringl<T, P, NS>, nullptr, nullptr);
+        else if (form == 1) go_ring(k_ppcg_inner_ring<T, P, NS>, nullptr, nullptr);
         else go(k_ppcg_inner_march<T, P, NS>);
     };
     if (g.prec == PREC_BLOCK) pick(std::integral_constant<int, PREC_BLOCK>{});
     else if (g.prec == PREC_DIAG) pick(std::integral_constant<int, PREC_DIAG>{});
     else pick(std::integral_constant<int, PREC_NONE>{});
+    return end_done;
 }
 
 int plan_first_pass(int remaining, const double *cost)
@@ -1697,16 +1744,14 @@ int plan_first_pass(int remaining, const double *cost)
     return first[remaining];
 }
 
-template <typename T, int NS> void launch_inner_n(const Ext &e, long pt, const StepCoefs &cf)
+template <typename T, int NS>
+bool launch_inner_n(const Ext &e, long pt, const StepCoefs &cf, const T *r_end, const T *rs_end)
 {
     constexpr bool X2 = std::is_same<T, float>::value;
     using Tile = InnerTile<X2, INNER_WAVES, NS>;
     static const char *names[MAX_PASS_STEPS] = {"k_ppcg_inner1", "k_ppcg_inner2", "k_ppcg_inner3", "k_ppcg_inner4",
                                                 "k_ppcg_inner5"};
-    if (use_march<T, NS>()) {
-        launch_inner_march<T, NS>(e, pt, cf, nm<T>(names[NS - 1]));
-        return;
-    }
+    if (use_march<T, NS>()) return launch_inner_march<T, NS>(e, pt, cf, nm<T>(names[NS - 1]), r_end, rs_end);
     settle_pending();
     const long ntiles = (long)((e.x1 - e.x0 + Tile::OUT_X) / Tile::OUT_X) * ((e.y1 - e.y0 + Tile::OUT_Y) / Tile::OUT_Y);
     const unsigned grid = (unsigned)std::min<long>(ntiles, grid_max());
@@ -1724,10 +1769,13 @@ template <typename T, int NS> void launch_inner_n(const Ext &e, long pt, const S
         else if (g.prec == PREC_DIAG) go(k_ppcg_inner_n<T, PREC_DIAG, INNER_WAVES, NS>);
         else go(k_ppcg_inner_n<T, PREC_NONE, INNER_WAVES, NS>);
     }
+    return false;
 }
 
-// Steps k0+1 .. k0+ns (0-based coefficient index k0) in one pass.
-template <typename T> void ppcg_inner_n_impl(int k0, int ns)
+// Steps k0+1 .. k0+ns (0-based coefficient index k0) in one pass.  with_end:
+// the pass may also form the end's (r - rstore).z (returns whether it did; its
+// partials are then the latest reduction's).
+template <typename T> bool ppcg_inner_n_impl(int k0, int ns, bool with_end = false)
 {
     const long pt = pitch_of<T>();
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
@@ -1753,15 +1801,17 @@ template <typename T> void ppcg_inner_n_impl(int k0, int ns)
         cf.b[k] = g.ch_b[k0 + k];
     }
     const Ext e = interior();
+    const T *re = with_end ? (const T *)FT<T>(F_R) : nullptr, *rse = with_end ? (const T *)FT<T>(X_RSTORE) : nullptr;
+    bool end_done = false;
     switch (ns) {
-    case 1: launch_inner_n<T, 1>(e, pt, cf); break;
-    case 2: launch_inner_n<T, 2>(e, pt, cf); break;
-    case 3: launch_inner_n<T, 3>(e, pt, cf); break;
-    case 4: launch_inner_n<T, 4>(e, pt, cf); break;
-    default: launch_inner_n<T, 5>(e, pt, cf); break;
+    case 1: end_done = launch_inner_n<T, 1>(e, pt, cf, re, rse); break;
+    case 2: end_done = launch_inner_n<T, 2>(e, pt, cf, re, rse); break;
+    case 3: end_done = launch_inner_n<T, 3>(e, pt, cf, re, rse); break;
+    case 4: end_done = launch_inner_n<T, 4>(e, pt, cf, re, rse); break;
+    default: end_done = launch_inner_n<T, 5>(e, pt, cf, re, rse); break;
     }
     hipError_t err = hipGetLastError();
-    if (err != hipSuccess) { die("launch of k_ppcg_inner%d failed: %s", ns, hipGetErrorString(err)); return; }
+    if (err != hipSuccess) { die("launch of k_ppcg_inner%d failed: %s", ns, hipGetErrorString(err)); return false; }
     if (std::is_same<T, double>::value) {
         std::swap(g.f[F_SD], g.f[X_SD2]);
         std::swap(g.fs[F_SD], g.fs[X_SD2]);
@@ -1771,6 +1821,7 @@ template <typename T> void ppcg_inner_n_impl(int k0, int ns)
         std::swap(g.fs32[F_SD], g.fs32[X_SD2]);
         std::swap(g.fs32[X_RTEMP], g.fs32[X_RT2]);
     }
+    return end_done;
 }
 
 template <typename T> void ppcg_store_r_impl()
@@ -1948,20 +1999,28 @@ template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double th
 // rewrites z, so z = utemp is not stored: begin reads utemp in its place.
 // inner_steps > 0: inner steps 1..inner_steps are queued behind ur (passes as
 // tea_engine_ppcg_inner_plan splits them) before the wait, which then waits
-// for ur's values only.
+// for ur's values only; with a preconditioner the last pass also forms the
+// next end's r.z (into slot 4 or 5), which the next call takes in place of
+// its end when no other entry point came in between.
 template <typename T>
 void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, double *pw_rro, double *rrn,
                             int inner_steps = 0)
 {
     const bool z_from_utemp = g.prec != PREC_NONE;
     const T *ut = FT<T>(X_UTEMP);
-    if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), ut,
-                       z_from_utemp ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE),
-                       red_ctx()) ||
-        !finish(3, 1, g.d_scal + 3))
+    int eslot = 3;
+    if (g.end_slot >= 0 && g.end_call == g.calls - 1) {
+        eslot = g.end_slot;
+    } else if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), ut,
+                              z_from_utemp ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R),
+                              (const T *)FT<T>(X_RSTORE), red_ctx()) ||
+               !finish(3, 1, g.d_scal + 3)) {
         return;
+    }
+    g.end_slot = -1;
     const bool wp = w_from_p<T>();
-    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + 3, z_from_utemp ? ut : nullptr, wp) || !finish(0, 2, g.d_scal))
+    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + eslot, z_from_utemp ? ut : nullptr, wp) ||
+        !finish(0, 2, g.d_scal))
         return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
     g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
@@ -1970,7 +2029,13 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
         if (hipEventRecord(g.ev_scal, g.stream) != hipSuccess) { die("hipEventRecord failed"); return; }
         for (int step = 1; step <= inner_steps;) {
             const int n = plan_first_pass(inner_steps - step + 1, pass_costs());
-            ppcg_inner_n_impl<T>(step - 1, n);
+            const bool last = step + n - 1 == inner_steps;
+            if (ppcg_inner_n_impl<T>(step - 1, n, last && z_from_utemp && fused_end<T>())) {
+                const int ns = eslot == 4 ? 5 : 4;
+                if (!finish(ns, 1, g.d_scal + ns)) return;
+                g.end_slot = ns;
+                g.end_call = g.calls;
+            }
             step += n;
         }
         const hipError_t err = hipEventSynchronize(g.ev_scal);
@@ -1980,7 +2045,7 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
         return;
     }
     *rrn = g.h_out[2];
-    *rrn_end = g.h_out[3];
+    *rrn_end = g.h_out[eslot];
 }
 
 // With a preconditioner z = utemp is owed rather than stored (the fused
@@ -2007,17 +2072,19 @@ void ppcg_iteration_impl(int with_p, double beta, double theta, int inner_steps,
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
     g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
     g.pending_z = 0;
+    const bool owe_z = g.prec != PREC_NONE;
+    bool end_done = false;   // the last pass formed end's r.z (z = utemp then owed, as end leaves it)
     for (int step = 1; step <= inner_steps;) {
         const int n = plan_first_pass(inner_steps - step + 1, pass_costs());
-        ppcg_inner_n_impl<T>(step - 1, n);
+        end_done = ppcg_inner_n_impl<T>(step - 1, n, step + n - 1 == inner_steps && owe_z && fused_end<T>());
         step += n;
     }
     settle_rt_ut();
-    const bool owe_z = g.prec != PREC_NONE;
-    if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), (const T *)FT<T>(X_UTEMP),
-                       owe_z ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()) ||
-        !finish(3, 1, nullptr))
+    if (!end_done && !launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(),
+                                    (const T *)FT<T>(X_UTEMP), owe_z ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R),
+                                    (const T *)FT<T>(X_RSTORE), red_ctx()))
         return;
+    if (!finish(3, 1, nullptr)) return;
     g.pending_z = owe_z ? 8 * (int)sizeof(T) : 0;
     if (!collect(pw_rro, 0, 2)) return;
     *rrn_ur = g.h_out[2];

@@ -1083,11 +1083,12 @@ __device__ __forceinline__ double lane0_value(double v)
 // its rtemp and utemp are never stored), and an advance a chunk's strips do
 // not need (the pipeline fill above the chunk: strip s needs steps up to
 // NS - its distance from the chunk) is skipped.
-template <typename T, int PREC, int NS>
+template <typename T, int PREC, int NS, bool END = false>
 __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
     Ext e, long pitch, int chunk, const T *__restrict__ sd_in, T *__restrict__ sd_out, const T *__restrict__ rt_in,
     T *__restrict__ rt_out, const T *ut_in, T *ut_out, const T *__restrict__ Mi, const T *__restrict__ Kx,
-    const T *__restrict__ Ky, StepCoefs cf)
+    const T *__restrict__ Ky, StepCoefs cf, const T *__restrict__ r_end,
+    const T *__restrict__ rs_end, Red red)
 {
     static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
     constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS), RN = NS + 1;
@@ -1098,7 +1099,9 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
     const int nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
     const int nchunks = (nstrips + chunk - 1) / chunk;
     const int w = blockIdx.x * MARCH_WAVES + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
-    if (w >= nbands * nchunks) return;
+    // END: the last workgroup's spare waves still join the reduction
+    const bool active = w < nbands * nchunks;
+    if (!END && !active) return;
     const int band = w % nbands, ck = w / nbands;
     const int c = e.x0 + band * OUT_X - HL + lane;
     const bool col_in = c >= e.x0 && c <= e.x1;
@@ -1107,7 +1110,8 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
     const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
     const unsigned bytes = (unsigned)(e.y1 + 2) * row_b;
     const FieldBuf<T> b_sd(sd_in, bytes), b_rt(rt_in, bytes), b_ut(ut_in, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
-        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes), b_uto(ut_out, bytes);
+        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes), b_uto(ut_out, bytes), b_re(r_end, bytes),
+        b_rse(rs_end, bytes);
     auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
     const int s_first = ck * chunk;
     const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
@@ -1223,12 +1227,28 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
     for (int k = 0; k < STRIP; k++) sde0[k] = T(0.0);
     const int i0 = s_first - NS, i1 = s_last + NS;
     MarchRaw<T> nxt;
-    fetch(i0, nxt);
+    DD eacc[1] = {{0.0, 0.0}};
+    // r and rstore at the cells of the strip that leaves finished at
+    // iteration i (strip i-NS, whose slot still holds its rows), 0 elsewhere
+    auto end_loads = [&](const auto &f, int i, T (&er)[STRIP], T (&es)[STRIP]) {
+        const int sf = i - NS;
+        const bool out = out_lane && sf >= s_first && sf <= s_last;
+        const unsigned b0 = off(f.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const unsigned bk = out && col_in && k < f.len ? b0 + k * row_b : OOB;
+            er[k] = b_re.load(bk);
+            es[k] = b_rse.load(bk);
+        }
+    };
+    if (active) fetch(i0, nxt);
     // One iteration at ring phase PH = (i - i0) mod (NS+1): strip i enters
     // slot PH, the strip of age m (i-1-m, slot PH-1-m) moves from step m to
     // m+1, and strip i-NS (slot PH+1) leaves finished.
     auto body = [&](auto ph, int i) {
         constexpr int PH = decltype(ph)::value;
+        T er[STRIP], es[STRIP];   // END: r and rstore of strip i-NS, read before anything else this iteration
+        if constexpr (END) end_loads(R[(PH + 1) % RN], i, er, es);
         MarchStrip<T> &cur = R[PH];
         prepare(i, nxt, cur);
         T sde_cur[STRIP];   // lane 0's values, read out before the next fetch overwrites them
@@ -1257,6 +1277,13 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
             b_rto.store(bk, f.rt[k]);
             b_uto.store(bk, f.ut[k]);
         }
+        if constexpr (END) {   // k_ppcg_end's (r - rstore).z with z = the utemp just written
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const bool o = out && col_in && k < f.len;
+                dd_add_prod(eacc[0], (double)(er[k] - es[k]), o ? (double)f.ut[k] : 0.0);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < STRIP; k++) sde0[k] = sde_cur[k];
     };
@@ -1266,7 +1293,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
     using I3 = std::integral_constant<int, 3>;
     using I4 = std::integral_constant<int, 4>;
     using I5 = std::integral_constant<int, 5>;
-    for (int i = i0;;) {
+    for (int i = i0; active;) {
         body(I0{}, i);
         if (++i > i1) break;
         body(I1{}, i);
@@ -1288,6 +1315,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
             if (++i > i1) break;
         }
     }
+    if constexpr (END) block_reduce<1>(eacc, red);
 }
 
 // k_ppcg_inner_ring with the face coefficients of the strips in flight in
@@ -1298,11 +1326,12 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
 // lane shift gives it), which frees 9 registers per strip: longer passes keep
 // two waves per SIMD.  Each wave owns its part of LDS; its writes and reads
 // are in program order, so no barrier is needed.
-template <typename T, int PREC, int NS>
+template <typename T, int PREC, int NS, bool END = false>
 __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     Ext e, long pitch, int chunk, const T *__restrict__ sd_in, T *__restrict__ sd_out, const T *__restrict__ rt_in,
     T *__restrict__ rt_out, const T *ut_in, T *ut_out, const T *__restrict__ Mi, const T *__restrict__ Kx,
-    const T *__restrict__ Ky, StepCoefs cf)
+    const T *__restrict__ Ky, StepCoefs cf, const T *__restrict__ r_end,
+    const T *__restrict__ rs_end, Red red)
 {
     static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
     constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS), RN = NS + 1;
@@ -1316,7 +1345,9 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     const int nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
     const int nchunks = (nstrips + chunk - 1) / chunk;
     const int w = blockIdx.x * MARCH_WAVES + wv;
-    if (w >= nbands * nchunks) return;
+    // END: the last workgroup's spare waves still join the reduction
+    const bool active = w < nbands * nchunks;
+    if (!END && !active) return;
     T(*L)[NK][LW] = lds[wv];
     if (lane == WAVE - 1) {   // the column past the last lane reads as zero
 #pragma unroll
@@ -1332,7 +1363,8 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
     const unsigned bytes = (unsigned)(e.y1 + 2) * row_b;
     const FieldBuf<T> b_sd(sd_in, bytes), b_rt(rt_in, bytes), b_ut(ut_in, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
-        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes), b_uto(ut_out, bytes);
+        b_ky(Ky, bytes), b_sdo(sd_out, bytes), b_rto(rt_out, bytes), b_uto(ut_out, bytes), b_re(r_end, bytes),
+        b_rse(rs_end, bytes);
     auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
     const int s_first = ck * chunk;
     const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
@@ -1455,9 +1487,25 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     for (int k = 0; k < STRIP; k++) sde0[k] = T(0.0);
     const int i0 = s_first - NS, i1 = s_last + NS;
     MarchRaw<T> nxt;
-    fetch(i0, nxt);
+    DD eacc[1] = {{0.0, 0.0}};
+    // r and rstore at the cells of the strip that leaves finished at
+    // iteration i (strip i-NS, whose slot still holds its rows), 0 elsewhere
+    auto end_loads = [&](const auto &f, int i, T (&er)[STRIP], T (&es)[STRIP]) {
+        const int sf = i - NS;
+        const bool out = out_lane && sf >= s_first && sf <= s_last;
+        const unsigned b0 = off(f.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const unsigned bk = out && col_in && k < f.len ? b0 + k * row_b : OOB;
+            er[k] = b_re.load(bk);
+            es[k] = b_rse.load(bk);
+        }
+    };
+    if (active) fetch(i0, nxt);
     auto body = [&](auto ph, int i) {
         constexpr int PH = decltype(ph)::value;
+        T er[STRIP], es[STRIP];   // END: r and rstore of strip i-NS, read before anything else this iteration
+        if constexpr (END) end_loads(R[(PH + 1) % RN], i, er, es);
         prepare(i, nxt, R[PH], L[PH]);
         T sde_cur[STRIP];
 #pragma unroll
@@ -1486,6 +1534,13 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
             b_rto.store(bk, f.rt[k]);
             b_uto.store(bk, f.ut[k]);
         }
+        if constexpr (END) {   // k_ppcg_end's (r - rstore).z with z = the utemp just written
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const bool o = out && col_in && k < f.len;
+                dd_add_prod(eacc[0], (double)(er[k] - es[k]), o ? (double)f.ut[k] : 0.0);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < STRIP; k++) sde0[k] = sde_cur[k];
     };
@@ -1495,7 +1550,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     using I3 = std::integral_constant<int, 3>;
     using I4 = std::integral_constant<int, 4>;
     using I5 = std::integral_constant<int, 5>;
-    for (int i = i0;;) {
+    for (int i = i0; active;) {
         body(I0{}, i);
         if (++i > i1) break;
         body(I1{}, i);
@@ -1517,6 +1572,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
             if (++i > i1) break;
         }
     }
+    if constexpr (END) block_reduce<1>(eacc, red);
 }
 
 // The fused begin (k_ppcg_begin) as a streaming pass (no reference

@@ -108,7 +108,7 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
                                     "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY",
-                                    "TEA_PPCG_ITERATION"])
+                                    "TEA_PPCG_ITERATION", "TEA_FUSED_END"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
                                   "cg_block_ragged", "cheby_none", "mixed_cheby_diag"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
@@ -122,8 +122,9 @@ def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracl
     kernels (TEA_BEGIN_MARCH=0), Chebyshev iterations on the separate
     calc_p / calc_u kernels (TEA_FUSED_CHEBY=0), and PPCG outer iterations
     through begin_ur, the passes and end as separate calls instead of one
-    (TEA_PPCG_ITERATION=0), on the GPU give the same results as the
-    oracle."""
+    (TEA_PPCG_ITERATION=0), and the end of an outer iteration as its own
+    launch instead of in the last inner pass (TEA_FUSED_END=0), on the GPU
+    give the same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -238,6 +239,17 @@ PPCG_CALLS = {
     "fused_ppcg_end_begin_ur_inner6": lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_end_begin_ur_inner(0.8, 1.7, 6)),
+    # the last pass of one call forms the next call's end (taken only when no other call comes between)
+    "fused_ppcg_end_begin_ur_inner6_twice": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_end_begin_ur_inner(0.8, 1.7, 6), e.ppcg_end_begin_ur_inner(0.6, 1.7, 6),
+        e.ppcg_end_begin_ur(0.7, 1.7)),
+    "fused_ppcg_end_begin_ur_inner6_norm_between": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_end_begin_ur_inner(0.8, 1.7, 6), e.calc_2norm(1), e.ppcg_end_begin_ur_inner(0.6, 1.7, 6)),
+    "fused_ppcg_end_begin_ur_inner6_then_end": lambda e: (
+        e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
+        e.ppcg_end_begin_ur_inner(0.8, 1.7, 6), e.ppcg_end()),
     "fused_ppcg_inner_n4_then_end_begin_ur": lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_begin_ur(False, 0.0, 1.7), e.ppcg_inner_n(1, 4), e.ppcg_end_begin_ur(0.8, 1.7)),

@@ -111,11 +111,11 @@ def variants(n):
     return {
         "headline": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n)))),
         "fp32_extrapolate": dict(inner_precision="fp32", warm_start="extrapolate"),
-        # the Galerkin span over the differences of fewer previous steps (tl_warm_start_history, default 6)
+        # the Galerkin span over the differences of fewer previous steps (tl_warm_start_history, default 8)
         "headline_h2": dict(inner_precision="fp32", warm_start=str(WEIGHTS.get(n, weights_for(n))), warm_start_history=2),
         "fp32_extrapolate_h2": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=2),
         "fp32_extrapolate_h4": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=4),
-        "fp32_extrapolate_h8": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=8),
+        "fp32_extrapolate_h6": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_history=6),
         # the coefficients that minimise the error's energy norm instead of the residual's 2-norm
         "fp32_extrapolate_energy": dict(inner_precision="fp32", warm_start="extrapolate", warm_start_norm="energy"),
         "fp32": dict(inner_precision="fp32"),

@@ -14,7 +14,7 @@
  * handed to the solver is
  *     u = u^n + c0 (u^n - u^{n-1}) + c1 (u^{n-1} - u^{n-2}) + ... + cN s net,
  * over the differences of the last H steps (the deck key
- * tl_warm_start_history H, 1..8, default 6), with the coefficients that
+ * tl_warm_start_history H, 1..8, default 8), with the coefficients that
  * minimise the 2-norm of the residual ||u0 - A u||_2 over that span (least
  * squares: G c = g with G_ab = A v_a . A v_b, g_a = A v_a . (u0 - A u^n)), or
  * with tl_warm_start_norm=energy the error's energy norm ||u* - u||_A (a

@@ -218,7 +218,7 @@ struct Chunk {
     // previous u0 fields, most recent first (a ring: the oldest buffer takes
     // the next u0), ws_hist of them in use
     double *ws_prev[WS_HMAX] = {nullptr};
-    int ws_hist = 6;              // previous-step differences in the guess (tl_warm_start_history)
+    int ws_hist = 8;              // previous-step differences in the guess (tl_warm_start_history)
     bool ws_minres = true;        // minimise the residual's 2-norm (tl_warm_start_norm residual) or the
                                   // error's energy norm (energy)
     int ws_nprev = 0;             // previous u0 fields held (0..ws_hist)
@@ -2726,7 +2726,7 @@ EXPORT int tea_engine_warm_start_load(const float *params, int n)
             die("tl_warm_start_history %d outside 1..%d", hist, WS_HMAX);
             return -1;
         }
-        g.ws_hist = hist == -1 ? 6 : hist;
+        g.ws_hist = hist == -1 ? 8 : hist;
         const std::string norm = read_string("tl_warm_start_norm");
         if (!norm.empty() && norm != "residual" && norm != "energy") {
             die("tl_warm_start_norm %s: expected residual or energy", norm.c_str());

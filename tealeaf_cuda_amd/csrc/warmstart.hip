@@ -605,9 +605,12 @@ template <int ROWS> void forward_rows(const Net &net, const Acts &a, const float
                        a.H, a.W);
 }
 
+// Inference takes workgroups of 16 rows where the height allows it: a smaller
+// share of re-read halo rows (18/16 against 10/8 per staged window).
 void forward(const Net &net, const Acts &a, const float *planes, int npl, hipStream_t stream)
 {
-    if (conv_rows(a.H) == 8) forward_rows<8>(net, a, planes, npl, stream);
+    if (a.H % 16 == 0) forward_rows<16>(net, a, planes, npl, stream);
+    else if (conv_rows(a.H) == 8) forward_rows<8>(net, a, planes, npl, stream);
     else forward_rows<4>(net, a, planes, npl, stream);
 }
 

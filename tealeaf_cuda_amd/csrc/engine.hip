@@ -1645,6 +1645,17 @@ template <typename T, int NS> bool use_march()
     return (double)pitch_of<T>() * (double)g.rows * sizeof(T) < 4294967296.0 - 65536.0;
 }
 
+// Waves per SIMD the row-streaming kernels are sized for (longer runs re-read
+// fewer rows, shorter ones hide more latency): the measured best at 4096^2,
+// 8 for k_begin_march and 4 for k_cheby_march (199 against 212 us per
+// Chebyshev iteration at 8), or the value of the named environment variable.
+int stream_waves(const char *env, int def)
+{
+    const char *s = getenv(env);
+    const int v = s ? atoi(s) : 0;
+    return v > 0 && v <= 32 ? v : def;
+}
+
 int cu_count()
 {
     static int v = [] {
@@ -1877,7 +1888,7 @@ bool begin_march_launch(const char *name, int with_p, double beta, const double 
     if (env && env[0] == '0') return false;
     const Ext e = interior();
     const int nbands = (e.x1 - e.x0 + BEGIN_OUT_X) / BEGIN_OUT_X, ny = e.y1 - e.y0 + 1;
-    const int target = cu_count() * 4 * 8;
+    const int target = cu_count() * 4 * stream_waves("TEA_BEGIN_WAVES", 8);
     int runs = target / nbands;
     runs = runs < 1 ? 1 : runs;
     int rows = (ny + runs - 1) / runs;
@@ -2235,7 +2246,7 @@ template <typename T> void cheby_iterate_impl(double alpha, double beta)
             g.u_ring_dirty = false;
         }
         const int nbands = (e.x1 - e.x0 + BEGIN_OUT_X) / BEGIN_OUT_X, ny = e.y1 - e.y0 + 1;
-        int runs = cu_count() * 4 * 8 / nbands;
+        int runs = cu_count() * 4 * stream_waves("TEA_CHEBY_WAVES", 4) / nbands;
         runs = runs < 1 ? 1 : runs;
         int rows = (ny + runs - 1) / runs;
         rows = rows < 8 ? 8 : rows;

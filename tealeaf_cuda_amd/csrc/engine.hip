@@ -214,14 +214,20 @@ struct Chunk {
     ws::Net ws_net;
     ws::Acts ws_acts;
     float *ws_planes = nullptr;   // 3 x ny x nx features
+    double *ws_qa = nullptr;      // the network's direction q and A q on the field layout (two fields)
     double *ws_slab = nullptr;
     // previous u0 fields, most recent first (a ring: the oldest buffer takes
     // the next u0), ws_hist of them in use
-    double *ws_prev[WS_HMAX] = {nullptr};
+    double *ws_prev[WS_HMAX + 1] = {nullptr};
+    // ws_D[i][j] = v_i . v_j of the history differences v_i = U_i - U_{i+1}
+    // (U_0 = u0, U_{i+1} = ws_prev[i]) for i, j < ws_dn: at each step the old
+    // v_i become v_{i+1}, so only the row of the new v_0 is reduced
+    double ws_D[WS_HMAX + 1][WS_HMAX + 1] = {{0}};
+    int ws_dn = 0;
     int ws_hist = 8;              // previous-step differences in the guess (tl_warm_start_history)
     bool ws_minres = true;        // minimise the residual's 2-norm (tl_warm_start_norm residual) or the
                                   // error's energy norm (energy)
-    int ws_nprev = 0;             // previous u0 fields held (0..ws_hist)
+    int ws_nprev = 0;             // previous u0 fields held (0..ws_hist + 1)
     double ws_coef[WS_HMAX + 1] = {0};  // coefficients of the last guess (history..., network)
 
     Profiler prof;
@@ -755,12 +761,15 @@ __global__ __launch_bounds__(BLOCK) void k_ws_features(Ext e, long pitch, int h,
 // (their faces carry K = 0 under the zero-flux boundary).
 struct WsDirs {
     const double *u0;
-    const double *up[WS_HMAX];
-    const double *la, *lb;   // the two fields of the last history direction (U_{nd-1}, U_nd)
+    const double *up[WS_HMAX + 1];
+    const double *la, *lb;   // has_last: the two fields of the last direction (U_{nd-1}, U_nd)
     const float *net;
     double scale;
+    const double *q, *aq;   // the network's direction (scale * net) and A q, from k_ws_netdir
     int h, nx, ny;
-    int nh;   // history differences available (0..HN)
+    int nh;   // history directions in the guess (0..HN)
+    int nv;   // differences v_0..v_{nv-1} the Gram pass reduces against v_0 (nv <= HN + 1)
+    int has_last;   // A v_{nh-1} by the stencil (v_nh not held yet)
 };
 
 __device__ __forceinline__ double ws_net_at(const WsDirs &d, int c, int r)
@@ -794,68 +803,91 @@ __device__ __forceinline__ void block_reduce_lanes(const double (&v)[NV], const 
     }
 }
 
-// Gram matrix and right-hand side of the combination of the history
-// directions v_a = U_a - U_{a+1} (a < nd = d.nh; U_0 = u0, U_{a+1} = up[a],
-// the earlier steps' u0, newest first) and the network's q that minimises
-// the 2-norm of the residual (MINRES true: G[a][b] = A v_a . A v_b (a <= b,
-// row by row) then g[a] = A v_a . r0) or the energy norm of the error (MINRES
-// false: G[a][b] = v_a . A v_b, g[a] = v_a . r0); directions not present
-// count 0.  Each earlier step's solve left A U_a = U_{a+1} to its tolerance
-// (U_a was that step's solution and U_{a+1} its right-hand side), so
-// A v_a = v_{a+1} for a + 1 < nd and the residual of the u = u0 guess is
-// r0 = u0 - A u0 = v_0, both to that tolerance: only A v_{nd-1} (on d.la -
-// d.lb) and A q take the stencil.
-template <int HN, bool MINRES>
-__global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
-                                                   const double *__restrict__ Ky, Red red)
+// The new dot products of the warm start's Gram system at this step (the host
+// keeps the rest from the steps before).  Each earlier step's solve left
+// A U_a = U_{a+1} to its tolerance (U_a was that step's solution and U_{a+1}
+// its right-hand side), so for the differences v_a = U_a - U_{a+1},
+// A v_a = v_{a+1}, and the residual of the u = u0 guess is r0 = v_0: every
+// entry of the system is a dot product v_i . v_j, and only v_0 is new.  Out
+// (in this order): v_0 . v_j for j < nv; with has_last (A v_{nh-1} by the
+// stencil on la - lb, while v_nh is not held yet) last . last and last . v_j
+// (j < nv); with the network's direction q (A q by the stencil) A q . A q,
+// q . A q, q . v_0, A q . v_j (j < nv) and, with has_last, A q . last.
+// q = scale * net on the field layout and A q (the stencil over q, zero
+// outside the interior), for the Gram pass and the guess.
+__global__ __launch_bounds__(BLOCK) void k_ws_netdir(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
+                                                     const double *__restrict__ Ky, double *__restrict__ q,
+                                                     double *__restrict__ aq)
 {
-    constexpr int D = HN + 1, NG = D * (D + 1) / 2;
-    const int nd = d.nh;
-    double acc[NG + D];
-#pragma unroll
-    for (int j = 0; j < NG + D; j++) acc[j] = 0.0;
     STRIP_BEGIN
     for (int k = 0; k < s.len; k++) {
         const long i = i0 + k * pitch;
         const int c = s.c, r = s.r0 + k;
-        double v[D], av[D];
-        double prev = d.u0[i];
-        v[HN] = 0.0;
-        av[HN] = 0.0;
+        const double qv = ws_net_at(d, c, r);
+        const double ky1 = Ky[i + pitch], ky0 = Ky[i], kx1 = Kx[i + 1], kx0 = Kx[i];
+        q[i] = qv;
+        aq[i] = (1.0 + (ky1 + ky0) + (kx1 + kx0)) * qv -
+                (ky1 * ws_net_at(d, c, r + 1) + ky0 * ws_net_at(d, c, r - 1)) -
+                (kx1 * ws_net_at(d, c + 1, r) + kx0 * ws_net_at(d, c - 1, r));
+    }
+    STRIP_END
+}
+
+// LAST and NET (has_last and the network) are template switches: the
+// steady-state pass without the network keeps NV sums only.
+template <int HN, bool LAST, bool NET>
+__global__ __launch_bounds__(BLOCK) void k_ws_gram(Ext e, long pitch, WsDirs d, const double *__restrict__ Kx,
+                                                   const double *__restrict__ Ky, Red red)
+{
+    constexpr int NV = HN + 1, NACC = NET ? 3 * NV + 5 : (LAST ? 2 * NV + 1 : NV);
+    const int nv = d.nv;
+    double acc[NACC];
 #pragma unroll
-        for (int j = 0; j < HN; j++) {
+    for (int j = 0; j < NACC; j++) acc[j] = 0.0;
+    STRIP_BEGIN
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        const int c = s.c, r = s.r0 + k;
+        double v[NV];
+        double prev = d.u0[i];
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
             v[j] = 0.0;
-            if (j < nd) {
+            if (j < nv) {
                 const double uj = d.up[j][i];
                 v[j] = prev - uj;
                 prev = uj;
             }
         }
-        // A v_{nd-1}: the stencil on the difference of its two fields
-        const double ky1 = Ky[i + pitch], ky0 = Ky[i], kx1 = Kx[i + 1], kx0 = Kx[i];
-        const double last = (1.0 + (ky1 + ky0) + (kx1 + kx0)) * (d.la[i] - d.lb[i]) -
-                            (ky1 * (d.la[i + pitch] - d.lb[i + pitch]) + ky0 * (d.la[i - pitch] - d.lb[i - pitch])) -
-                            (kx1 * (d.la[i + 1] - d.lb[i + 1]) + kx0 * (d.la[i - 1] - d.lb[i - 1]));
 #pragma unroll
-        for (int j = 0; j < HN; j++) av[j] = j + 1 < nd ? v[j + 1] : j + 1 == nd ? last : 0.0;
-        if (d.net) {
-            const double q = ws_net_at(d, c, r);
-            v[HN] = q;
-            av[HN] = (1.0 + (ky1 + ky0) + (kx1 + kx0)) * q -
-                     (ky1 * ws_net_at(d, c, r + 1) + ky0 * ws_net_at(d, c, r - 1)) -
-                     (kx1 * ws_net_at(d, c + 1, r) + kx0 * ws_net_at(d, c - 1, r));
+        for (int j = 0; j < NV; j++) acc[j] += v[0] * v[j];
+        constexpr int q0 = NV, q = 2 * NV + 1;
+        double last = 0.0;
+        if constexpr (LAST || NET) {
+            if constexpr (LAST) {
+                const double ky1 = Ky[i + pitch], ky0 = Ky[i], kx1 = Kx[i + 1], kx0 = Kx[i];
+                last = (1.0 + (ky1 + ky0) + (kx1 + kx0)) * (d.la[i] - d.lb[i]) -
+                       (ky1 * (d.la[i + pitch] - d.lb[i + pitch]) + ky0 * (d.la[i - pitch] - d.lb[i - pitch])) -
+                       (kx1 * (d.la[i + 1] - d.lb[i + 1]) + kx0 * (d.la[i - 1] - d.lb[i - 1]));
+                acc[q0] += last * last;
+#pragma unroll
+                for (int j = 0; j < NV; j++) acc[q0 + 1 + j] += last * v[j];
+            }
+            if constexpr (NET) {
+                const double qv = d.q[i], aq = d.aq[i];
+                acc[q] += aq * aq;
+                acc[q + 1] += qv * aq;
+                acc[q + 2] += qv * v[0];
+                acc[q + 3] += aq * last;
+#pragma unroll
+                for (int j = 0; j < NV; j++) acc[q + 4 + j] += aq * v[j];
+            }
         }
-        const double r0 = v[0];
-        int q = 0;
-#pragma unroll
-        for (int x = 0; x < D; x++)
-#pragma unroll
-            for (int y = x; y < D; y++) acc[q++] += (MINRES ? av[x] : v[x]) * av[y];
-#pragma unroll
-        for (int x = 0; x < D; x++) acc[NG + x] += (MINRES ? av[x] : v[x]) * r0;
+        (void)c;
+        (void)r;
     }
     STRIP_END
-    block_reduce_lanes<NG + D>(acc, red);
+    block_reduce_lanes<NACC>(acc, red);
 }
 
 // u = u0 + sum_j c[j] v_j + c[HN] v_HN (apply == 0: no guess); and u0 goes to
@@ -877,7 +909,7 @@ __global__ __launch_bounds__(BLOCK) void k_ws_apply(Ext e, long pitch, double *_
                 if (j < d.nh && cf.c[j] != 0.0) v += cf.c[j] * (up_prev - d.up[j][i]);
                 if (j < d.nh) up_prev = d.up[j][i];
             }
-            if (cf.c[HN] != 0.0) v += cf.c[HN] * ws_net_at(d, s.c, s.r0 + k);
+            if (cf.c[HN] != 0.0) v += cf.c[HN] * d.q[i];   // q = scale * net (k_ws_netdir)
             u[i] = v;
         }
         keep[i] = u0;
@@ -1028,6 +1060,7 @@ void release_all()
     ws::net_free(g.ws_net);
     ws::acts_free(g.ws_acts);
     if (g.ws_planes) (void)hipFree(g.ws_planes);
+    if (g.ws_qa) (void)hipFree(g.ws_qa);
     if (g.ws_slab) (void)hipFree(g.ws_slab);
     if (g.stream) (void)hipStreamDestroy(g.stream);
     Profiler keep = g.prof;
@@ -2745,25 +2778,32 @@ EXPORT int tea_engine_warm_start_load(const float *params, int n)
         }
         g.ws_minres = norm != "energy";
         const size_t elems = (size_t)g.pitch * (size_t)g.rows + 16;
-        HIP_OK_R(hipMalloc(&g.ws_slab, (size_t)g.ws_hist * elems * sizeof(double)), -1);
-        HIP_OK_R(hipMemset(g.ws_slab, 0, (size_t)g.ws_hist * elems * sizeof(double)), -1);
-        for (int j = 0; j < g.ws_hist; j++) g.ws_prev[j] = g.ws_slab + (g.fs[F_U] - g.f[F_U]) + (size_t)j * elems;
+        HIP_OK_R(hipMalloc(&g.ws_slab, (size_t)(g.ws_hist + 1) * elems * sizeof(double)), -1);
+        HIP_OK_R(hipMemset(g.ws_slab, 0, (size_t)(g.ws_hist + 1) * elems * sizeof(double)), -1);
+        for (int j = 0; j <= g.ws_hist; j++) g.ws_prev[j] = g.ws_slab + (g.fs[F_U] - g.f[F_U]) + (size_t)j * elems;
     }
     if (params) {
         if (!g.ws_net.params) {
             HIP_OK_R(ws::net_alloc(g.ws_net), -1);
             HIP_OK_R(ws::acts_alloc(g.ws_acts, 1, g.ny, g.nx, false), -1);
             HIP_OK_R(hipMalloc(&g.ws_planes, 3 * (size_t)g.nx * g.ny * sizeof(float)), -1);
+            const size_t elems = (size_t)g.pitch * (size_t)g.rows + 16;
+            HIP_OK_R(hipMalloc(&g.ws_qa, 2 * elems * sizeof(double)), -1);
         }
         HIP_OK_R(ws::net_upload(g.ws_net, params, g.stream), -1);
     }
     g.ws_on = true;
     g.ws_use_net = params != nullptr;
     g.ws_nprev = 0;
+    g.ws_dn = 0;
     return 0;
 }
 
-EXPORT void tea_engine_warm_start_reset(void) { g.ws_nprev = 0; }
+EXPORT void tea_engine_warm_start_reset(void)
+{
+    g.ws_nprev = 0;
+    g.ws_dn = 0;
+}
 
 namespace {
 // Solves the k x k system G c = b (G symmetric positive semi-definite) by
@@ -2805,15 +2845,17 @@ void ws_solve(int k, double G[WS_DMAX][WS_DMAX], const double *b, double *c)
 
 namespace {
 // At most WS_GRAM_GRID workgroups, each over many tiles: the workgroup's
-// reduction of up to MAX_RED values then costs little beside its cells.
+// reduction of up to 3 (HN + 1) + 5 values then costs little beside its cells
+// (measured against 4096 and 16384 at 4096^2).
 constexpr int WS_GRAM_GRID = 1024;
 template <int HN> bool ws_gram_launch(const WsDirs &d)
 {
-    if (g.ws_minres)
-        return launch_capped("ws_gram", WS_GRAM_GRID, k_ws_gram<HN, true>, interior(), g.pitch, d,
-                             (const double *)F(X_KX), (const double *)F(X_KY), red_ctx());
-    return launch_capped("ws_gram", WS_GRAM_GRID, k_ws_gram<HN, false>, interior(), g.pitch, d,
-                         (const double *)F(X_KX), (const double *)F(X_KY), red_ctx());
+    auto go = [&](auto kernel) {
+        return launch_capped("ws_gram", WS_GRAM_GRID, kernel, interior(), g.pitch, d, (const double *)F(X_KX),
+                             (const double *)F(X_KY), red_ctx());
+    };
+    if (d.net) return d.has_last ? go(k_ws_gram<HN, true, true>) : go(k_ws_gram<HN, false, true>);
+    return d.has_last ? go(k_ws_gram<HN, true, false>) : go(k_ws_gram<HN, false, false>);
 }
 template <int HN> bool ws_apply_launch(const WsDirs &d, double *keep, const WsCoefs &cf, int apply)
 {
@@ -2830,13 +2872,15 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
     const int HN = g.ws_hist;
     WsDirs d{};
     d.u0 = (const double *)F(X_U0);
-    for (int j = 0; j < HN; j++) d.up[j] = g.ws_prev[j];
+    for (int j = 0; j <= HN; j++) d.up[j] = g.ws_prev[j];
     d.net = nullptr;
     d.scale = 0.0;
     d.h = g.h;
     d.nx = g.nx;
     d.ny = g.ny;
-    d.nh = g.ws_nprev < HN ? g.ws_nprev : HN;
+    d.nv = g.ws_nprev < HN + 1 ? g.ws_nprev : HN + 1;   // differences held
+    d.nh = d.nv < HN ? d.nv : HN;                        // directions in the guess
+    d.has_last = d.nh > 0 && d.nh == d.nv;               // v_nh not held yet: A v_{nh-1} by the stencil
     d.la = d.nh > 1 ? d.up[d.nh - 2] : d.u0;
     d.lb = d.nh > 0 ? d.up[d.nh - 1] : d.u0;
     WsCoefs cf{};
@@ -2861,6 +2905,13 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
                 // the network predicts (du - (u0 - up1)) / s: direction s * net
                 d.net = g.ws_acts.out;
                 d.scale = sc;
+                const size_t elems = (size_t)g.pitch * (size_t)g.rows + 16;
+                double *q = g.ws_qa + (g.fs[F_U] - g.f[F_U]), *aq = q + elems;
+                if (!launch("ws_netdir", k_ws_netdir, in, g.pitch, d, (const double *)F(X_KX), (const double *)F(X_KY),
+                            q, aq))
+                    return;
+                d.q = q;
+                d.aq = aq;
             }
         }
         bool ok;
@@ -2875,13 +2926,45 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
         default: ok = ws_gram_launch<8>(d); break;
         }
         if (!ok) return;
-        const int D = HN + 1, NG = D * (D + 1) / 2;
-        double r[MAX_RED];
-        if (!fetch(r, NG + D)) return;
-        double G[WS_DMAX][WS_DMAX], b[WS_DMAX];
-        for (int x = 0, q = 0; x < D; x++)
-            for (int y = x; y < D; y++, q++) G[x][y] = G[y][x] = r[q];
-        for (int x = 0; x < D; x++) b[x] = r[NG + x];
+        const int NV = HN + 1, nv = d.nv;
+        double r[MAX_RED] = {0};
+        const int nred = d.net ? 3 * NV + 5 : (d.has_last ? 2 * NV + 1 : NV);
+        if (!fetch(r, nred)) return;
+        // the old differences v_i are this step's v_{i+1}: their dot products carry over
+        if (g.ws_dn < nv - 1) { die("tea_engine_warm_start_apply: history bookkeeping lost"); return; }
+        double Dn[WS_HMAX + 1][WS_HMAX + 1] = {{0}};
+        for (int x = 0; x + 1 < nv; x++)
+            for (int y = 0; y + 1 < nv; y++) Dn[x + 1][y + 1] = g.ws_D[x][y];
+        for (int j = 0; j < nv; j++) Dn[0][j] = Dn[j][0] = r[j];
+        for (int x = 0; x < nv; x++)
+            for (int y = 0; y < nv; y++) g.ws_D[x][y] = Dn[x][y];
+        g.ws_dn = nv;
+        const double LL = r[NV], *X = r + NV + 1;                  // last . last, last . v_j
+        const double AqAq = r[2 * NV + 1], qAq = r[2 * NV + 2], qv0 = r[2 * NV + 3], Aqlast = r[2 * NV + 4];
+        const double *Y = r + 2 * NV + 5;                          // A q . v_j
+        // A v_a = v_{a+1} (index a+1) or, for the last direction with has_last, the stencil's (-1)
+        auto img = [&](int a) { return a + 1 < nv ? a + 1 : -1; };
+        auto AA = [&](int a, int b) {   // A v_a . A v_b
+            const int ia = img(a), ib = img(b);
+            if (ia >= 0 && ib >= 0) return Dn[ia][ib];
+            if (ia < 0 && ib < 0) return LL;
+            return X[ia >= 0 ? ia : ib];
+        };
+        auto VA = [&](int a, int b) {   // v_a . A v_b
+            const int ib = img(b);
+            return ib >= 0 ? Dn[a][ib] : X[a];
+        };
+        double G[WS_DMAX][WS_DMAX] = {{0}}, b[WS_DMAX] = {0};
+        for (int x = 0; x < d.nh; x++) {
+            for (int y = x; y < d.nh; y++) G[x][y] = G[y][x] = g.ws_minres ? AA(x, y) : VA(x, y);
+            b[x] = g.ws_minres ? (img(x) >= 0 ? Dn[img(x)][0] : X[0]) : Dn[x][0];
+        }
+        if (d.net) {
+            for (int x = 0; x < d.nh; x++)
+                G[x][HN] = G[HN][x] = g.ws_minres ? (img(x) >= 0 ? Y[img(x)] : Aqlast) : Y[x];
+            G[HN][HN] = g.ws_minres ? AqAq : qAq;
+            b[HN] = g.ws_minres ? Y[0] : qv0;
+        }
         // active directions in order: the history differences held, [the network]
         int idx[WS_DMAX], k = 0;
         for (int j = 0; j < d.nh; j++) idx[k++] = j;
@@ -2898,7 +2981,7 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
     for (int a = 0; a <= WS_HMAX; a++) g.ws_coef[a] = 0.0;
     for (int a = 0; a <= HN; a++) g.ws_coef[a] = cf.c[a];
     // u0 goes to the oldest buffer of the ring, which then becomes up_1
-    double *keep = g.ws_prev[HN - 1];
+    double *keep = g.ws_prev[HN];
     bool ok;
     switch (HN) {
     case 1: ok = ws_apply_launch<1>(d, keep, cf, apply); break;
@@ -2911,9 +2994,9 @@ EXPORT void tea_engine_warm_start_apply(int *applied)
     default: ok = ws_apply_launch<8>(d, keep, cf, apply); break;
     }
     if (!ok) return;
-    for (int j = HN - 1; j > 0; j--) g.ws_prev[j] = g.ws_prev[j - 1];
+    for (int j = HN; j > 0; j--) g.ws_prev[j] = g.ws_prev[j - 1];
     g.ws_prev[0] = keep;
-    g.ws_nprev = g.ws_nprev < HN ? g.ws_nprev + 1 : HN;
+    g.ws_nprev = g.ws_nprev < HN + 1 ? g.ws_nprev + 1 : HN + 1;
     *applied = apply;
 }
 

@@ -78,6 +78,8 @@ COMPULSORY_BYTES = {
     # inner-step pass (any number of steps): sd, rtemp, utemp, Kx, Ky read; sd, rtemp, utemp written
     **{f"k_ppcg_inner{n}_f32": 32 for n in range(1, 6)},
     **{f"k_ppcg_inner{n}": 64 for n in range(1, 6)},
+    # the last pass of an FP64 outer iteration also reads r and rstore for the end's r.z
+    **{f"k_ppcg_inner{n}_end": 80 for n in range(1, 6)},
     "k_ppcg_inner_fused_f32": 32, "k_ppcg_inner_fused": 64,
     # fused begin: p, z (utemp), Kx, Ky, r read; p', r_store written (w formed again by ur)
     "ppcg_begin_f32": 28, "ppcg_begin": 56,

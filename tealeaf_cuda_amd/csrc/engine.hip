@@ -1735,9 +1735,12 @@ bool launch_inner_march(const Ext &e, long pt, const StepCoefs &cf, const char *
                 FT<T>(X_SD2), rt_in, FT<T>(X_RT2), ut_in, FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
                 (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
     };
+    // the instances with the end's reduction are timed under their own name
+    static const char *end_names[MAX_PASS_STEPS] = {"k_ppcg_inner1_end", "k_ppcg_inner2_end", "k_ppcg_inner3_end",
+                                                    "k_ppcg_inner4_end", "k_ppcg_inner5_end"};
     auto go_ring = [&](auto kernel, const T *re, const T *rse) {
         g.last_grid = (int)grid;
-        klaunch(name, kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk, (const T *)FT<T>(F_SD),
+        klaunch(re ? nm<T>(end_names[NS - 1]) : name, kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk, (const T *)FT<T>(F_SD),
                 FT<T>(X_SD2), rt_in, FT<T>(X_RT2), ut_in, FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
                 (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf, re, rse, red_ctx());
     };

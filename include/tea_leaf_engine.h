@@ -264,6 +264,13 @@ void tea_engine_ppcg_iteration(int with_p, double beta, double theta, int inner_
  * boundaries (the reference driver's update_halo of p between calc_p and
  * calc_w is a no-op there). */
 void tea_engine_cg_begin_ur(int with_p, double beta, double rro, double *pw, double *rrn);
+/* ---- Chebyshev iterations in passes ------------------------------------------- *
+ * No counterpart in the reference: tea_leaf_cheby_iterate_kernel_cuda_ for
+ * cheby_calc_step = step .. step + nsteps - 1 (tea_leaf_kernel_cuda.cu:125-135),
+ * with identical results, in streaming passes of several iterations over HBM
+ * (temporal blocking).  For decks without reflective boundaries (the reference
+ * driver's update_halo of u between iterations is a no-op there). */
+void tea_engine_cheby_iterate_n(int step, int nsteps);
 
 /* Inner steps `step` .. `step + nsteps - 1` (1-based; nsteps 1..5) in one
  * pass over HBM (temporal blocking): the same results as

@@ -662,6 +662,13 @@ EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
     if (g.fp32) cheby_step_f32(0, 0.0, g.ch_a[k], g.ch_b[k]); else cheby_step_f64(0, 0.0, g.ch_a[k], g.ch_b[k]);
 }
 
+/* The engine's run of Chebyshev iterations, as the composition it names. */
+EXPORT void tea_engine_cheby_iterate_n(int step, int nsteps)
+{
+    if (nsteps < 1) { die("cheby_iterate_n: %d steps", nsteps); return; }
+    for (int s = step; s < step + nsteps; s++) tea_leaf_cheby_iterate_kernel_cuda_(&s);
+}
+
 /* ---------------- solver: Jacobi ----------------------------------------------------- */
 
 /* tea_leaf_kernel_cuda.cu:221-228; tea_leaf_jacobi.cuknl:5-46 (un lives in the Mi buffer) */

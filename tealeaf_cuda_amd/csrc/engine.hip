@@ -2309,6 +2309,90 @@ template <typename T> void cheby_iterate_impl(double alpha, double beta)
     if (!cheby_p<T, 0>(0.0, alpha, beta)) return;
     launch(nm<T>("cheby_calc_u"), k_add<T>, interior(), pitch_of<T>(), FT<T>(F_U), (const T *)FT<T>(F_P));
 }
+
+// Chebyshev iterations per streaming pass (k_cheby_ring): the most whose
+// pipeline fits 256 registers at two waves per SIMD in FP32 and one in FP64.
+template <typename T> constexpr int cheby_pass_max() { return sizeof(T) == 4 ? 4 : 3; }
+
+// Chebyshev iterations k0+1 .. k0+n (0-based coefficient index k0) as
+// streaming passes of up to cheby_pass_max iterations (TEA_CHEBY_PASSES=0:
+// one cheby_iterate_impl each); fields of 4 GiB or more take the latter.
+template <typename T> void cheby_iterate_n_impl(int k0, int n)
+{
+    const char *env = getenv("TEA_CHEBY_PASSES");
+    const bool fits = (double)pitch_of<T>() * (double)g.rows * sizeof(T) < 4294967296.0 - 65536.0;
+    if ((env && env[0] == '0') || !fits) {
+        for (int k = k0; k < k0 + n; k++) cheby_iterate_impl<T>(g.ch_a[k], g.ch_b[k]);
+        return;
+    }
+    const Ext e = interior();
+    const long pt = pitch_of<T>();
+    const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
+    for (int k = k0; k < k0 + n;) {
+        const int ns = std::min(cheby_pass_max<T>(), k0 + n - k);
+        if (g.u_ring_dirty) {   // the spare takes u's halo, which the pass keeps
+            Timed t("u_halo_ring");
+            hipLaunchKernelGGL(k_copy_ring<T>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream,
+                               FT<T>(X_U2), (const T *)FT<T>(F_U), pt, g.h, g.nx, g.ny);
+            g.u_ring_dirty = false;
+        }
+        bool &p_dirty = g.p_halo_dirty[std::is_same<T, double>::value ? 0 : 1];
+        if (p_dirty) {   // and p's, which the pass keeps as well
+            Timed t("p_halo_ring");
+            hipLaunchKernelGGL(k_copy_ring<T>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream,
+                               FT<T>(X_P2), (const T *)FT<T>(F_P), pt, g.h, g.nx, g.ny);
+            p_dirty = false;
+        }
+        StepCoefs cf{};
+        for (int j = 0; j < ns; j++) {
+            cf.a[j] = g.ch_a[k + j];
+            cf.b[j] = g.ch_b[k + j];
+        }
+        auto go = [&](auto ns_c) {
+            constexpr int NS = decltype(ns_c)::value;
+            constexpr int OX = march_out_x(NS);
+            const int nbands = (e.x1 - e.x0 + OX) / OX, nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
+            const int per_simd = sizeof(T) == 4 ? 2 : 1;
+            int nchunks = cu_count() * 4 * per_simd / nbands;
+            nchunks = nchunks < 1 ? 1 : (nchunks > nstrips ? nstrips : nchunks);
+            int chunk = (nstrips + nchunks - 1) / nchunks;
+            if (const char *c = getenv("TEA_MARCH_CHUNK")) {
+                const int v = atoi(c);
+                if (v > 0) chunk = v < nstrips ? v : nstrips;
+            }
+            nchunks = (nstrips + chunk - 1) / chunk;
+            const unsigned grid = (unsigned)((nbands * nchunks + MARCH_WAVES - 1) / MARCH_WAVES);
+            auto launch_prec = [&](auto kernel) {
+                klaunch(nm<T>("cheby_pass"), kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk,
+                        (const T *)FT<T>(F_U), FT<T>(X_U2), (const T *)FT<T>(F_P), FT<T>(X_P2),
+                        (const T *)FT<T>(X_U0), FT<T>(F_R), FT<T>(X_W), (const T *)FT<T>(X_MI),
+                        (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf);
+            };
+            if (g.prec == PREC_BLOCK) launch_prec(k_cheby_ring<T, PREC_BLOCK, NS>);
+            else if (g.prec == PREC_DIAG) launch_prec(k_cheby_ring<T, PREC_DIAG, NS>);
+            else launch_prec(k_cheby_ring<T, PREC_NONE, NS>);
+        };
+        switch (ns) {
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 3: go(std::integral_constant<int, 3>{}); break;
+        default: go(std::integral_constant<int, cheby_pass_max<T>() >= 4 ? 4 : 3>{}); break;
+        }
+        const hipError_t err = hipGetLastError();
+        if (err != hipSuccess) { die("launch of k_cheby_ring failed: %s", hipGetErrorString(err)); return; }
+        if (std::is_same<T, double>::value) {
+            std::swap(g.f[F_U], g.f[X_U2]);
+            std::swap(g.fs[F_U], g.fs[X_U2]);
+            std::swap(g.f[F_P], g.f[X_P2]);
+            std::swap(g.fs[F_P], g.fs[X_P2]);
+        } else {
+            std::swap(g.fs32[F_U], g.fs32[X_U2]);
+            std::swap(g.fs32[F_P], g.fs32[X_P2]);
+        }
+        g.pending_cheby = 0;   // the pass wrote the last iteration's w and r
+        k += ns;               // both u buffers keep u's halo ring: the spare stays clean
+    }
+}
 }  // namespace
 
 EXPORT void tea_leaf_cheby_init_kernel_cuda_(const double *ch_alphas, const double *ch_betas, int *n_coefs,
@@ -2332,6 +2416,23 @@ EXPORT void tea_leaf_cheby_iterate_kernel_cuda_(const int *cheby_calc_step)
         return;
     }
     BY_PRECISION(cheby_iterate_impl, g.ch_a[k], g.ch_b[k]);
+}
+
+// Chebyshev iterations step .. step + nsteps - 1 (1-based coefficient index,
+// as tea_leaf_cheby_iterate_kernel_cuda_ takes it) in streaming passes.
+EXPORT void tea_engine_cheby_iterate_n(int step, int nsteps)
+{
+    CHECK_INIT_KEEP_PENDING();
+    g.pending_cheby = 0;   // the passes form w and r again
+    settle_pending();
+    REQUIRE_SOLVER(1 << SOLVER_CHEBY);
+    const int k = step - 1;
+    if (nsteps < 1 || k < 0 || k + nsteps > (int)g.ch_a.size()) {
+        die("cheby_iterate_n: steps %d..%d outside the %d uploaded coefficients", step, step + nsteps - 1,
+            (int)g.ch_a.size());
+        return;
+    }
+    BY_PRECISION(cheby_iterate_n_impl, k, nsteps);
 }
 
 EXPORT void tea_leaf_jacobi_solve_kernel_cuda_(double *error)

@@ -1340,6 +1340,221 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ring(
     if constexpr (END) block_reduce<1>(eacc, red);
 }
 
+// NS Chebyshev iterations (tea_leaf_cheby.cuknl:84-152 then :71-82, the
+// expressions of k_cheby_march: w = A u, r = u0 - w, p = alpha_k p + beta_k
+// M^-1 r, u = u + p) in one streaming pass, with the pipeline of
+// k_ppcg_inner_ring: bands of WAVE lanes (NS - 1 halo lanes at the left, NS
+// at the right) walk down chunks of strips, each strip advanced NS times in
+// its ring slot (u the stencil field in place of sd, p in place of rtemp, u0
+// read-only in place of utemp; no reference counterpart as a kernel, the same
+// values bit for bit).  u and p go to their spare buffers (neighbouring waves
+// read both); the last iteration's w and r, which the reference leaves in
+// their fields, are written by the strip's last advance.
+template <typename T, int PREC, int NS>
+__global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_cheby_ring(
+    Ext e, long pitch, int chunk, const T *__restrict__ u_in, T *__restrict__ u_out, const T *__restrict__ p_in,
+    T *__restrict__ p_out, const T *__restrict__ u0, T *__restrict__ r_out, T *__restrict__ w_out,
+    const T *__restrict__ Mi, const T *__restrict__ Kx, const T *__restrict__ Ky, StepCoefs cf)
+{
+    static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 iterations per pass");
+    constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS), RN = NS + 1;
+    static_assert(OUT_X == WAVE - HL - HR, "band layout");
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (e.x1 < e.x0 || e.y1 < e.y0) return;
+    const int nbands = (e.x1 - e.x0 + OUT_X) / OUT_X;
+    const int nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
+    const int nchunks = (nstrips + chunk - 1) / chunk;
+    const int w = blockIdx.x * MARCH_WAVES + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    if (w >= nbands * nchunks) return;
+    const int band = w % nbands, ck = w / nbands;
+    const int c = e.x0 + band * OUT_X - HL + lane;
+    const bool col_in = c >= e.x0 && c <= e.x1;
+    const bool col_mem = c >= e.x0 - 1 && c <= e.x1 + 1;
+    const bool out_lane = lane >= HL && lane <= WAVE - 1 - HR;
+    const unsigned row_b = (unsigned)pitch * (unsigned)sizeof(T);
+    const unsigned bytes = (unsigned)(e.y1 + 2) * row_b;
+    const FieldBuf<T> b_u(u_in, bytes), b_p(p_in, bytes), b_u0(u0, bytes), b_mi(Mi, bytes), b_kx(Kx, bytes),
+        b_ky(Ky, bytes), b_uo(u_out, bytes), b_po(p_out, bytes), b_ro(r_out, bytes), b_wo(w_out, bytes);
+    auto off = [&](int r0) { return ((unsigned)c + (unsigned)r0 * (unsigned)pitch) * (unsigned)sizeof(T); };
+    const int s_first = ck * chunk;
+    const int s_last = (s_first + chunk < nstrips ? s_first + chunk : nstrips) - 1;
+
+    auto strip_len = [&](int r0) {   // rows of the strip inside the extent (uniform across the wave)
+        int len = r0 < e.y0 ? 0 : e.y1 - r0 + 1;
+        return len < 0 ? 0 : (len > STRIP ? STRIP : len);
+    };
+    // Issues the reads of strip s: u (sd's offsets: the halo rows and columns
+    // read as stored), p and u0 at the cells inside the extent, lane 0's left
+    // column of u.
+    auto fetch = [&](int s, MarchRaw<T> &m) {
+        const int r0 = e.y0 + s * STRIP, len = strip_len(r0), act = col_in ? len : 0;
+        const unsigned b0 = off(r0);
+        constexpr unsigned W = sizeof(T);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const int rr = r0 + k;
+            m.sd[k] = b_u.load((col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) ? b0 + k * row_b : OOB);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = b_ky.load((k <= act && act > 0) ? b0 + k * row_b : OOB);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const bool a = k < act;
+            const unsigned bk = b0 + k * row_b;
+            m.kx[k] = b_kx.load(col_mem && k < len ? bk : OOB);
+            m.rt[k] = b_p.load(a ? bk : OOB);
+            m.ut[k] = b_u0.load(a ? bk : OOB);
+            m.mi[k] = PREC == PREC_DIAG ? b_mi.load(a ? bk : OOB) : T(0.0);
+            m.sde[k] = b_u.load(a && lane == 0 ? bk - W : OOB);
+        }
+    };
+    auto prepare = [&](int s, const MarchRaw<T> &raw, MarchStrip<T> &m) {
+        m.r0 = e.y0 + s * STRIP;
+        m.len = strip_len(m.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            m.sd[k] = raw.sd[k];
+            m.rt[k] = raw.rt[k];
+            m.ut[k] = raw.ut[k];
+            m.kx[k] = raw.kx[k];
+            m.mi[k] = raw.mi[k];
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP + 1; k++) m.ky[k] = raw.ky[k];
+        if (PREC == PREC_BLOCK && m.len > 0) {
+            T diag[STRIP];
+#pragma unroll
+            for (int k = 0; k < STRIP; k++) {
+                const T kxr = lane_from_above(m.kx[k]);
+                diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (kxr + m.kx[k]);
+            }
+            if (m.len == STRIP) strip_factor<T>(STRIP, diag, m.ky, m.cp, m.bfp);
+            else strip_factor<T>(m.len, diag, m.ky, m.cp, m.bfp);
+        }
+    };
+    // One Chebyshev iteration of strip m in place (lo, hi: u of the rows
+    // below and above it; sde: lane 0's left column at iteration 0); the last
+    // one (st == NS-1) of a strip this wave outputs also writes w and r.
+    auto advance_len = [&](auto len_c, MarchStrip<T> &m, T lo, T hi, const T (&sde)[STRIP], int st, bool out_s) {
+        constexpr int LEN = decltype(len_c)::value;
+        const int len = LEN == STRIP ? STRIP : m.len;
+        const T alpha = (T)cf.a[st], beta = (T)cf.b[st];
+        T wv[STRIP], rv[STRIP], zv[STRIP], diag[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const T from_left = lane_from_below(m.sd[k]);
+            const T from_right = lane_from_above(m.sd[k]);
+            const T kxr = lane_from_above(m.kx[k]);
+            diag[k] = T(1.0) + (m.ky[k + 1] + m.ky[k]) + (kxr + m.kx[k]);
+            const T below = k == 0 ? lo : m.sd[k - 1];
+            const T above = k == STRIP - 1 ? hi : m.sd[k + 1];
+            const T left = (st > 0 || lane > 0) ? from_left : sde[k];
+            wv[k] = diag[k] * m.sd[k] - (m.ky[k + 1] * above + m.ky[k] * below) - (kxr * from_right + m.kx[k] * left);
+            rv[k] = m.ut[k] - wv[k];
+        }
+        if (PREC == PREC_BLOCK) strip_apply<T>(rv, zv, len, diag[0], m.ky, m.cp, m.bfp);
+        const bool wr = st == NS - 1 && out_s;
+        const unsigned b0 = off(m.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            if (LEN == STRIP || k < len) {
+                T pv;
+                if (PREC == PREC_BLOCK) pv = alpha * m.rt[k] + beta * zv[k];
+                else if (PREC == PREC_DIAG) pv = alpha * m.rt[k] + beta * m.mi[k] * rv[k];
+                else pv = alpha * m.rt[k] + beta * rv[k];
+                if (wr) {
+                    const unsigned bk = col_in ? b0 + k * row_b : OOB;
+                    b_wo.store(bk, wv[k]);
+                    b_ro.store(bk, rv[k]);
+                }
+                m.rt[k] = col_in ? pv : m.rt[k];
+                m.sd[k] = col_in ? m.sd[k] + pv : m.sd[k];
+            }
+        }
+    };
+    auto advance = [&](MarchStrip<T> &m, T lo, T hi, const T (&sde)[STRIP], int st, bool out_s) {
+        if (m.len == STRIP) advance_len(std::integral_constant<int, STRIP>{}, m, lo, hi, sde, st, out_s);
+        else if (m.len > 0) advance_len(std::integral_constant<int, 0>{}, m, lo, hi, sde, st, out_s);
+    };
+
+    MarchStrip<T> R[RN];
+    T P[NS];
+    T sde0[STRIP];
+#pragma unroll
+    for (int j = 0; j < RN; j++) {
+        R[j] = MarchStrip<T>{};
+        R[j].len = 0;
+        R[j].r0 = e.y0;
+    }
+#pragma unroll
+    for (int m = 0; m < NS; m++) P[m] = T(0.0);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) sde0[k] = T(0.0);
+    const int i0 = s_first - NS, i1 = s_last + NS;
+    MarchRaw<T> nxt;
+    fetch(i0, nxt);
+    auto body = [&](auto ph, int i) {
+        constexpr int PH = decltype(ph)::value;
+        MarchStrip<T> &cur = R[PH];
+        prepare(i, nxt, cur);
+        T sde_cur[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde_cur[k] = lane0_value(nxt.sde[k]);
+        if (i < i1) fetch(i + 1, nxt);
+        T hi = cur.sd[0];
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+            MarchStrip<T> &h = R[(PH - 1 - m + 2 * RN) % RN];
+            const T last = h.sd[STRIP - 1];
+            const int s = i - 1 - m;
+            const int dist = s < s_first ? s_first - s : (s > s_last ? s - s_last : 0);
+            if (dist <= NS - 1 - m) advance(h, P[m], hi, sde0, m, out_lane && dist == 0);
+            P[m] = last;
+            hi = h.sd[0];
+        }
+        const MarchStrip<T> &f = R[(PH + 1) % RN];   // strip i-NS, finished
+        const int sf = i - NS;
+        const bool out = out_lane && sf >= s_first && sf <= s_last;
+        const unsigned b0 = off(f.r0);
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) {
+            const unsigned bk = out && col_in && k < f.len ? b0 + k * row_b : OOB;
+            b_uo.store(bk, f.sd[k]);
+            b_po.store(bk, f.rt[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) sde0[k] = sde_cur[k];
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    for (int i = i0;;) {
+        body(I0{}, i);
+        if (++i > i1) break;
+        body(I1{}, i);
+        if (++i > i1) break;
+        if constexpr (RN > 2) {
+            body(I2{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 3) {
+            body(I3{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 4) {
+            body(I4{}, i);
+            if (++i > i1) break;
+        }
+        if constexpr (RN > 5) {
+            body(I5{}, i);
+            if (++i > i1) break;
+        }
+    }
+}
+
 // k_ppcg_inner_ring with the face coefficients of the strips in flight in
 // LDS instead of registers (no reference counterpart; the same results bit
 // for bit).  A strip's Kx (4 rows), Ky (5 rows) and, in FP32, diagonal (4

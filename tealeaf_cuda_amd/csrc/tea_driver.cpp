@@ -97,6 +97,7 @@ struct Backend {
     void (*ppcg_end_begin_ur)(double, double, double *, double *, double *) = nullptr;
     void (*ppcg_end_begin_ur_inner)(double, double, int, double *, double *, double *) = nullptr;
     void (*ppcg_iteration)(int, double, double, int, double *, double *, double *) = nullptr;
+    void (*cheby_iterate_n)(int, int) = nullptr;
     int (*ppcg_inner_plan)(int) = nullptr;
     void (*cg_begin_ur)(int, double, double, double *, double *) = nullptr;
 
@@ -145,6 +146,7 @@ Backend load_backend(const char *path)
         dlsym(b.handle, "tea_engine_ppcg_end_begin_ur_inner"));
     b.ppcg_iteration = reinterpret_cast<void (*)(int, double, double, int, double *, double *, double *)>(
         dlsym(b.handle, "tea_engine_ppcg_iteration"));
+    b.cheby_iterate_n = reinterpret_cast<void (*)(int, int)>(dlsym(b.handle, "tea_engine_cheby_iterate_n"));
     b.cg_begin_ur =
         reinterpret_cast<void (*)(int, double, double, double *, double *)>(dlsym(b.handle, "tea_engine_cg_begin_ur"));
     return b;
@@ -921,6 +923,11 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                                  d.tl_ppcg_inner_steps > 0 && !(passes_env && passes_env[0] == '0') &&
                                  !(spec_env && spec_env[0] == '0');
     const bool queue_always = spec_env && spec_env[0] == '2';   // every time it may (tests the undo)
+    // Chebyshev iterations between norm checks in streaming passes
+    // (tea_engine_cheby_iterate_n) unless TEA_CHEBY_PASSES=0
+    const char *cheby_env = std::getenv("TEA_CHEBY_PASSES");
+    const bool cheby_batch = d.tl_use_chebyshev && b.cheby_iterate_n && !d.reflective_boundary &&
+                             !(cheby_env && cheby_env[0] == '0');
     // end not deferred and no residual replacement: the whole outer iteration
     // (begin_ur, the inner passes, end) in one call with one host wait
     const char *iter_env = std::getenv("TEA_PPCG_ITERATION");
@@ -1037,6 +1044,35 @@ Run::Solve Run::iterate(double initial_residual_sq, double eps)
                     est_itc = (int)std::lround(std::log(it_alpha) / (2.0 * std::log(gamm)));
                     log("    est itc\n%11d\n", est_itc);
                     cheby_calc_steps = 1;
+                } else if (cheby_batch) {
+                    // iterations n .. last in streaming passes: up to the next
+                    // norm check, tl_max_iters, the last coefficient, or the
+                    // iteration after which the loop's own test stops it (between
+                    // checks error only has its square root taken, below); the
+                    // skipped iterations' bookkeeping is replayed as the loop does it
+                    int last = n;
+                    double e = error;
+                    while (last - n + 1 < 100 && !(last >= est_itc && last % 10 == 0) && last < max_iters &&
+                           cheby_calc_steps + (last + 1 - n) <= max_cheby_iters) {
+                        const double e_after = std::sqrt(std::fabs(e));
+                        if (std::fabs(e_after) < eps * initial_residual) break;
+                        e = e_after;
+                        last++;
+                    }
+                    b.cheby_iterate_n(cheby_calc_steps, last - n + 1);
+                    b.check();
+                    for (; n < last; n++) {
+                        cheby_calc_steps = cheby_calc_steps + 1;
+                        update_halo(fields, 1);
+                        error = std::sqrt(std::fabs(error));
+                        if (d.verbose_on) log(" Residual %.16E\n", error);
+                        old_error = error;
+                    }
+                    if (n >= est_itc && n % 10 == 0) {
+                        int which = 1;
+                        b.tea_leaf_calc_2norm_kernel_cuda_(&which, &error);
+                        b.check();
+                    }
                 } else {
                     int st = cheby_calc_steps;
                     b.tea_leaf_cheby_iterate_kernel_cuda_(&st);

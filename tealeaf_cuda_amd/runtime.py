@@ -120,6 +120,7 @@ _EXT = {
     "tea_engine_ppcg_iteration": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)], None),
+    "tea_engine_cheby_iterate_n": ([ctypes.c_int, ctypes.c_int], None),
     "tea_engine_ppcg_inner_plan": ([ctypes.c_int], ctypes.c_int),
     "tea_engine_cg_begin_ur": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)], None),
@@ -278,6 +279,9 @@ class Engine:
 
     def cheby_iterate(self, step):
         self._call("tea_leaf_cheby_iterate_kernel_cuda_", _ip(step))
+
+    def cheby_iterate_n(self, step: int, nsteps: int):
+        self._call("tea_engine_cheby_iterate_n", int(step), int(nsteps))
 
     # -- Jacobi
     def jacobi_solve(self):

@@ -108,7 +108,7 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
                                     "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY",
-                                    "TEA_PPCG_ITERATION", "TEA_FUSED_END"])
+                                    "TEA_PPCG_ITERATION", "TEA_FUSED_END", "TEA_CHEBY_PASSES"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
                                   "cg_block_ragged", "cheby_none", "mixed_cheby_diag"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
@@ -123,8 +123,9 @@ def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracl
     calc_p / calc_u kernels (TEA_FUSED_CHEBY=0), and PPCG outer iterations
     through begin_ur, the passes and end as separate calls instead of one
     (TEA_PPCG_ITERATION=0), and the end of an outer iteration as its own
-    launch instead of in the last inner pass (TEA_FUSED_END=0), on the GPU
-    give the same results as the oracle."""
+    launch instead of in the last inner pass (TEA_FUSED_END=0), and
+    Chebyshev iterations one call each instead of in runs of streaming passes
+    (TEA_CHEBY_PASSES=0), on the GPU give the same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -275,6 +276,7 @@ PPCG_CALLS = {
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_inner_n(7 - n, n))))(n) for n in range(1, 6)},
 }
+CH_A7, CH_B7 = [0.2, 0.4, 0.6, 0.3, 0.5, 0.7, 0.25], [0.9, 0.8, 0.7, 0.85, 0.75, 0.65, 0.95]
 CHEBY_CALLS = {
     "cheby_init": lambda e: e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3),
     "cheby_iterate": lambda e: (e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3), e.cheby_iterate(3)),
@@ -285,6 +287,12 @@ CHEBY_CALLS = {
                                              e.calc_2norm(1), e.cheby_iterate(2)),
     "cheby_iterate_halo_iterate": lambda e: (e.cheby_init([0.2, 0.4, 0.6], [0.9, 0.8, 0.7], 1.3), e.cheby_iterate(1),
                                              e.update_halo([0, 0, 0, 1, 0, 0, 0], 2), e.cheby_iterate(2)),
+    # runs of iterations in streaming passes (tea_engine_cheby_iterate_n): one pass, several with a norm
+    # after them, and after a fused single iteration
+    "cheby_iterate_n3": lambda e: (e.cheby_init(CH_A7, CH_B7, 1.3), e.cheby_iterate_n(1, 3)),
+    "cheby_iterate_n7_norm": lambda e: (e.cheby_init(CH_A7, CH_B7, 1.3), e.cheby_iterate_n(1, 7), e.calc_2norm(1)),
+    "cheby_iterate_then_n5": lambda e: (e.cheby_init(CH_A7, CH_B7, 1.3), e.cheby_iterate(1),
+                                        e.cheby_iterate_n(2, 5), e.cheby_iterate(7)),
 }
 
 

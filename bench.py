@@ -80,11 +80,16 @@ COMPULSORY_BYTES = {
     **{f"k_ppcg_inner{n}": 64 for n in range(1, 6)},
     # the last pass of an FP64 outer iteration also reads r and rstore for the end's r.z
     **{f"k_ppcg_inner{n}_end": 80 for n in range(1, 6)},
+    # the first pass of an outer iteration with sd formed from r: r (as rtemp), Kx, Ky read;
+    # sd, rtemp, utemp written
+    **{f"k_ppcg_inner{n}_from_r_f32": 24 for n in range(1, 6)},
+    **{f"k_ppcg_inner{n}_from_r": 48 for n in range(1, 6)},
     "k_ppcg_inner_fused_f32": 32, "k_ppcg_inner_fused": 64,
     # fused begin: p, z (utemp), Kx, Ky, r read; p', r_store written (w formed again by ur)
     "ppcg_begin_f32": 28, "ppcg_begin": 56,
-    # fused ur + init_sd_new: u, p', r, Kx, Ky read; u, r, z, sd written (rtemp, utemp left to the pass)
-    "k_ppcg_ur_sd_f32": 36, "k_ppcg_ur_sd": 72,
+    # fused ur + init_sd_new: u, p', r, Kx, Ky read; u, r written (z and sd left to the first pass,
+    # which forms sd from r; after a residual replacement ur also writes z and sd, 8 or 16 more)
+    "k_ppcg_ur_sd_f32": 28, "k_ppcg_ur_sd": 56,
     # fused end: utemp, r, r_store read (z owed to the next begin)
     "ppcg_end_f32": 12, "ppcg_end": 24,
     # CG iteration kernels: w = Ap (p, Kx, Ky read; w written); u/r/z update (u, p, r, w, Kx, Ky read;

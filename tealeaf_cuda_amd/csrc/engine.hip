@@ -166,6 +166,11 @@ struct Chunk {
     // z = utemp owed by a separate ppcg_end with a preconditioner (the next
     // fused begin reads utemp in its place and ur rewrites z), 32 or 64, or 0
     int pending_z = 0;
+    // z = M^-1 r and sd = z / sd_r_theta (block Jacobi) owed by a fused ur
+    // that left them to the first inner pass, which forms sd from r (32 or
+    // 64, 0: none); k_minv_r writes them out for any other reader
+    int pending_zr = 0, pending_sd_r = 0;
+    double sd_r_theta = 0.0;
     // w and r of the u held in the spare u buffer owed by a fused Chebyshev
     // iteration (32 or 64: the precision of the fields, 0: none)
     int pending_cheby = 0;
@@ -1485,9 +1490,30 @@ template <typename T> void settle_fields()
     launch(nm<T>("settle_utemp"), k_copy<T>, in, pitch_of<T>(), FT<T>(X_UTEMP), (const T *)FT<T>(F_SD));
 }
 
-// z = utemp owed (only)
+// z = M^-1 r (z) and sd = z / theta (sd) owed by a fused ur, written out
+void settle_minv_r(bool z, bool sd)
+{
+    const int zb = z ? g.pending_zr : 0, sb = sd ? g.pending_sd_r : 0;
+    if (z) g.pending_zr = 0;
+    if (sd) g.pending_sd_r = 0;
+    const int bits = zb ? zb : sb;
+    if (bits == 32)
+        launch("settle_minv_r_f32", k_minv_r<float>, interior(), g.pitch32, (const float *)g.fs32[F_R],
+               zb ? g.fs32[X_Z] : (float *)nullptr, sb ? g.fs32[F_SD] : (float *)nullptr, g.sd_r_theta,
+               (const float *)g.fs32[X_KX], (const float *)g.fs32[X_KY]);
+    else if (bits == 64)
+        launch("settle_minv_r", k_minv_r<double>, interior(), g.pitch, (const double *)g.fs[F_R],
+               zb ? g.fs[X_Z] : (double *)nullptr, sb ? g.fs[F_SD] : (double *)nullptr, g.sd_r_theta,
+               (const double *)g.fs[X_KX], (const double *)g.fs[X_KY]);
+}
+
+// z = M^-1 r owed (only)
+void settle_z_r() { settle_minv_r(true, false); }
+
+// z owed (only): z = utemp, or z = M^-1 r
 void settle_z()
 {
+    settle_minv_r(true, false);
     const int bits = g.pending_z;
     g.pending_z = 0;
     if (bits == 32)
@@ -1496,9 +1522,10 @@ void settle_z()
         launch("settle_z", k_copy<double>, interior(), g.pitch, g.fs[X_Z], (const double *)g.fs[X_UTEMP]);
 }
 
-// rtemp and utemp owed (only)
+// rtemp and utemp owed (only; an sd they copy that is itself owed first)
 void settle_rt_ut()
 {
+    if (g.pending_rt_ut) settle_minv_r(false, true);
     const int bits = g.pending_rt_ut;
     g.pending_rt_ut = 0;
     if (bits == 32) settle_fields<float>();
@@ -1507,6 +1534,7 @@ void settle_rt_ut()
 
 void settle_pending()
 {
+    settle_minv_r(true, true);   // one launch for z and sd owed together
     settle_z();   // before the rtemp/utemp debt rewrites utemp
     settle_rt_ut();
     const int cbits = g.pending_cheby;
@@ -1700,6 +1728,11 @@ int cu_count()
     return v;
 }
 
+// The passes that can form sd = M^-1 r / theta from r themselves
+// (k_ppcg_inner_ringl FROM_R): the first pass of ten block-Jacobi inner steps
+// (5 + 5 in FP32, 4 + 3 + 3 in FP64).
+template <typename T, int NS> constexpr bool has_from_r() { return sizeof(T) == 4 ? NS == 5 : NS == 4; }
+
 // One wave per band of columns and chunk of strips: two waves per SIMD in
 // FP32 (its kernels fit 256 registers up to five steps), one in FP64 (the
 // longer chunks that leaves measured faster than two waves per SIMD even
@@ -1707,11 +1740,25 @@ int cu_count()
 // With r_end (and rs_end) given and a ring form with a preconditioner
 // running, the pass also forms k_ppcg_end's (r - rstore).z over the utemp it
 // writes (z owed, not stored) into the reduction partials, and returns true.
+// sd owed by a fused ur (with rtemp and utemp): the LDS ring pass forms it
+// from r where it can (FROM_R), else it is written out first.
 template <typename T, int NS>
-bool launch_inner_march(const Ext &e, long pt, const StepCoefs &cf, const char *name, const T *r_end = nullptr,
+bool launch_inner_march(const Ext &e, long pt, const StepCoefs &cf_in, const char *name, const T *r_end = nullptr,
                         const T *rs_end = nullptr)
 {
-    constexpr int OX = march_out_x(NS);
+    const char *ring_env = getenv("TEA_INNER_RING");
+    const int forced = ring_env && ring_env[0] >= '0' && ring_env[0] <= '2' ? ring_env[0] - '0' : -1;
+    const int bits = 8 * (int)sizeof(T);
+    bool from_r = false;
+    if (g.pending_sd_r == bits) {
+        from_r = has_from_r<T, NS>() && g.prec == PREC_BLOCK && g.pending_rt_ut == bits && !(r_end && rs_end) &&
+                 (forced >= 0 ? forced : inner_form<T, PREC_BLOCK, NS>()) == 2;
+        if (!from_r) settle_minv_r(false, true);
+        g.pending_sd_r = 0;
+    }
+    StepCoefs cf = cf_in;
+    cf.theta = g.sd_r_theta;
+    const int OX = march_out_x(NS, from_r);
     const int nbands = (e.x1 - e.x0 + OX) / OX, nstrips = (e.y1 - e.y0 + STRIP) / STRIP;
     if (nbands < 1 || nstrips < 1) return false;
     const int per_simd = sizeof(T) == 4 ? 2 : 1;
@@ -1738,19 +1785,27 @@ bool launch_inner_march(const Ext &e, long pt, const StepCoefs &cf, const char *
     // the instances with the end's reduction are timed under their own name
     static const char *end_names[MAX_PASS_STEPS] = {"k_ppcg_inner1_end", "k_ppcg_inner2_end", "k_ppcg_inner3_end",
                                                     "k_ppcg_inner4_end", "k_ppcg_inner5_end"};
+    static const char *from_r_names[MAX_PASS_STEPS] = {"k_ppcg_inner1_from_r", "k_ppcg_inner2_from_r",
+                                                       "k_ppcg_inner3_from_r", "k_ppcg_inner4_from_r",
+                                                       "k_ppcg_inner5_from_r"};
     auto go_ring = [&](auto kernel, const T *re, const T *rse) {
         g.last_grid = (int)grid;
-        klaunch(re ? nm<T>(end_names[NS - 1]) : name, kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk, (const T *)FT<T>(F_SD),
+        const char *nmr = re ? nm<T>(end_names[NS - 1]) : from_r ? nm<T>(from_r_names[NS - 1]) : name;
+        klaunch(nmr, kernel, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pt, chunk, (const T *)FT<T>(F_SD),
                 FT<T>(X_SD2), rt_in, FT<T>(X_RT2), ut_in, FT<T>(X_UTEMP), (const T *)FT<T>(X_MI),
                 (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), cf, re, rse, red_ctx());
     };
-    const char *ring_env = getenv("TEA_INNER_RING");
-    const int forced = ring_env && ring_env[0] >= '0' && ring_env[0] <= '2' ? ring_env[0] - '0' : -1;
     const bool want_end = r_end && rs_end && (int)grid <= g.max_blocks;
     bool end_done = false;
     auto pick = [&](auto prec_c) {
         constexpr int P = decltype(prec_c)::value;
         const int form = forced >= 0 ? forced : inner_form<T, P, NS>();
+        if constexpr (P == PREC_BLOCK && has_from_r<T, NS>()) {
+            if (from_r) {   // chosen above: form 2, no end
+                go_ring(k_ppcg_inner_ringl<T, P, NS, false, true>, nullptr, nullptr);
+                return;
+            }
+        }
         if constexpr (P != PREC_NONE) {
             if (want_end && form == 2) {
                 go_ring(k_ppcg_inner_ringl<T, P, NS, true>, r_end, rs_end);
@@ -1913,6 +1968,16 @@ template <typename T> bool w_from_p()
     return g.prec == PREC_BLOCK && !(s && s[0] == '0');
 }
 
+// Whether a fused ur leaves z and sd to the first inner pass (block Jacobi,
+// w formed from p, rtemp and utemp deferred to a streaming pass) unless
+// TEA_SD_FROM_R=0: the pass forms sd from r if its form can (else the
+// values are written out first).
+template <typename T> bool sd_from_r()
+{
+    const char *s = getenv("TEA_SD_FROM_R");
+    return g.prec == PREC_BLOCK && w_from_p<T>() && use_march<T, 1>() && !(s && s[0] == '0');
+}
+
 // The streaming form of the fused begin (k_begin_march) unless
 // TEA_BEGIN_MARCH=0: bands of 62 columns, runs of rows sized for about eight
 // waves per SIMD.  Returns false (nothing launched) when it does not apply.
@@ -1987,18 +2052,34 @@ template <typename T> void ppcg_begin_impl(int with_p, double beta, double *out)
 // defer: leave rtemp = r and utemp = sd to the next streaming inner pass
 // (recorded in g.pending_rt_ut).
 // w_from_p: w formed from p (block Jacobi; begin left it unwritten).
+// from_r (with defer and w_from_p): z = M^-1 r and sd = z / theta are not
+// stored either (g.pending_zr, g.pending_sd_r): the first inner pass forms sd
+// from r, and z is written out only for a reader before the end replaces it.
 template <typename T>
-bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer = false, bool w_from_p = false)
+bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer = false, bool w_from_p = false,
+                    bool from_r = false)
 {
     T *rt = defer ? nullptr : FT<T>(X_RTEMP), *ut = defer ? nullptr : FT<T>(X_UTEMP);
+    from_r = from_r && defer && w_from_p && g.prec == PREC_BLOCK;
+    T *zf = from_r ? nullptr : FT<T>(X_Z), *sdf = from_r ? nullptr : FT<T>(F_SD);
+    // z and sd as this ur leaves them (owed or stored): any older debt is void
+    const int bits = 8 * (int)sizeof(T);
+    g.pending_zr = g.pending_sd_r = 0;
+    auto owe = [&](bool ok) {
+        if (ok && from_r) {
+            g.pending_zr = g.pending_sd_r = bits;
+            g.sd_r_theta = theta;
+        }
+        return ok;
+    };
     bool ok;
     if constexpr (std::is_same<T, float>::value) {
         if (w_from_p && g.prec == PREC_BLOCK) {
             ok = launch_x2_capped("k_ppcg_ur_sd_f32", 4096, k_ppcg_ur_sd_x2_wp, interior(), pitch_of<T>(), alpha,
-                                  pw_rro, theta, FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), FT<T>(X_Z),
-                                  (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), FT<T>(F_SD), rt, ut, red_ctx());
+                                  pw_rro, theta, FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), zf,
+                                  (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), sdf, rt, ut, red_ctx());
             if (ok && defer) g.pending_rt_ut = 32;
-            return ok;
+            return owe(ok);
         }
         auto go = [&](auto kernel) {
             return launch_x2("k_ppcg_ur_sd_f32", kernel, interior(), pitch_of<T>(), alpha, pw_rro, theta, FT<T>(F_U),
@@ -2010,9 +2091,9 @@ bool ppcg_ur_launch(double alpha, const double *pw_rro, double theta, bool defer
              : g.prec == PREC_DIAG ? go(k_ppcg_ur_sd_x2<PREC_DIAG>)
                                    : go(k_ppcg_ur_sd_x2<PREC_NONE>);
     } else if (w_from_p && g.prec == PREC_BLOCK) {
-        ok = launch("k_ppcg_ur_sd", k_ppcg_ur_sd_wp<T>, interior(), pitch_of<T>(), alpha, pw_rro, theta, FT<T>(F_U),
-                    (const T *)FT<T>(F_P), FT<T>(F_R), FT<T>(X_Z), (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY),
-                    FT<T>(F_SD), rt, ut, red_ctx());
+        ok = owe(launch("k_ppcg_ur_sd", k_ppcg_ur_sd_wp<T>, interior(), pitch_of<T>(), alpha, pw_rro, theta,
+                        FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), zf, (const T *)FT<T>(X_KX),
+                        (const T *)FT<T>(X_KY), sdf, rt, ut, red_ctx()));
     } else {
         ok = PREC_SWITCH_T(T, g.prec, "k_ppcg_ur_sd", k_ppcg_ur_sd, interior(), pitch_of<T>(), alpha, pw_rro, theta,
                            FT<T>(F_U), (const T *)FT<T>(F_P), FT<T>(F_R), (const T *)FT<T>(X_W), FT<T>(X_Z),
@@ -2033,9 +2114,11 @@ template <typename T> void ppcg_ur_impl(double alpha, double theta, double *rrn)
 template <typename T> void ppcg_begin_ur_impl(int with_p, double beta, double theta, double *pw_rro, double *rrn)
 {
     const bool wp = w_from_p<T>();
+    settle_z_r();   // begin reads z
     const T *z_src = g.pending_z == 8 * (int)sizeof(T) ? (const T *)FT<T>(X_UTEMP) : nullptr;
     if (!ppcg_begin_launch<T>(with_p, beta, nullptr, z_src, wp) || !finish(0, 2, g.d_scal)) return;
-    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp, sd_from_r<T>()) || !finish(2, 1, nullptr))
+        return;
     g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
     g.pending_z = 0;
     if (collect(pw_rro, 0, 2)) *rrn = g.h_out[2];
@@ -2065,11 +2148,13 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
         return;
     }
     g.end_slot = -1;
+    g.pending_zr = 0;   // the end's z = utemp replaces z (begin reads utemp)
     const bool wp = w_from_p<T>();
     if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + eslot, z_from_utemp ? ut : nullptr, wp) ||
         !finish(0, 2, g.d_scal))
         return;
-    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp, sd_from_r<T>()) || !finish(2, 1, nullptr))
+        return;
     g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
     g.pending_z = 0;   // end recomputed what z owed; ur rewrote z
     if (inner_steps > 0) {
@@ -2104,6 +2189,7 @@ template <typename T> void ppcg_end_impl(double *rrn)
                        owe_z ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R), (const T *)FT<T>(X_RSTORE), red_ctx()))
         return;
     g.pending_z = owe_z ? 8 * (int)sizeof(T) : 0;
+    g.pending_zr = 0;   // z = utemp replaces z = M^-1 r
     fetch(rrn, 1);
 }
 
@@ -2114,9 +2200,11 @@ void ppcg_iteration_impl(int with_p, double beta, double theta, int inner_steps,
                          double *rrn_end)
 {
     const bool wp = w_from_p<T>();
+    settle_z_r();   // begin reads z
     const T *z_src = g.pending_z == 8 * (int)sizeof(T) ? (const T *)FT<T>(X_UTEMP) : nullptr;
     if (!ppcg_begin_launch<T>(with_p, beta, nullptr, z_src, wp) || !finish(0, 2, g.d_scal)) return;
-    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp) || !finish(2, 1, nullptr)) return;
+    if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp, sd_from_r<T>()) || !finish(2, 1, nullptr))
+        return;
     g.pending_w = wp ? 8 * (int)sizeof(T) : 0;
     g.pending_z = 0;
     const bool owe_z = g.prec != PREC_NONE;
@@ -2133,6 +2221,7 @@ void ppcg_iteration_impl(int with_p, double beta, double theta, int inner_steps,
         return;
     if (!finish(3, 1, nullptr)) return;
     g.pending_z = owe_z ? 8 * (int)sizeof(T) : 0;
+    g.pending_zr = 0;   // z = utemp replaces z = M^-1 r
     if (!collect(pw_rro, 0, 2)) return;
     *rrn_ur = g.h_out[2];
     *rrn_end = g.h_out[3];
@@ -2466,6 +2555,7 @@ EXPORT void tea_leaf_ppcg_init_kernel_cuda_(const int *step, double *rro)
 // Both overwrite sd, rtemp and utemp: rtemp and utemp still owed need no writing.
 static void drop_or_settle_pending()
 {
+    g.pending_sd_r = 0;   // sd is rewritten
     settle_z();   // init_sd_new reads z
     if (g.pending_rt_ut == (g.fp32 ? 32 : 64)) g.pending_rt_ut = 0;
     else settle_rt_ut();
@@ -2798,6 +2888,7 @@ EXPORT void tea_engine_mp_replace(double theta, double *rr64, double *rr32)
     CHECK_INIT_KEEP_PENDING();
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     if (!g.fp32 || !g.slab32) { die("tea_engine_mp_replace: call from the FP32 solver mode"); return; }
+    g.pending_zr = 0;   // z (a block-Jacobi debt) is rewritten with r
     drop_or_settle_pending();
     const Ext in = interior();
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;

@@ -97,6 +97,33 @@ __device__ __forceinline__ void precond_strip(const T (&rv)[STRIP], T (&zv)[STRI
     }
 }
 
+// z = M^-1 r (block Jacobi) and sd = z / theta, each where its pointer is not
+// null: the values a fused ur (k_ppcg_ur_sd_wp, k_ppcg_ur_sd_x2_wp) forms and
+// leaves unstored when the first inner pass forms them itself (no reference
+// counterpart as a kernel; the same expressions, so the same values).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_minv_r(Ext e, long pitch, const T *__restrict__ r, T *__restrict__ z,
+                                                  T *__restrict__ sd, double theta_in, const T *__restrict__ Kx,
+                                                  const T *__restrict__ Ky)
+{
+    const T theta_r = T(1.0) / (T)theta_in;
+    STRIP_BEGIN
+    T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) rv[k] = r[i0 + k * pitch];
+    precond_strip<T, PREC_BLOCK>(rv, zv, s.len, i0, pitch, (const T *)nullptr, Kx, Ky);
+#pragma unroll
+    for (int k = 0; k < STRIP; k++) {
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            if (z) z[i] = zv[k];
+            if (sd) sd[i] = zv[k] * theta_r;
+        }
+    }
+    STRIP_END
+}
+
 // ---- CG ---------------------------------------------------------------------------
 
 // tea_leaf_cg.cuknl:5-39 (+ block_solve for jac_block): z = M^-1 r, p = z or r, sum r*p
@@ -471,6 +498,7 @@ __device__ __forceinline__ void strip_apply_rd(const float (&rhs)[STRIP], float 
 constexpr int MAX_PASS_STEPS = 5;
 struct StepCoefs {
     double a[MAX_PASS_STEPS], b[MAX_PASS_STEPS];
+    double theta;   // passes that form sd = M^-1 r / theta from r (FROM_R)
 };
 
 // Tile of a pass of NS inner steps with NW waves per workgroup: HL halo
@@ -862,9 +890,13 @@ template <typename T> struct MarchStrip {
 };
 
 constexpr int MARCH_WAVES = 4;   // independent waves per workgroup
-// Columns a streaming pass of ns steps writes per band of WAVE lanes (ns - 1
-// halo lanes at the left, ns at the right).
-constexpr int march_out_x(int ns) { return WAVE - 2 * ns + 1; }
+// Columns a streaming pass of ns steps writes per band of WAVE lanes: ns - 1
+// halo lanes at the left (lane 0 reads its left neighbour's step-0 sd from
+// memory) and ns at the right (the last lane lacks its right neighbour's Kx,
+// so its values are stale from the first step on).  Where the pass forms sd
+// from r itself (from_r), one more on each side: lane 0 has no neighbour's sd
+// to read, and the last lane's own sd is formed with the missing Kx.
+constexpr int march_out_x(int ns, bool from_r = false) { return WAVE - 2 * ns + 1 - (from_r ? 2 : 0); }
 
 // Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, VALU moves with
 // no LDS round trip): lane l receives v from lane l-1 (from_below) or l+1
@@ -1565,7 +1597,12 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_cheby_ring(
 // waves per SIMD, and saves the diagonal's four additions per cell and step
 // (FP32 five-step pass 164.5 -> 159.1 us at 4096^2).  Each wave owns its part of LDS; its writes and reads
 // are in program order, so no barrier is needed.
-template <typename T, int PREC, int NS, bool END = false>
+// FROM_R (block Jacobi): the pass starts from sd = M^-1 r / theta, rtemp = r
+// and utemp = sd, as a fused ur (k_ppcg_ur_sd_x2_wp, k_ppcg_ur_sd_wp) forms
+// them, formed here from r (read as rtemp) when each strip enters, by the
+// expressions of the inner steps' z = M^-1 rtemp: that ur then stores
+// neither sd nor z.  Only sd's halo ring is read.
+template <typename T, int PREC, int NS, bool END = false, bool FROM_R = false>
 __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     Ext e, long pitch, int chunk, const T *__restrict__ sd_in, T *__restrict__ sd_out, const T *__restrict__ rt_in,
     T *__restrict__ rt_out, const T *ut_in, T *ut_out, const T *__restrict__ Mi, const T *__restrict__ Kx,
@@ -1573,7 +1610,8 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
     const T *__restrict__ rs_end, Red red)
 {
     static_assert(NS >= 1 && NS <= MAX_PASS_STEPS, "1..5 steps per pass");
-    constexpr int HL = NS - 1, HR = NS, OUT_X = march_out_x(NS), RN = NS + 1;
+    static_assert(!FROM_R || PREC == PREC_BLOCK, "sd formed from r with block Jacobi");
+    constexpr int HL = FROM_R ? NS : NS - 1, HR = FROM_R ? NS + 1 : NS, OUT_X = march_out_x(NS, FROM_R), RN = NS + 1;
     // LDS rows per slot: Kx 0..3, Ky 0..4 and, in FP32, the diagonal 0..3 (in FP64 the
     // diagonal is formed again at each step: measured faster there)
     constexpr bool CACHE_DIAG = sizeof(T) == 4;
@@ -1628,7 +1666,9 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
 #pragma unroll
         for (int k = 0; k < STRIP; k++) {
             const int rr = r0 + k;
-            m.sd[k] = b_sd.load((col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1) ? b0 + k * row_b : OOB);
+            // FROM_R: the cells inside the extent (k < act) take sd formed from r
+            const bool rd_sd = col_mem && rr >= e.y0 - 1 && rr <= e.y1 + 1 && !(FROM_R && k < act);
+            m.sd[k] = b_sd.load(rd_sd ? b0 + k * row_b : OOB);
         }
 #pragma unroll
         for (int k = 0; k < STRIP + 1; k++) m.ky[k] = b_ky.load((k <= act && act > 0) ? b0 + k * row_b : OOB);
@@ -1638,13 +1678,14 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
             const unsigned bk = b0 + k * row_b;
             m.kx[k] = b_kx.load(col_mem && k < len ? bk : OOB);
             m.rt[k] = b_rt.load(a ? bk : OOB);
-            m.ut[k] = b_ut.load(a && out_lane ? bk : OOB);
+            m.ut[k] = b_ut.load(!FROM_R && a && out_lane ? bk : OOB);
             m.mi[k] = PREC == PREC_DIAG ? b_mi.load(a ? bk : OOB) : T(0.0);
-            m.sde[k] = b_sd.load(a && lane == 0 ? bk - W : OOB);
+            m.sde[k] = b_sd.load(!FROM_R && a && lane == 0 ? bk - W : OOB);
         }
     };
     // Forms ring slot j from strip s's reads: the fields into registers, the
-    // coefficients into the slot's LDS rows, the block-Jacobi factors.
+    // coefficients into the slot's LDS rows, the block-Jacobi factors (FROM_R:
+    // and sd = utemp = M^-1 r / theta inside the extent).
     auto prepare = [&](int s, const MarchRaw<T> &raw, Slot &m, T (*Lj)[LW]) {
         m.r0 = e.y0 + s * STRIP;
         m.len = strip_len(m.r0);
@@ -1671,6 +1712,19 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
             if (m.len == STRIP) strip_factor<T>(STRIP, diag, raw.ky, m.cp, m.bfp);
             else strip_factor<T>(m.len, diag, raw.ky, m.cp, m.bfp);
             if constexpr (sizeof(T) == 4) m.rd = 1.0 / (double)diag[0];
+            if constexpr (FROM_R) {
+                const T theta_r = T(1.0) / (T)cf.theta;
+                T zv[STRIP];
+                if constexpr (sizeof(T) == 4) strip_apply_rd(m.rt, zv, m.len, m.rd, raw.ky, m.cp, m.bfp);
+                else strip_apply<T>(m.rt, zv, m.len, diag[0], raw.ky, m.cp, m.bfp);
+#pragma unroll
+                for (int k = 0; k < STRIP; k++) {
+                    if (col_in && k < m.len) {
+                        m.sd[k] = zv[k] * theta_r;
+                        m.ut[k] = m.sd[k];
+                    }
+                }
+            }
         }
     };
     // Moves the strip in slot m (coefficients in Lj) from step st to st+1 in
@@ -2352,16 +2406,17 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_x2_wp(Ext e, long pitch, d
                 for (int q = 0; q < 2; q++) sdv[q] = zv[q][k] * theta_r;
                 const int nq = k < act[1] ? 2 : 1;
                 for (int q = 0; q < nq; q++) dd_add_prod(acc[0], (double)rv[q][k], (double)zv[q][k]);
+                // z, sd null: both left to the first inner pass, which forms them from r
                 if (nq == 2) {
-                    *(float2 *)(z + i) = make_float2(zv[0][k], zv[1][k]);
-                    *(float2 *)(sd + i) = make_float2(sdv[0], sdv[1]);
+                    if (z) *(float2 *)(z + i) = make_float2(zv[0][k], zv[1][k]);
+                    if (sd) *(float2 *)(sd + i) = make_float2(sdv[0], sdv[1]);
                     if (rtemp) {   // null: rtemp = r and utemp = sd left to the first inner pass
                         *(float2 *)(rtemp + i) = make_float2(rv[0][k], rv[1][k]);
                         *(float2 *)(utemp + i) = make_float2(sdv[0], sdv[1]);
                     }
                 } else {
-                    z[i] = zv[0][k];
-                    sd[i] = sdv[0];
+                    if (z) z[i] = zv[0][k];
+                    if (sd) sd[i] = sdv[0];
                     if (rtemp) {
                         rtemp[i] = rv[0][k];
                         utemp[i] = sdv[0];
@@ -2422,10 +2477,10 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_wp(Ext e, long pitch, doub
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
             const long i = i0 + k * pitch;
-            z[i] = zv[k];
+            if (z) z[i] = zv[k];   // z, sd null: left to the first inner pass, which forms them from r
             dd_add_prod(acc[0], (double)rv[k], (double)zv[k]);
             const T sdv = zv[k] * theta_r;
-            sd[i] = sdv;
+            if (sd) sd[i] = sdv;
             if (rtemp) {   // null: rtemp = r and utemp = sd left to the first inner pass
                 rtemp[i] = rv[k];
                 utemp[i] = sdv;

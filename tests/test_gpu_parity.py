@@ -108,7 +108,7 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
                                     "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY",
-                                    "TEA_PPCG_ITERATION", "TEA_FUSED_END", "TEA_CHEBY_PASSES"])
+                                    "TEA_PPCG_ITERATION", "TEA_FUSED_END", "TEA_CHEBY_PASSES", "TEA_SD_FROM_R"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
                                   "cg_block_ragged", "cheby_none", "mixed_cheby_diag"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
@@ -125,7 +125,9 @@ def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracl
     (TEA_PPCG_ITERATION=0), and the end of an outer iteration as its own
     launch instead of in the last inner pass (TEA_FUSED_END=0), and
     Chebyshev iterations one call each instead of in runs of streaming passes
-    (TEA_CHEBY_PASSES=0), on the GPU give the same results as the oracle."""
+    (TEA_CHEBY_PASSES=0), and a fused ur that stores z and sd itself instead of
+    leaving sd to the first inner pass (TEA_SD_FROM_R=0), on the GPU give the
+    same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -214,6 +216,8 @@ def _tile_kernels(fn):
     return call
 
 
+A10 = [0.3, 0.5, 0.7, 0.2, 0.6, 0.4, 0.35, 0.55, 0.25, 0.45]
+B10 = [1.1, 0.9, 0.8, 1.2, 0.7, 1.0, 0.95, 0.85, 1.05, 0.75]
 PPCG_CALLS = {
     "ppcg_init_1": lambda e: e.ppcg_init(1),
     "ppcg_init_2": lambda e: e.ppcg_init(2),
@@ -254,6 +258,27 @@ PPCG_CALLS = {
     "fused_ppcg_inner_n4_then_end_begin_ur": lambda e: (
         e.ppcg_init_constants([0.3, 0.5, 0.7, 0.2, 0.6, 0.4], [1.1, 0.9, 0.8, 1.2, 0.7, 1.0]),
         e.ppcg_begin_ur(False, 0.0, 1.7), e.ppcg_inner_n(1, 4), e.ppcg_end_begin_ur(0.8, 1.7)),
+    # ten block-Jacobi steps: the first pass forms sd from r (ur stores neither
+    # z nor sd); z = M^-1 r stays owed until the end replaces it or a reader
+    # (the norm of r.z, the field comparison) has it written out
+    "fused_ppcg_end_begin_ur_inner10_twice": lambda e: (
+        e.ppcg_init_constants(A10, B10), e.ppcg_end_begin_ur_inner(0.8, 1.7, 10),
+        e.ppcg_end_begin_ur_inner(0.6, 1.7, 10), e.ppcg_end_begin_ur(0.7, 1.7)),
+    "fused_ppcg_end_begin_ur_inner10_norm_between": lambda e: (
+        e.ppcg_init_constants(A10, B10), e.ppcg_end_begin_ur_inner(0.8, 1.7, 10), e.calc_2norm(2),
+        e.ppcg_end_begin_ur_inner(0.6, 1.7, 10)),
+    "fused_ppcg_end_begin_ur_inner10": lambda e: (
+        e.ppcg_init_constants(A10, B10), e.ppcg_end_begin_ur_inner(0.8, 1.7, 10)),
+    "fused_ppcg_iteration10_twice": lambda e: (
+        e.ppcg_init_constants(A10, B10), e.ppcg_iteration(False, 0.0, 1.7, 10), e.ppcg_iteration(True, 0.45, 1.7, 10)),
+    "fused_ppcg_begin_ur_then_inner_n4_n3_n3_end": lambda e: (
+        e.ppcg_init_constants(A10, B10), e.ppcg_begin_ur(True, 0.45, 1.7), e.ppcg_inner_n(1, 4),
+        e.ppcg_inner_n(5, 3), e.ppcg_inner_n(8, 3), e.ppcg_end()),
+    "fused_ppcg_begin_ur_then_inner_n5_n5_end": lambda e: (
+        e.ppcg_init_constants(A10, B10), e.ppcg_begin_ur(True, 0.45, 1.7), e.ppcg_inner_n(1, 5),
+        e.ppcg_inner_n(6, 5), e.ppcg_end()),
+    "fused_ppcg_begin_ur_then_begin_ur": lambda e: (
+        e.ppcg_begin_ur(False, 0.0, 1.7), e.ppcg_begin_ur(True, 0.45, 1.7)),
     "fused_ppcg_inner2": lambda e: (e.ppcg_init_constants([0.3, 0.5, 0.7], [1.1, 0.9, 0.8]), e.ppcg_inner2(2)),
     # a whole outer iteration in one call, twice (the second with p owed), and with no inner steps
     "fused_ppcg_iteration_twice": lambda e: (

@@ -269,6 +269,169 @@ __global__ __launch_bounds__(ROWS * 64) void k_conv_out(const __bf16 *__restrict
     }
 }
 
+// The 32-channel-input layers (k_conv32 forward, k_conv_out) as a march down a
+// run of run_rows rows of one 64-pixel band (inference): the workgroup keeps
+// the ROWS + 2 input rows of the current block of ROWS output rows in an LDS
+// ring (row y in slot (y + 1) % (ROWS + 2)) and, while the matrix cores work
+// on that block, has the next block's ROWS new rows in flight into registers;
+// they replace the rows no later block needs once every wave is done.  So
+// each input row is read once per run (a window per block re-reads 2 of every
+// ROWS + 2), and the loads of a block overlap the products of the one before.
+// The operands and their order are those of the windowed kernels, so the
+// values are the same.  Grid (W/64, H/run_rows, B); run_rows a multiple of ROWS.
+template <int ROWS> struct MarchRing {
+    static constexpr int NR = ROWS + 2;
+    static constexpr int PIECES = WIN_X * 4;                    // 16-byte pieces per staged row
+    static constexpr int NEW = ROWS * PIECES;                   // pieces per block of new rows
+    static constexpr int PER = (NEW + ROWS * 64 - 1) / (ROWS * 64);
+    __bf16 *win;
+    const __bf16 *in;
+    int b, x0, H, W;
+    __device__ __forceinline__ static int slot(int y) { return (y + 1) % NR; }
+    __device__ __forceinline__ const bf16x8 *src(int y, int px, int q) const
+    {
+        return (const bf16x8 *)(in + padded(b, y, x0 - 1 + px, H, W) * CH + q * 8);
+    }
+    __device__ __forceinline__ bf16x8 *dst(int y, int px, int q) const
+    {
+        return (bf16x8 *)(win + (slot(y) * WIN_X + px) * LDS_PIX + q * 8);
+    }
+    // rows y_first .. y_first + n - 1 straight into the ring
+    __device__ __forceinline__ void stage(int y_first, int n) const
+    {
+        for (int c = threadIdx.x; c < n * PIECES; c += ROWS * 64) {
+            const int r = c / PIECES, rem = c - r * PIECES;
+            *dst(y_first + r, rem >> 2, rem & 3) = *src(y_first + r, rem >> 2, rem & 3);
+        }
+    }
+    __device__ __forceinline__ void prefetch(bf16x8 (&pf)[PER], int y_first) const
+    {
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int c = threadIdx.x + j * ROWS * 64;
+            if (c < NEW) {
+                const int r = c / PIECES, rem = c - r * PIECES;
+                pf[j] = *src(y_first + r, rem >> 2, rem & 3);
+            }
+        }
+    }
+    __device__ __forceinline__ void commit(const bf16x8 (&pf)[PER], int y_first) const
+    {
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int c = threadIdx.x + j * ROWS * 64;
+            if (c < NEW) {
+                const int r = c / PIECES, rem = c - r * PIECES;
+                *dst(y_first + r, rem >> 2, rem & 3) = pf[j];
+            }
+        }
+    }
+    // input row y, pixel x0 + px (px = -1 .. 64), this lane's 8 channels
+    __device__ __forceinline__ bf16x8 at(int y, int px, int kg) const
+    {
+        return *(const bf16x8 *)(win + (slot(y) * WIN_X + 1 + px) * LDS_PIX + kg * 8);
+    }
+};
+
+// body(ring, y0): the products and stores of the block of output rows y0 .. y0+ROWS-1
+template <int ROWS, typename Body>
+__device__ __forceinline__ void conv_march(const __bf16 *__restrict__ in, __bf16 *win, int H, int W, int run_rows,
+                                           Body body)
+{
+    using Ring = MarchRing<ROWS>;
+    const Ring ring{win, in, (int)blockIdx.z, (int)blockIdx.x * 64, H, W};
+    const int ya = blockIdx.y * run_rows, yb = ya + run_rows < H ? ya + run_rows : H;
+    ring.stage(ya - 1, ROWS + 2);
+    bf16x8 pf[Ring::PER];
+    for (int y0 = ya; y0 < yb; y0 += ROWS) {
+        const bool more = y0 + ROWS < yb;
+        if (more) ring.prefetch(pf, y0 + ROWS + 1);   // the next block's new rows
+        __syncthreads();                              // rows y0-1 .. y0+ROWS in the ring
+        body(ring, y0);
+        __syncthreads();                              // rows y0-1 .. y0+ROWS-2 no longer read
+        if (more) ring.commit(pf, y0 + ROWS + 1);
+    }
+}
+
+// The weight fragments and biases are read once per workgroup, into LDS and
+// registers: a global read inside the march would wait for the next block's
+// rows in flight before it (the memory counter completes in order).
+template <int ROWS>
+__global__ __launch_bounds__(ROWS * 64) void k_conv32_march(const __bf16 *__restrict__ in, __bf16 *__restrict__ out,
+                                                            const bf16x8 *__restrict__ frag,
+                                                            const float *__restrict__ bias, int H, int W, int run_rows)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 win[(ROWS + 2) * WIN_X * LDS_PIX];
+    __shared__ bf16x8 wsh[TAPS * 2 * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    for (int i = threadIdx.x; i < TAPS * 2 * 64; i += ROWS * 64) wsh[i] = frag[i];
+    float bv[2][4];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) bv[nb][i] = bias[nb * 16 + kg * 4 + i];
+    conv_march<ROWS>(in, win, H, W, run_rows, [&](const MarchRing<ROWS> &ring, int y0) {
+        floatx4 acc[4][2];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) acc[mb][0] = acc[mb][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < TAPS; t++) {
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            const bf16x8 w0 = wsh[(t * 2 + 0) * 64 + lane];
+            const bf16x8 w1 = wsh[(t * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) {
+                const bf16x8 a = ring.at(y0 + wave + dy, mb * 16 + m + dx, kg);
+                acc[mb][0] = mfma(w0, a, acc[mb][0]);
+                acc[mb][1] = mfma(w1, a, acc[mb][1]);
+            }
+        }
+        // store_nhwc32's forward expression with the biases held in registers
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                const long idx = padded(ring.b, y0 + wave, ring.x0 + mb * 16 + m, H, W) * CH + nb * 16 + kg * 4;
+                bf16x4 o;
+#pragma unroll
+                for (int i = 0; i < 4; i++) o[i] = (__bf16)fmaxf(acc[mb][nb][i] + bv[nb][i], 0.0f);
+                *(bf16x4 *)(out + idx) = o;
+            }
+    });
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(ROWS * 64) void k_conv_out_march(const __bf16 *__restrict__ a5,
+                                                              const bf16x8 *__restrict__ frag,
+                                                              const float *__restrict__ b6, float *__restrict__ out,
+                                                              int H, int W, int run_rows)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 win[(ROWS + 2) * WIN_X * LDS_PIX];
+    __shared__ bf16x8 wsh[TAPS * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    for (int i = threadIdx.x; i < TAPS * 64; i += ROWS * 64) wsh[i] = frag[i];
+    const float bv = b6[0];
+    conv_march<ROWS>(a5, win, H, W, run_rows, [&](const MarchRing<ROWS> &ring, int y0) {
+        floatx4 acc[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) acc[mb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < TAPS; t++) {
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            const bf16x8 w = wsh[t * 64 + lane];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) acc[mb] = mfma(w, ring.at(y0 + wave + dy, mb * 16 + m + dx, kg), acc[mb]);
+        }
+        if (kg == 0) {
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+                out[((long)ring.b * H + y0 + wave) * W + ring.x0 + mb * 16 + m] = acc[mb][0] + bv;
+        }
+    });
+}
+
 // ---------------------------------------------------------------- loss and backward
 
 // Loss over the cells M at least `margin` from the crop edge, e = out - target:
@@ -592,17 +755,53 @@ void im2col(const Acts &a, const float *planes, int npl, hipStream_t stream)
                        a.B, a.H, a.W);
 }
 
+// Rows per run of the marching layers: about one workgroup (95 KB of LDS)
+// per compute unit, a multiple of rows.
+int march_run_rows(int H, int W, int B, int rows)
+{
+    static int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+            n = 256;
+        return n;
+    }();
+    const long bands = (long)(W / 64) * B;
+    const long runs = bands >= cus ? 1 : cus / bands;
+    const long per = (H + runs - 1) / runs;
+    return (int)((per + rows - 1) / rows * rows);
+}
+
+// The marching layers (k_conv32_march, k_conv_out_march) for one image where
+// the height is a multiple of 16, unless TEA_WS_MARCH=0.
 template <int ROWS> void forward_rows(const Net &net, const Acts &a, const float *planes, int npl, hipStream_t stream)
 {
     const dim3 grid(a.W / 64, a.H / ROWS, a.B), block(ROWS * 64);
     const bf16x8 *pk = (const bf16x8 *)net.packed;
+    const char *env = getenv("TEA_WS_MARCH");
+    const bool march = ROWS == 16 && a.B == 1 && !(env && env[0] == '0');
+    int run_rows = march_run_rows(a.H, a.W, a.B, ROWS);
+    // TEA_WS_MARCH_ROWS=<rows> sets the run length (tests march several blocks on small images)
+    if (const char *s = getenv("TEA_WS_MARCH_ROWS")) {
+        const int v = atoi(s);
+        if (v > 0) run_rows = (v + ROWS - 1) / ROWS * ROWS;
+    }
+    const dim3 mgrid(a.W / 64, (a.H + run_rows - 1) / run_rows, a.B);
     hipLaunchKernelGGL(k_conv_first<ROWS>, grid, block, 0, stream, planes, npl, a.a[1], pk + PK_L1 / 8,
                        net.params + OFF_B1, a.H, a.W);
-    for (int l = 2; l <= 5; l++)
-        hipLaunchKernelGGL((k_conv32<ROWS, false>), grid, block, 0, stream, a.a[l - 1], a.a[l], pk + pk_fwd(l) / 8,
-                           net.params + off_b(l), (const __bf16 *)nullptr, a.H, a.W);
-    hipLaunchKernelGGL(k_conv_out<ROWS>, grid, block, 0, stream, a.a[5], pk + PK_OUT / 8, net.params + OFF_B6, a.out,
-                       a.H, a.W);
+    for (int l = 2; l <= 5; l++) {
+        if (march)
+            hipLaunchKernelGGL(k_conv32_march<ROWS>, mgrid, block, 0, stream, a.a[l - 1], a.a[l], pk + pk_fwd(l) / 8,
+                               net.params + off_b(l), a.H, a.W, run_rows);
+        else
+            hipLaunchKernelGGL((k_conv32<ROWS, false>), grid, block, 0, stream, a.a[l - 1], a.a[l],
+                               pk + pk_fwd(l) / 8, net.params + off_b(l), (const __bf16 *)nullptr, a.H, a.W);
+    }
+    if (march)
+        hipLaunchKernelGGL(k_conv_out_march<ROWS>, mgrid, block, 0, stream, a.a[5], pk + PK_OUT / 8,
+                           net.params + OFF_B6, a.out, a.H, a.W, run_rows);
+    else
+        hipLaunchKernelGGL(k_conv_out<ROWS>, grid, block, 0, stream, a.a[5], pk + PK_OUT / 8, net.params + OFF_B6,
+                           a.out, a.H, a.W);
 }
 
 // Inference takes workgroups of 16 rows where the height allows it: a smaller

@@ -64,6 +64,32 @@ def test_forward_matches_oracle():
         t.close()
 
 
+
+@pytest.mark.parametrize("run_rows", [None, 48, 64, 96])
+def test_marching_layers_match_windowed(run_rows, monkeypatch):
+    """One image with a height a multiple of 16: the hidden and output layers
+    march down runs of rows (k_conv32_march, k_conv_out_march; runs of several
+    blocks forced with TEA_WS_MARCH_ROWS) and give the windowed kernels'
+    output bit for bit (TEA_WS_MARCH=0), which the oracle also checks."""
+    print(f"[gpu warmstart] marching layers, runs of {run_rows} rows", flush=True)
+    B, H, W = 1, 96, 128
+    t = wsm.Trainer(B, H, W, margin=1)
+    try:
+        p = _params(3)
+        t.set_params(p)
+        s, sd = _samples(4, B, H, W)
+        monkeypatch.setenv("TEA_WS_MARCH", "0")
+        windowed = t.forward(sd)
+        monkeypatch.delenv("TEA_WS_MARCH")
+        if run_rows:
+            monkeypatch.setenv("TEA_WS_MARCH_ROWS", str(run_rows))
+        marched = t.forward(sd)
+        assert np.array_equal(marched, windowed)
+        ref, _ = co.forward(p, s, mode="bf16")
+        assert _rel(marched, ref) < 5e-3
+    finally:
+        t.close()
+
 @pytest.mark.parametrize("loss_kind", ["mse", "energy"])
 def test_gradient_matches_oracle(loss_kind):
     print(f"[gpu warmstart] gradient vs oracle ({loss_kind})", flush=True)

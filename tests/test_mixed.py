@@ -197,6 +197,45 @@ def test_mixed_reaches_fp64_true_residual(solver, prec, kw, oracle_so, oracle_en
     assert "FP32 refinements" in text
 
 
+@pytest.mark.parametrize("solver,prec,kw", [MIXED_CASES[0], MIXED_CASES[4], MIXED_CASES[6],
+                                             ("ppcg", "jac_block", dict(inner_precision="fp64"))])
+def test_exact_error_is_the_final_true_residual(solver, prec, kw, oracle_so, oracle_env, tmp_path):
+    """tl_check_result after a refinement loop reports the loop's last true
+    residual instead of computing it again: it must be ||u0 - A u|| / ||r0||
+    for the u the step ends with, checked here against numpy on the fields
+    (r0 = u0 - A u0: no warm start, so the solve starts from u = u0)."""
+    oracle_env("dot2")
+    eng = Engine(str(oracle_so))
+    measured = []
+
+    def on_step(step):
+        eng.refresh()
+        h, ny, nx = eng.h, 80, 96
+        u = eng.get_field(rt.FIELD_U)
+        u0 = eng.get_field(rt.XFIELD_U0)
+        kx, ky = eng.get_field(rt.XFIELD_KX), eng.get_field(rt.XFIELD_KY)
+
+        def residual(v):
+            c = v[h:h + ny, h:h + nx]
+            kl, kr = kx[h:h + ny, h:h + nx], kx[h:h + ny, h + 1:h + nx + 1]
+            kb, kt = ky[h:h + ny, h:h + nx], ky[h + 1:h + ny + 1, h:h + nx]
+            av = ((1.0 + (kt + kb) + (kr + kl)) * c - (kt * v[h + 1:h + ny + 1, h:h + nx] + kb * v[h - 1:h + ny - 1, h:h + nx])
+                  - (kr * v[h:h + ny, h + 1:h + nx + 1] + kl * v[h:h + ny, h - 1:h + nx - 1]))
+            return u0[h:h + ny, h:h + nx] - av
+        measured.append(np.linalg.norm(residual(u)) / np.linalg.norm(residual(u0)))
+
+    inner = dict(ppcg_inner_steps=10) if solver == "ppcg" else {}
+    kw = dict(kw)
+    kw.setdefault("inner_precision", "fp32")
+    deck = Deck(x_cells=96, y_cells=80, end_step=3, eps=1e-10, check_result=True, summary_frequency=1, states=LAYOUT,
+                solver=solver, preconditioner=prec, **kw, **inner)
+    res = run_deck(deck.write(tmp_path / "tea.in"), backend=oracle_so, on_step=on_step)
+    assert len(measured) == len(res.exact_error) == 3
+    for got, want in zip(res.exact_error, measured):
+        assert 0 < got < 1e-10
+        assert got == pytest.approx(want, rel=1e-4), (res.exact_error, measured)
+
+
 @pytest.mark.parametrize("solver", ["cg", "ppcg", "chebyshev"])
 def test_fp64_restart_is_the_fp64_solve(solver, oracle_so, oracle_env, tmp_path):
     """tl_inner_precision fp64 with restarts runs its inner solves on the FP64

@@ -401,6 +401,162 @@ __global__ __launch_bounds__(ROWS * 64) void k_conv32_march(const __bf16 *__rest
     });
 }
 
+// Layers 1 and 2 in one march (inference, one image): the rows of layer 1's
+// output that layer 2's window needs are formed in the LDS ring from the
+// feature planes as the march reaches them (k_conv_first's operands and
+// expressions, zero outside the image as its padded output holds), instead
+// of being stored and read back: 12 bytes per pixel read instead of
+// 12 + 64 written + 64 read.  Layer 1 covers 80 pixels (five 16-pixel
+// blocks from x0 - 8) of which the ring keeps x0 - 1 .. x0 + 64.  The next
+// block's feature rows are in flight into registers while layer 2 works.
+// The same values as k_conv_first followed by k_conv32.
+template <int ROWS> struct FirstRows {
+    static constexpr int FX = 82;                       // feature columns x0 - 9 .. x0 + 72
+    static constexpr int FY = ROWS + 2;                 // feature rows of ROWS layer-1 rows
+    static constexpr int N = NFEAT * FY * FX;
+    static constexpr int PER = (N + ROWS * 64 - 1) / (ROWS * 64);
+    const float *pl;
+    int H, W, x0;
+    // feature f at row y, column x0 - 9 + px (zero outside the image)
+    __device__ __forceinline__ float load(int c, int y_first) const
+    {
+        const int f = c / (FY * FX), rem = c - f * (FY * FX), r = rem / FX, px = rem - r * FX;
+        const int y = y_first - 1 + r, x = x0 - 9 + px;
+        return (y >= 0 && y < H && x >= 0 && x < W) ? pl[(long)f * H * W + (long)y * W + x] : 0.0f;
+    }
+    __device__ __forceinline__ void prefetch(float (&pf)[PER], int y_first) const
+    {
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int c = threadIdx.x + j * ROWS * 64;
+            if (c < N) pf[j] = load(c, y_first);
+        }
+    }
+    __device__ __forceinline__ void commit(float *fb, const float (&pf)[PER]) const
+    {
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int c = threadIdx.x + j * ROWS * 64;
+            if (c < N) fb[c] = pf[j];
+        }
+    }
+};
+
+template <int ROWS>
+__global__ __launch_bounds__(ROWS * 64) void k_first_conv32_march(const float *__restrict__ planes, int npl,
+                                                                  __bf16 *__restrict__ out,
+                                                                  const bf16x8 *__restrict__ frag1,
+                                                                  const float *__restrict__ bias1,
+                                                                  const bf16x8 *__restrict__ frag2,
+                                                                  const float *__restrict__ bias2, int H, int W,
+                                                                  int run_rows)
+{
+    using FR = FirstRows<ROWS>;
+    __shared__ __attribute__((aligned(16))) __bf16 win[(ROWS + 2) * WIN_X * LDS_PIX];
+    __shared__ float fb[FR::N];
+    __shared__ bf16x8 wsh1[2 * 64], wsh2[TAPS * 2 * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    const int x0 = blockIdx.x * 64, b = blockIdx.z;
+    const FR fr{planes + (long)b * npl * H * W, H, W, x0};
+    const MarchRing<ROWS> ring{win, nullptr, b, x0, H, W};
+    for (int i = threadIdx.x; i < TAPS * 2 * 64; i += ROWS * 64) wsh2[i] = frag2[i];
+    for (int i = threadIdx.x; i < 2 * 64; i += ROWS * 64) wsh1[i] = frag1[i];
+    float bv1[2][4], bv2[2][4];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            bv1[nb][i] = bias1[nb * 16 + kg * 4 + i];
+            bv2[nb][i] = bias2[nb * 16 + kg * 4 + i];
+        }
+    // this lane's 8 im2col K-values: offsets into fb relative to the pixel (k_conv_first's)
+    int off[8];
+    bool real[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = kg * 8 + i, t = k / 3, f = k - 3 * t;
+        real[i] = k < K1;
+        off[i] = real[i] ? (f * FR::FY + (t / 3 - 1)) * FR::FX + (t % 3 - 1) : 0;
+    }
+    // layer-1 rows y_first .. y_first + n - 1 (wave w: row y_first + w) into the ring from the
+    // features in fb (rows y_first - 1 .. y_first + n)
+    auto form = [&](int y_first, int n) {
+        if (wave >= n) return;
+        const int y = y_first + wave;
+        const bool row_in = y >= 0 && y < H;
+        const float *base = fb + (wave + 1) * FR::FX + 9;   // pixel x0 + 0 of row y
+        const bf16x8 w0 = wsh1[lane], w1 = wsh1[64 + lane];
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++) {
+            const int px = mb * 16 + m - 8;   // relative to x0
+            bf16x8 a;
+#pragma unroll
+            for (int i = 0; i < 8; i++) a[i] = (__bf16)(real[i] ? base[px + off[i]] : 0.0f);
+            const floatx4 z = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            const floatx4 acc[2] = {mfma(w0, a, z), mfma(w1, a, z)};
+            if (px >= -1 && px <= 64) {
+                const bool in = row_in && x0 + px >= 0 && x0 + px < W;
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++) {
+                    bf16x4 o;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) o[i] = in ? (__bf16)fmaxf(acc[nb][i] + bv1[nb][i], 0.0f) : (__bf16)0.0f;
+                    *(bf16x4 *)(win + (MarchRing<ROWS>::slot(y) * WIN_X + 1 + px) * LDS_PIX + nb * 16 + kg * 4) = o;
+                }
+            }
+        }
+    };
+    const int ya = blockIdx.y * run_rows, yb = ya + run_rows < H ? ya + run_rows : H;
+    float pf[FR::PER];
+    // prologue: layer-1 rows ya - 1 .. ya + ROWS
+    fr.prefetch(pf, ya - 1);
+    fr.commit(fb, pf);
+    __syncthreads();
+    form(ya - 1, ROWS);
+    __syncthreads();
+    fr.prefetch(pf, ya - 1 + ROWS);
+    fr.commit(fb, pf);
+    __syncthreads();
+    form(ya - 1 + ROWS, 2);
+    for (int y0 = ya; y0 < yb; y0 += ROWS) {
+        const bool more = y0 + ROWS < yb;
+        if (more) fr.prefetch(pf, y0 + ROWS + 1);   // features of the next block's layer-1 rows
+        __syncthreads();                            // layer-1 rows y0-1 .. y0+ROWS in the ring
+        floatx4 acc[4][2];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) acc[mb][0] = acc[mb][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < TAPS; t++) {
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            const bf16x8 w0 = wsh2[(t * 2 + 0) * 64 + lane];
+            const bf16x8 w1 = wsh2[(t * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) {
+                const bf16x8 a = ring.at(y0 + wave + dy, mb * 16 + m + dx, kg);
+                acc[mb][0] = mfma(w0, a, acc[mb][0]);
+                acc[mb][1] = mfma(w1, a, acc[mb][1]);
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                const long idx = padded(b, y0 + wave, x0 + mb * 16 + m, H, W) * CH + nb * 16 + kg * 4;
+                bf16x4 o;
+#pragma unroll
+                for (int i = 0; i < 4; i++) o[i] = (__bf16)fmaxf(acc[mb][nb][i] + bv2[nb][i], 0.0f);
+                *(bf16x4 *)(out + idx) = o;
+            }
+        __syncthreads();   // layer-1 rows y0-1 .. y0+ROWS-2 and the features no longer read
+        if (more) {
+            fr.commit(fb, pf);
+            __syncthreads();
+            form(y0 + ROWS + 1, ROWS);
+        }
+    }
+}
+
 template <int ROWS>
 __global__ __launch_bounds__(ROWS * 64) void k_conv_out_march(const __bf16 *__restrict__ a5,
                                                               const bf16x8 *__restrict__ frag,
@@ -771,8 +927,9 @@ int march_run_rows(int H, int W, int B, int rows)
     return (int)((per + rows - 1) / rows * rows);
 }
 
-// The marching layers (k_conv32_march, k_conv_out_march) for one image where
-// the height is a multiple of 16, unless TEA_WS_MARCH=0.
+// The marching layers (k_first_conv32_march, k_conv32_march,
+// k_conv_out_march) for one image where the height is a multiple of 16,
+// unless TEA_WS_MARCH=0.
 template <int ROWS> void forward_rows(const Net &net, const Acts &a, const float *planes, int npl, hipStream_t stream)
 {
     const dim3 grid(a.W / 64, a.H / ROWS, a.B), block(ROWS * 64);
@@ -786,9 +943,13 @@ template <int ROWS> void forward_rows(const Net &net, const Acts &a, const float
         if (v > 0) run_rows = (v + ROWS - 1) / ROWS * ROWS;
     }
     const dim3 mgrid(a.W / 64, (a.H + run_rows - 1) / run_rows, a.B);
-    hipLaunchKernelGGL(k_conv_first<ROWS>, grid, block, 0, stream, planes, npl, a.a[1], pk + PK_L1 / 8,
-                       net.params + OFF_B1, a.H, a.W);
-    for (int l = 2; l <= 5; l++) {
+    if (march)   // layers 1 and 2 in one march (layer 1's output stays in LDS)
+        hipLaunchKernelGGL(k_first_conv32_march<ROWS>, mgrid, block, 0, stream, planes, npl, a.a[2], pk + PK_L1 / 8,
+                           net.params + OFF_B1, pk + pk_fwd(2) / 8, net.params + off_b(2), a.H, a.W, run_rows);
+    else
+        hipLaunchKernelGGL(k_conv_first<ROWS>, grid, block, 0, stream, planes, npl, a.a[1], pk + PK_L1 / 8,
+                           net.params + OFF_B1, a.H, a.W);
+    for (int l = march ? 3 : 2; l <= 5; l++) {
         if (march)
             hipLaunchKernelGGL(k_conv32_march<ROWS>, mgrid, block, 0, stream, a.a[l - 1], a.a[l], pk + pk_fwd(l) / 8,
                                net.params + off_b(l), a.H, a.W, run_rows);

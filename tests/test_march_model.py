@@ -15,6 +15,10 @@ NaN here, so a use of one would reach the compared outputs.
 * the model's NS steps equal the oracle's NS single steps bit for bit for
   NS = 1..5, both precisions, the three preconditioners, ragged meshes, halo
   depths 1 and 2, and chunks of 1 strip up to the whole mesh;
+* the first pass of an outer iteration that forms sd = M^-1 r / theta from r
+  itself (block Jacobi) equals the oracle's z = M^-1 r, init_sd_new and NS
+  steps bit for bit with NS halo lanes at the left and NS + 1 at the right,
+  and not with NS at the right;
 * utemp is read and written in place by the kernel.  A wave reads utemp at the
   strips of the neighbouring chunks it advances (written by other waves in the
   same launch), but those values never reach a store: the model gives the same
@@ -50,9 +54,9 @@ def _above(v):   # lane l gets lane l+1's value, the last lane gets 0 (wave_shl:
     return out
 
 
-def launch_shape(nx, ny, ns, per_simd, cu=256, chunk=None):
+def launch_shape(nx, ny, ns, per_simd, cu=256, chunk=None, from_r=False):
     """Bands, strips and chunk length as launch_inner_march chooses them."""
-    ox = WAVE - 2 * ns + 1
+    ox = WAVE - 2 * ns + 1 - (2 if from_r else 0)
     nbands = (nx - 1 + ox) // ox
     nstrips = (ny - 1 + STRIP) // STRIP
     if chunk is None:
@@ -62,14 +66,22 @@ def launch_shape(nx, ny, ns, per_simd, cu=256, chunk=None):
     return nbands, nstrips, chunk, (nstrips + chunk - 1) // chunk
 
 
-def model_pass(SD, RT, UT, KX, KY, MI, h, nx, ny, ns, chunk, a, b, prec, poison_foreign_ut=False):
+def model_pass(SD, RT, UT, KX, KY, MI, h, nx, ny, ns, chunk, a, b, prec, poison_foreign_ut=False, from_r=False,
+               theta=None, halo_r=None):
     """NS inner steps as the streaming pass computes them; returns the three
     output arrays (cells the pass does not write keep the input values) and
-    the number of times each interior cell was written."""
+    the number of times each interior cell was written.  from_r (block
+    Jacobi): the pass forms sd = M^-1 r / theta and utemp = sd inside the
+    extent from RT (= r) as each strip enters, reading only sd's halo ring;
+    halo_r overrides the right halo lanes (NS + 1 for that pass)."""
     T = SD.dtype.type
     x0, x1, y0, y1 = h, h + nx - 1, h, h + ny - 1
-    hl, hr, ox, rn = ns - 1, ns, WAVE - 2 * ns + 1, ns + 1
-    nbands, nstrips, chunk, nchunks = launch_shape(nx, ny, ns, 2, chunk=chunk)
+    hl, hr, rn = (ns, ns + 1, ns + 1) if from_r else (ns - 1, ns, ns + 1)
+    if halo_r is not None:
+        hr = halo_r
+    ox = WAVE - hl - hr
+    nbands, nstrips, chunk, nchunks = launch_shape(nx, ny, ns, 2, chunk=chunk, from_r=from_r)
+    nbands = (nx - 1 + ox) // ox
     W = nbands * nchunks
     wid = np.arange(W)
     band, ck = wid % nbands, wid // nbands
@@ -99,21 +111,37 @@ def model_pass(SD, RT, UT, KX, KY, MI, h, nx, ny, ns, chunk, a, b, prec, poison_
         ln = strip_len(r0)
         act = np.where(col_in, ln, 0)
         m = {"r0": r0, "len": ln}
-        m["sd"] = [ld(SD, r0 + k, col_mem & (r0 + k >= y0 - 1) & (r0 + k <= y1 + 1)) for k in range(STRIP)]
+        m["sd"] = [ld(SD, r0 + k, col_mem & (r0 + k >= y0 - 1) & (r0 + k <= y1 + 1) & ~(from_r & (k < act)))
+                   for k in range(STRIP)]
         m["ky"] = [ld(KY, r0 + k, (k <= act) & (act > 0)) for k in range(STRIP + 1)]
         m["kx"] = [ld(KX, r0 + k, col_mem & (k < ln)) for k in range(STRIP)]
         m["rt"] = [ld(RT, r0 + k, k < act) for k in range(STRIP)]
         own = ((s >= s_first) & (s <= s_last))[:, None]
-        ut = [ld(UT, r0 + k, (k < act) & out_lane) for k in range(STRIP)]
+        ut = [ld(UT, r0 + k, (k < act) & out_lane & (not from_r)) for k in range(STRIP)]
         if poison_foreign_ut:   # values of other waves' chunks
             ut = [np.where(own | ~((k < act) & out_lane), u, T(np.nan)).astype(T) for k, u in enumerate(ut)]
         m["ut"] = ut
         m["mi"] = [ld(MI, r0 + k, k < act) if prec == "diag" else np.zeros((W, WAVE), T) for k in range(STRIP)]
         # lane 0's left neighbour column
         rr = np.clip(r0, 0, rows - 1)
-        m["sde"] = [np.where((k < act) & (lane == 0), SD[np.clip(rr + k, 0, rows - 1), np.clip(c - 1, 0, cols - 1)],
-                             T(0.0)).astype(T) for k in range(STRIP)]
+        m["sde"] = [np.where((k < act) & (lane == 0) & (not from_r),
+                             SD[np.clip(rr + k, 0, rows - 1), np.clip(c - 1, 0, cols - 1)], T(0.0)).astype(T)
+                    for k in range(STRIP)]
         return m
+
+    def apply_block(rv, diag, ky, ln, cp, bfp):   # strip_apply (rows past the strip stay NaN)
+        zv = [np.full((W, WAVE), np.nan, T) for _ in range(STRIP)]
+        dp = [np.full((W, WAVE), np.nan, T) for _ in range(STRIP)]
+        dp[0] = rv[0] / diag[0]
+        for k in range(1, STRIP):
+            dp[k] = np.where(k < ln, (rv[k] - (-ky[k]) * dp[k - 1]) * bfp[k], T(np.nan)).astype(T)
+        for k in range(STRIP - 1, -1, -1):
+            if k == STRIP - 1:
+                zv[k] = np.where(ln == STRIP, dp[k], zv[k]).astype(T)
+            else:
+                z_last = np.where(ln == k + 1, dp[k], zv[k])
+                zv[k] = np.where(k < ln - 1, dp[k] - cp[k] * zv[k + 1], z_last).astype(T)
+        return zv
 
     def factor(m):
         ln = m["len"]
@@ -128,6 +156,13 @@ def model_pass(SD, RT, UT, KX, KY, MI, h, nx, ny, ns, chunk, a, b, prec, poison_
                     ck_ = (-m["ky"][k + 1]) * bk
                     bfp[k] = np.where(k < ln, bk, T(0.0)).astype(T)
                     cp[k] = np.where(k < ln, ck_, T(0.0)).astype(T)
+                if from_r:   # sd = utemp = M^-1 r / theta inside the extent (the last lane lacks kxr: stale)
+                    zv = apply_block(m["rt"], diag, m["ky"], ln, cp, bfp)
+                    theta_r = T(1.0) / T(theta)
+                    for k in range(STRIP):
+                        inside = col_in & (k < ln)
+                        m["sd"][k] = np.where(inside, zv[k] * theta_r, m["sd"][k]).astype(T)
+                        m["ut"][k] = np.where(inside, m["sd"][k], m["ut"][k]).astype(T)
         m["cp"], m["bfp"] = cp, bfp
 
     def advance(m, lo, hi, sde, st, active):
@@ -147,16 +182,7 @@ def model_pass(SD, RT, UT, KX, KY, MI, h, nx, ny, ns, chunk, a, b, prec, poison_
                 diag.append(dg)
             zv = [np.full((W, WAVE), np.nan, T) for _ in range(STRIP)]
             if prec == "block":   # strip_apply over the strip's rows (rows past it stay NaN)
-                dp = [np.full((W, WAVE), np.nan, T) for _ in range(STRIP)]
-                dp[0] = rv[0] / diag[0]
-                for k in range(1, STRIP):
-                    dp[k] = np.where(k < ln, (rv[k] - (-ky[k]) * dp[k - 1]) * m["bfp"][k], T(np.nan)).astype(T)
-                for k in range(STRIP - 1, -1, -1):
-                    if k == STRIP - 1:
-                        zv[k] = np.where(ln == STRIP, dp[k], zv[k]).astype(T)
-                    else:
-                        z_last = np.where(ln == k + 1, dp[k], zv[k])
-                        zv[k] = np.where(k < ln - 1, dp[k] - m["cp"][k] * zv[k + 1], z_last).astype(T)
+                zv = apply_block(rv, diag, ky, ln, m["cp"], m["bfp"])
             for k in range(STRIP):
                 if prec == "block":
                     sdn = al * sd[k] + be * zv[k]
@@ -309,3 +335,44 @@ def test_no_cross_wave_hazard(n, ns, per_simd):
                 assert own[0] - lo <= ns * STRIP and hi - own[1] <= ns * STRIP
     assert rows_written[0][0] == y0 and rows_written[-1][1] == y1
     assert all(rows_written[k][1] + 1 == rows_written[k + 1][0] for k in range(nchunks - 1))
+
+
+FROM_R_CASES = [(96, 80, 1, 5, 3), (96, 80, 1, 5, None), (131, 77, 1, 4, 2), (57, 9, 2, 4, 2), (150, 130, 1, 5, 40)]
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("nx,ny,h,ns,chunk", FROM_R_CASES)
+def test_from_r_model_equals_oracle(nx, ny, h, ns, chunk, fp32, oracle_so, oracle_env, tmp_path):
+    """The first pass of an outer iteration with sd formed from r
+    (k_ppcg_inner_ringl FROM_R, block Jacobi): the model's pass over r equals
+    the oracle's z = M^-1 r (ppcg_init step 2) and init_sd_new(theta)
+    followed by NS steps, bit for bit, with NS halo lanes at the left and
+    NS + 1 at the right.  With NS at the right it does not: the last lane
+    forms its own sd without its right neighbour's Kx, and that stale value
+    reaches a written column in NS steps."""
+    oracle_env("dot2")
+    a = [0.3, 0.55, 0.7, 0.2, 0.6, 0.45, 0.5]
+    b = [1.1, 0.9, 0.8, 1.2, 0.7, 1.0, 0.95]
+    theta = 1.7
+    eng, fields, coef, get = _oracle_case(oracle_so, tmp_path, nx, ny, h, "block", fp32, seed=3 * nx + ns)
+    put = eng.set_field32 if fp32 else eng.set_field
+    R = np.random.default_rng(nx * ny).standard_normal(eng.shape()).astype(np.float32 if fp32 else np.float64)
+    try:
+        put(rt.FIELD_R, R)
+        eng.ppcg_init_constants(a, b)
+        eng.ppcg_init(2)
+        eng.ppcg_init_sd_new(theta)
+        eng.ppcg_inner_n(1, ns)
+        expect = [get(f) for f in (rt.FIELD_SD, rt.XFIELD_RTEMP, rt.XFIELD_UTEMP)]
+    finally:
+        eng.finalize()
+    SD, UT = fields[rt.FIELD_SD], fields[rt.XFIELD_UTEMP]
+    args = (SD, R, UT, coef["KX"], coef["KY"], coef["MI"], h, nx, ny, ns, chunk, a, b, "block")
+    got = model_pass(*args, from_r=True, theta=theta)
+    inner = (slice(h, h + ny), slice(h, h + nx))
+    for g_, e_, name in zip(got[:3], expect, ("sd", "rtemp", "utemp")):
+        assert np.array_equal(g_[inner], e_[inner]), f"{name}: {np.sum(g_[inner] != e_[inner])} cells differ"
+    assert np.all(got[3][:, inner[0], inner[1]] == 1) and got[3].sum() == 3 * nx * ny
+    if nx >= WAVE - ns:   # the first band's last lane lies inside the extent
+        short = model_pass(*args, from_r=True, theta=theta, halo_r=ns)
+        assert not np.array_equal(short[0][inner], expect[0][inner])

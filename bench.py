@@ -92,6 +92,10 @@ COMPULSORY_BYTES = {
     "k_ppcg_ur_sd_f32": 28, "k_ppcg_ur_sd": 56,
     # fused end: utemp, r, r_store read (z owed to the next begin)
     "ppcg_end_f32": 12, "ppcg_end": 24,
+    # the end with the next begin's store_r and r.z: utemp, r, r_store read; r_store written
+    "ppcg_end_rr_f32": 16, "ppcg_end_rr": 32,
+    # the begin after it: p, z (utemp), Kx, Ky read; p' written
+    "ppcg_begin_pw_f32": 20, "ppcg_begin_pw": 40,
     # CG iteration kernels: w = Ap (p, Kx, Ky read; w written); u/r/z update (u, p, r, w, Kx, Ky read;
     # u, r, z written); p = z + beta p (z, p read; p written)
     "cg_calc_w_f32": 16, "cg_calc_w": 32,

@@ -2005,10 +2005,11 @@ bool begin_march_launch(const char *name, int with_p, double beta, const double 
     return true;
 }
 
-// z_src: where z is read (null: the z field); w_later: w not stored.
+// z_src: where z is read (null: the z field); w_later: w not stored;
+// store_r false: rstore = r and r.z left out (k_ppcg_end_rr formed them).
 template <typename T>
 bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr, const T *z_src = nullptr,
-                       bool w_later = false)
+                       bool w_later = false, bool store_r = true)
 {
     const T *zf = z_src ? z_src : (const T *)FT<T>(X_Z);
     const long pt = pitch_of<T>();
@@ -2021,7 +2022,11 @@ bool ppcg_begin_launch(int with_p, double beta, const double *beta_num = nullptr
         dirty = false;
     }
     bool ok;
-    if constexpr (std::is_same<T, float>::value)
+    if (!store_r && begin_march_launch<T, false>(nm<T>("ppcg_begin_pw"), with_p, beta, beta_num,
+                                                 (const T *)FT<T>(F_P), FT<T>(X_P2), zf,
+                                                 w_later ? nullptr : FT<T>(X_W), (const T *)nullptr, (T *)nullptr))
+        ok = true;   // else the begins below, whose rstore = r and r.z repeat the end's
+    else if constexpr (std::is_same<T, float>::value)
         ok = begin_march_launch<T, true>("ppcg_begin_f32", with_p, beta, beta_num, (const T *)FT<T>(F_P), FT<T>(X_P2),
                                          zf, w_later ? nullptr : FT<T>(X_W), (const T *)FT<T>(F_R), FT<T>(X_RSTORE)) ||
              launch_x2("ppcg_begin_f32", k_ppcg_begin_x2, interior(), pt, with_p, beta, beta_num, (const T *)FT<T>(F_P),
@@ -2139,8 +2144,19 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
     const bool z_from_utemp = g.prec != PREC_NONE;
     const T *ut = FT<T>(X_UTEMP);
     int eslot = 3;
-    if (g.end_slot >= 0 && g.end_call == g.calls - 1) {
+    // end_rr: the end also stores r and forms the begin's r.z (k_ppcg_end_rr),
+    // unless TEA_END_STORE_R=0: finished into h_out[6..7] and d_scal[0..1], where
+    // begin takes beta's numerator and ur the r.z; begin's p.w then goes to slot 0
+    const char *rr_env = getenv("TEA_END_STORE_R");
+    const bool end_rr = !(g.end_slot >= 0 && g.end_call == g.calls - 1) && !(rr_env && rr_env[0] == '0');
+    if (!end_rr && g.end_slot >= 0 && g.end_call == g.calls - 1) {
         eslot = g.end_slot;
+    } else if (end_rr) {
+        eslot = 6;
+        if (!launch_capped(nm<T>("ppcg_end_rr"), 2048, k_ppcg_end_rr<T>, interior(), pitch_of<T>(), ut,
+                           z_from_utemp ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R), FT<T>(X_RSTORE), red_ctx()) ||
+            !finish(6, 2, g.d_scal))
+            return;
     } else if (!launch_capped(nm<T>("ppcg_end"), 2048, k_ppcg_end<T>, interior(), pitch_of<T>(), ut,
                               z_from_utemp ? nullptr : FT<T>(X_Z), (const T *)FT<T>(F_R),
                               (const T *)FT<T>(X_RSTORE), red_ctx()) ||
@@ -2150,8 +2166,9 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
     g.end_slot = -1;
     g.pending_zr = 0;   // the end's z = utemp replaces z (begin reads utemp)
     const bool wp = w_from_p<T>();
-    if (!ppcg_begin_launch<T>(1, rro_prev, g.d_scal + eslot, z_from_utemp ? ut : nullptr, wp) ||
-        !finish(0, 2, g.d_scal))
+    if (!ppcg_begin_launch<T>(1, rro_prev, end_rr ? g.d_scal : g.d_scal + eslot, z_from_utemp ? ut : nullptr, wp,
+                              !end_rr) ||
+        !finish(0, end_rr ? 1 : 2, g.d_scal))
         return;
     if (!ppcg_ur_launch<T>(0.0, g.d_scal, theta, use_march<T, 1>(), wp, sd_from_r<T>()) || !finish(2, 1, nullptr))
         return;
@@ -2176,6 +2193,7 @@ void ppcg_end_begin_ur_impl(double rro_prev, double theta, double *rrn_end, doub
     } else if (!collect(pw_rro, 0, 2)) {
         return;
     }
+    if (end_rr) pw_rro[1] = g.h_out[7];
     *rrn = g.h_out[2];
     *rrn_end = g.h_out[eslot];
 }

@@ -2623,6 +2623,31 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_end(Ext e, long pitch, const T *
     block_reduce<1>(acc, red);
 }
 
+// k_ppcg_end followed by the next fused begin's store_r and r.z (no reference
+// counterpart as a kernel; the same values): rstore = r and sum r.utemp,
+// which the begin would form from the same r (nothing changes r in between)
+// and utemp (the z it reads); the begin then reads neither r nor rstore.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_ppcg_end_rr(Ext e, long pitch, const T *__restrict__ utemp,
+                                                       T *__restrict__ z, const T *__restrict__ r,
+                                                       T *__restrict__ rstore, Red red)
+{
+    DD acc[2] = {{0, 0}, {0, 0}};
+    STRIP_BEGIN
+#pragma unroll
+    for (int k = 0; k < STRIP; k++)
+        if (k < s.len) {
+            const long i = i0 + k * pitch;
+            const T zv = utemp[i], rv = r[i];
+            if (z) z[i] = zv;
+            dd_add_prod(acc[0], (double)(rv - rstore[i]), (double)zv);
+            rstore[i] = rv;
+            dd_add_prod(acc[1], (double)rv, (double)zv);
+        }
+    STRIP_END
+    block_reduce<2>(acc, red);
+}
+
 // ---- mixed precision glue (no reference counterpart) -------------------------------
 
 // FP32 copies of the FP64 face coefficients over the whole array (halo included).

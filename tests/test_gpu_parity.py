@@ -108,7 +108,8 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
                                     "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY",
-                                    "TEA_PPCG_ITERATION", "TEA_FUSED_END", "TEA_CHEBY_PASSES", "TEA_SD_FROM_R"])
+                                    "TEA_PPCG_ITERATION", "TEA_FUSED_END", "TEA_CHEBY_PASSES", "TEA_SD_FROM_R",
+                                    "TEA_END_STORE_R"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
                                   "cg_block_ragged", "cheby_none", "mixed_cheby_diag"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
@@ -126,8 +127,9 @@ def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracl
     launch instead of in the last inner pass (TEA_FUSED_END=0), and
     Chebyshev iterations one call each instead of in runs of streaming passes
     (TEA_CHEBY_PASSES=0), and a fused ur that stores z and sd itself instead of
-    leaving sd to the first inner pass (TEA_SD_FROM_R=0), on the GPU give the
-    same results as the oracle."""
+    leaving sd to the first inner pass (TEA_SD_FROM_R=0), and the fused begin
+    storing r and forming r.z itself instead of the end before it
+    (TEA_END_STORE_R=0), on the GPU give the same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)

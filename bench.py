@@ -102,8 +102,11 @@ COMPULSORY_BYTES = {
     "k_cg_calc_ur_f32": 36, "k_cg_calc_ur": 72,
     "cg_calc_p_f32": 12, "cg_calc_p": 24, "ppcg_calc_p_f32": 12, "ppcg_calc_p": 24,
     # residual replacement in one pass: u, u0, Kx, Ky (FP64), u32, Kx32, Ky32 read;
-    # u, r (FP64), r32, u0_32, z32, sd32, u32 written
-    "mp_replace": 8 * 4 + 4 * 3 + 8 * 2 + 4 * 5,
+    # u, r (FP64), r32, u0_32, z32, sd32, rtemp32, utemp32 written
+    "mp_replace": 8 * 4 + 4 * 3 + 8 * 2 + 4 * 6,
+    # the same with z32, sd32, rtemp32, utemp32 left to the first inner pass (formed from r32):
+    # u, u0, Kx, Ky (FP64), u32 read; u, r (FP64), r32, u0_32 written
+    "mp_replace_r": 8 * 4 + 4 + 8 * 2 + 4 * 2,
     "calc_residual": 40, "calc_2norm": 8, "calc_2norm_f32": 4,
 }
 

@@ -293,6 +293,11 @@ int tea_engine_ppcg_inner_plan(int remaining);
  * the set-up and finalise calls always use the FP64 fields.  Chebyshev and
  * Jacobi calls are rejected in FP32 mode.  (64) switches back. */
 int tea_engine_set_precision(int bits);
+/* tea_engine_set_precision(64) at the end of an FP32 solve (no reference
+ * counterpart): the FP32 inner-solver fields (w, z, sd, rtemp, utemp) are
+ * left with unspecified values instead of being brought up to date, for a
+ * caller whose next FP32 solve writes them before reading them. */
+int tea_engine_mp_leave(void);
 /* FP32 copies of the FP64 face coefficients (after tea_leaf_common_init). */
 void tea_engine_mp_setup(void);
 /* r32 = (float) r (the FP64 residual field), u32 = 0 (the FP32 correction);

@@ -919,6 +919,9 @@ EXPORT int tea_engine_set_precision(int bits)
     return 0;
 }
 
+/* Nothing is owed here: every field is stored as it is formed. */
+EXPORT int tea_engine_mp_leave(void) { return tea_engine_set_precision(64); }
+
 /* FP32 copies of the face coefficients, whole arrays */
 EXPORT void tea_engine_mp_setup(void)
 {

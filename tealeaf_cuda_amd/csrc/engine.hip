@@ -1510,16 +1510,22 @@ void settle_minv_r(bool z, bool sd)
 // z = M^-1 r owed (only)
 void settle_z_r() { settle_minv_r(true, false); }
 
-// z owed (only): z = utemp, or z = M^-1 r
-void settle_z()
+// z = utemp owed (only): a z = M^-1 r debt stays owed
+void settle_z_utemp()
 {
-    settle_minv_r(true, false);
     const int bits = g.pending_z;
     g.pending_z = 0;
     if (bits == 32)
         launch("settle_z_f32", k_copy<float>, interior(), g.pitch32, g.fs32[X_Z], (const float *)g.fs32[X_UTEMP]);
     else if (bits == 64)
         launch("settle_z", k_copy<double>, interior(), g.pitch, g.fs[X_Z], (const double *)g.fs[X_UTEMP]);
+}
+
+// z owed (only): z = utemp, or z = M^-1 r
+void settle_z()
+{
+    settle_minv_r(true, false);
+    settle_z_utemp();
 }
 
 // rtemp and utemp owed (only; an sd they copy that is itself owed first)
@@ -2735,7 +2741,7 @@ EXPORT void tea_engine_ppcg_iteration(int with_p, double beta, double theta, int
 EXPORT void tea_engine_ppcg_inner_n(int step, int nsteps)
 {
     CHECK_INIT_KEEP_PENDING();   // a streaming pass takes rtemp and utemp still owed from r and sd
-    settle_z();                  // the pass rewrites utemp
+    settle_z_utemp();            // the pass rewrites utemp (r, which z = M^-1 r reads, it keeps)
     REQUIRE_SOLVER(1 << SOLVER_PPCG);
     const int k = step - 1;
     if (nsteps < 1 || nsteps > MAX_PASS_STEPS) {
@@ -2872,6 +2878,19 @@ EXPORT int tea_engine_set_precision(int bits)
     return 0;
 }
 
+// Leave the FP32 mode at the end of an FP32 solve: tea_engine_set_precision(64),
+// except that the FP32 inner-solver fields still owed (w, z, sd, rtemp,
+// utemp) are dropped unwritten.  The next FP32 solve writes each of them
+// before reading it; until then their FP32 copies hold unspecified values.
+EXPORT int tea_engine_mp_leave(void)
+{
+    if (g.initialised) {
+        for (int *debt : {&g.pending_w, &g.pending_z, &g.pending_zr, &g.pending_sd_r, &g.pending_rt_ut})
+            if (*debt == 32) *debt = 0;
+    }
+    return tea_engine_set_precision(64);
+}
+
 EXPORT void tea_engine_mp_setup(void)
 {
     CHECK_INIT();
@@ -2900,7 +2919,10 @@ EXPORT void tea_engine_mp_end(void)
 // mp_end, set_precision(64), calc_residual, calc_2norm(1) -> rr64,
 // set_precision(32), mp_begin -> rr32, ppcg_init step 2, init_sd_new(theta):
 // one pass (k_mp_replace, u' into the spare u buffer, which becomes u), the
-// u32 = 0 fill, one wait.  rtemp and utemp still owed need no writing.
+// u32 = 0 fill, one wait.  rtemp and utemp still owed need no writing.  With
+// block Jacobi and the inner passes forming sd from r, z, sd, rtemp and utemp
+// are left owed from r as a fused ur leaves them (pending_zr, pending_sd_r,
+// pending_rt_ut; TEA_SD_FROM_R=0 writes them here).
 EXPORT void tea_engine_mp_replace(double theta, double *rr64, double *rr32)
 {
     CHECK_INIT_KEEP_PENDING();
@@ -2915,17 +2937,23 @@ EXPORT void tea_engine_mp_replace(double theta, double *rr64, double *rr32)
         hipLaunchKernelGGL(k_copy_ring<double>, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, g.stream, F(X_U2),
                            (const double *)F(F_U), g.pitch, g.h, g.nx, g.ny);
     }
+    const bool owe = g.prec == PREC_BLOCK && sd_from_r<float>();
     auto go = [&](auto kernel) {
-        return launch("mp_replace", kernel, in, g.pitch, g.pitch32, (const double *)F(F_U), F(X_U2),
-                      (const double *)F(X_U0), F(F_R), (const double *)F(X_KX), (const double *)F(X_KY),
+        return launch(owe ? "mp_replace_r" : "mp_replace", kernel, in, g.pitch, g.pitch32, (const double *)F(F_U),
+                      F(X_U2), (const double *)F(X_U0), F(F_R), (const double *)F(X_KX), (const double *)F(X_KY),
                       (const float *)g.fs32[F_U], g.fs32[F_R], g.fs32[X_U0], g.fs32[X_Z], g.fs32[F_SD],
                       g.fs32[X_RTEMP], g.fs32[X_UTEMP], (const float *)g.fs32[X_MI], (const float *)g.fs32[X_KX],
                       (const float *)g.fs32[X_KY], theta, red_ctx());
     };
-    const bool ok = g.prec == PREC_BLOCK ? go(k_mp_replace<PREC_BLOCK>)
-                    : g.prec == PREC_DIAG ? go(k_mp_replace<PREC_DIAG>)
-                                          : go(k_mp_replace<PREC_NONE>);
+    const bool ok = owe ? go(k_mp_replace<PREC_BLOCK, true>)
+                    : g.prec == PREC_BLOCK ? go(k_mp_replace<PREC_BLOCK>)
+                    : g.prec == PREC_DIAG  ? go(k_mp_replace<PREC_DIAG>)
+                                           : go(k_mp_replace<PREC_NONE>);
     if (!ok || !finish(0, 2, nullptr)) return;
+    if (owe) {
+        g.pending_zr = g.pending_sd_r = g.pending_rt_ut = 32;
+        g.sd_r_theta = theta;
+    }
     std::swap(g.f[F_U], g.f[X_U2]);
     std::swap(g.fs[F_U], g.fs[X_U2]);
     if (!launch("mp_zero_u32", k_fill<float>, in, g.pitch32, g.fs32[F_U], 0.0f)) return;

@@ -2717,8 +2717,10 @@ template <typename T> __global__ __launch_bounds__(BLOCK) void k_fill(Ext e, lon
 // left to a following launch (u32 is read at those neighbours too).  Each lane
 // forms u' down its column (the strip and the rows above and below it) and
 // takes its x neighbours from the adjacent lanes; edge lanes form those
-// themselves.
-template <int PREC>
+// themselves.  OWE (block Jacobi): z32, sd32, rtemp32 and utemp32 are not
+// formed; the caller leaves them owed from r32, which the first inner pass
+// reads to form them with the same expressions.
+template <int PREC, bool OWE = false>
 __global__ __launch_bounds__(BLOCK) void k_mp_replace(
     Ext e, long pitch, long pitch32, const double *__restrict__ u, double *__restrict__ u_out,
     const double *__restrict__ u0, double *__restrict__ r, const double *__restrict__ Kx,
@@ -2766,13 +2768,14 @@ __global__ __launch_bounds__(BLOCK) void k_mp_replace(
             dd_add_prod(acc[1], (double)rv[k], (double)rv[k]);
         }
     }
-    if (PREC != PREC_NONE) precond_strip<float, PREC>(rv, zv, s.len, j0, pitch32, Mi32, Kx32, Ky32);
+    if (PREC != PREC_NONE && !OWE) precond_strip<float, PREC>(rv, zv, s.len, j0, pitch32, Mi32, Kx32, Ky32);
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
             const long j = j0 + k * pitch32;
             r32[j] = rv[k];
             u0_32[j] = rv[k];
+            if (OWE) continue;
             if (PREC != PREC_NONE) z32[j] = zv[k];
             const float sdv = (PREC != PREC_NONE ? zv[k] : rv[k]) * theta_r;
             sd32[j] = sdv;

@@ -83,6 +83,7 @@ struct Backend {
     void (*clear_error)(void) = nullptr;
     void (*synchronize)(void) = nullptr;
     int (*set_precision)(int) = nullptr;
+    int (*mp_leave)(void) = nullptr;
     void (*mp_setup)(void) = nullptr;
     void (*mp_begin)(double *) = nullptr;
     void (*mp_end)(void) = nullptr;
@@ -127,6 +128,7 @@ Backend load_backend(const char *path)
     b.clear_error = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_clear_error"));
     b.synchronize = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_synchronize"));
     b.set_precision = reinterpret_cast<int (*)(int)>(dlsym(b.handle, "tea_engine_set_precision"));
+    b.mp_leave = reinterpret_cast<int (*)(void)>(dlsym(b.handle, "tea_engine_mp_leave"));
     b.mp_setup = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_setup"));
     b.mp_begin = reinterpret_cast<void (*)(double *)>(dlsym(b.handle, "tea_engine_mp_begin"));
     b.mp_end = reinterpret_cast<void (*)(void)>(dlsym(b.handle, "tea_engine_mp_end"));
@@ -1431,9 +1433,10 @@ bool Run::reliable_step(double q, bool refresh_z, bool *replaced)
         b.tea_leaf_calc_2norm_kernel_cuda_(&which, &rr);
         b.check();
     }
-    auto leave_fp32 = [&] {
+    auto leave_fp32 = [&] {   // the solve ends: its FP32 inner fields are not read again
         if (!fused) return;
-        b.set_precision(64);
+        if (b.mp_leave) b.mp_leave();
+        else b.set_precision(64);
         rel.fp32_live = false;
     };
     rel.updates++;

@@ -125,6 +125,7 @@ _EXT = {
     "tea_engine_cg_begin_ur": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)], None),
     "tea_engine_set_precision": ([ctypes.c_int], ctypes.c_int),
+    "tea_engine_mp_leave": ([], ctypes.c_int),
     "tea_engine_mp_setup": ([], None),
     "tea_engine_mp_begin": ([_D], None),
     "tea_engine_mp_end": ([], None),
@@ -425,6 +426,11 @@ class Engine:
         if self.lib.tea_engine_set_precision(int(bits)) != 0:
             self._check()
             raise TeaError(f"set_precision({bits}) failed")
+
+    def mp_leave(self):
+        if self.lib.tea_engine_mp_leave() != 0:
+            self._check()
+            raise TeaError("mp_leave failed")
 
     def mp_setup(self):
         self._call("tea_engine_mp_setup")

@@ -19,7 +19,7 @@ import os
 import numpy as np
 import pytest
 
-from tealeaf_cuda_amd import Deck, Engine, StateSpec, parse_tea_out, random_layout, run_deck
+from tealeaf_cuda_amd import Deck, Engine, StateSpec, TeaError, parse_tea_out, random_layout, run_deck
 from tealeaf_cuda_amd import runtime as rt
 
 pytestmark = pytest.mark.gpu
@@ -450,6 +450,22 @@ def test_fp32_entry_points_bitwise(prec, oracle_so, oracle_env, tmp_path):
             _compare(g, c, "mp_replace")
             for e in (g, c):
                 e.set_precision(32)
+        # a replacement that ends the solve, then mp_leave: the FP64 state and
+        # the FP32 fields outside the inner solver (u32, r32, u0_32) as after
+        # set_precision(64); the FP32 mode is off
+        print(f"[gpu parity] fp32 mp_replace {prec} then mp_leave", flush=True)
+        for e in (g, c):
+            e.set_precision(32)
+        outs = [e.mp_replace(1.3) for e in (g, c)]
+        assert outs[0] == outs[1]
+        for e in (g, c):
+            e.mp_leave()
+        _compare(g, c, "mp_leave")
+        for fid in (rt.FIELD_U, rt.FIELD_R, rt.XFIELD_U0):
+            assert np.array_equal(g.get_field32(fid), c.get_field32(fid), equal_nan=True), f"mp_leave: {fid}"
+        for e in (g, c):
+            with pytest.raises(TeaError, match="FP32 solver mode"):
+                e.mp_replace(1.3)
     finally:
         g.finalize()
         c.finalize()

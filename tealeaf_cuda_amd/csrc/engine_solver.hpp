@@ -2495,7 +2495,9 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_wp(Ext e, long pitch, doub
 // (no reference counterpart as a kernel: the fused CG iteration's begin
 // leaves w unwritten), alpha = rro / pw formed on the device from begin's
 // p.w: u += alpha p; r -= alpha w; z = M^-1 r; sum r*z (r*r).  The same
-// expressions as cg_calc_w and cg_calc_ur, so the same values.
+// expressions as cg_calc_w and cg_calc_ur, so the same values.  jac_diag:
+// Mi = 1/diag is formed from the stencil's own diagonal (the expression of
+// tea_leaf_common.cuknl:141-145, which is how Mi was set) instead of read.
 template <typename T, int PREC>
 __global__ __launch_bounds__(BLOCK) void k_cg_ur_wp(Ext e, long pitch, double rro, const double *__restrict__ pw,
                                                     T *__restrict__ u, const T *__restrict__ p, T *__restrict__ r,
@@ -2514,7 +2516,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur_wp(Ext e, long pitch, double rr
         if (s.len > 0 && q <= s.len + 1) col[q] = p[i0 + (long)(q - 1) * pitch];
     }
     const bool rlane = lane < WAVE - 1 && c + 1 <= e.x1;
-    T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0};
+    T rv[STRIP] = {0, 0, 0, 0}, zv[STRIP] = {0, 0, 0, 0}, mi[STRIP] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         const T from_left = __shfl_up(col[k + 1], 1, WAVE);
@@ -2524,15 +2526,22 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur_wp(Ext e, long pitch, double rr
             const T left = lane > 0 ? from_left : p[i - 1];
             const T right = rlane ? from_right : p[i + 1];
             const T pc = col[k + 1];
-            const T wv = (T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i])) * pc
+            const T diag = T(1.0) + (Ky[i + pitch] + Ky[i]) + (Kx[i + 1] + Kx[i]);
+            const T wv = diag * pc
                          - (Ky[i + pitch] * col[k + 2] + Ky[i] * col[k])
                          - (Kx[i + 1] * right + Kx[i] * left);
+            if (PREC == PREC_DIAG) mi[k] = T(1.0) / diag;
             u[i] += alpha * pc;
             rv[k] = r[i] - alpha * wv;
             r[i] = rv[k];
         }
     }
-    if (PREC != PREC_NONE) precond_strip<T, PREC>(rv, zv, s.len, i0, pitch, Mi, Kx, Ky);
+    if (PREC == PREC_DIAG) {
+#pragma unroll
+        for (int k = 0; k < STRIP; k++) zv[k] = rv[k] * mi[k];
+    } else if (PREC != PREC_NONE) {
+        precond_strip<T, PREC>(rv, zv, s.len, i0, pitch, Mi, Kx, Ky);
+    }
 #pragma unroll
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {

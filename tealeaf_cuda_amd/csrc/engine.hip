@@ -1497,6 +1497,12 @@ void settle_minv_r(bool z, bool sd)
     if (z) g.pending_zr = 0;
     if (sd) g.pending_sd_r = 0;
     const int bits = zb ? zb : sb;
+    if (g.prec == PREC_DIAG) {   // z owed by a fused CG iteration (FP64; no sd)
+        if (zb == 64)
+            launch("settle_minv_r_diag", k_minv_r_diag<double>, interior(), g.pitch, (const double *)g.fs[F_R],
+                   g.fs[X_Z], (const double *)g.fs[X_KX], (const double *)g.fs[X_KY]);
+        return;
+    }
     if (bits == 32)
         launch("settle_minv_r_f32", k_minv_r<float>, interior(), g.pitch32, (const float *)g.fs32[F_R],
                zb ? g.fs32[X_Z] : (float *)nullptr, sb ? g.fs32[F_SD] : (float *)nullptr, g.sd_r_theta,
@@ -1987,7 +1993,7 @@ template <typename T> bool sd_from_r()
 // The streaming form of the fused begin (k_begin_march) unless
 // TEA_BEGIN_MARCH=0: bands of 62 columns, runs of rows sized for about eight
 // waves per SIMD.  Returns false (nothing launched) when it does not apply.
-template <typename T, bool STORE_R>
+template <typename T, bool STORE_R, bool Z_FROM_R = false>
 bool begin_march_launch(const char *name, int with_p, double beta, const double *beta_num, const T *p, T *pn,
                         const T *z, T *w, const T *r, T *rstore)
 {
@@ -2004,7 +2010,7 @@ bool begin_march_launch(const char *name, int with_p, double beta, const double 
     const int grid = (nbands * runs + MARCH_WAVES - 1) / MARCH_WAVES;
     if (grid > g.max_blocks) return false;
     g.last_grid = grid;
-    klaunch(name, k_begin_march<T, STORE_R>, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pitch_of<T>(), rows, with_p,
+    klaunch(name, k_begin_march<T, STORE_R, Z_FROM_R>, dim3(grid), dim3(MARCH_WAVES * WAVE), e, pitch_of<T>(), rows, with_p,
             beta, beta_num, p, pn, z, w, (const T *)FT<T>(X_KX), (const T *)FT<T>(X_KY), r, rstore, red_ctx());
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) return die("launch of %s failed: %s", name, hipGetErrorString(err));
@@ -2293,9 +2299,17 @@ EXPORT void tea_leaf_cg_calc_p_kernel_cuda_(double *beta)
 // not stored), alpha is divided on the device, and ur forms w = A p' again
 // from p' (one host wait, two passes instead of three).  w is then owed
 // (pending_w): the next fused call rewrites it, any other entry point writes
-// it out first.  FP32 mode runs the separate calls.
+// it out first.  FP32 mode runs the separate calls.  With jac_diag z = Mi r
+// is owed too (pending_zr): ur does not store it, and the next fused begin
+// forms it from r (zr) unless TEA_CG_Z_FROM_R=0.
 namespace {
-bool cg_begin_ur_f64(int with_p, double beta, double rro, double *pw, double *rrn)
+bool cg_z_from_r()
+{
+    const char *env = getenv("TEA_CG_Z_FROM_R"), *march = getenv("TEA_BEGIN_MARCH");
+    return g.prec == PREC_DIAG && !(env && env[0] == '0') && !(march && march[0] == '0');
+}
+
+bool cg_begin_ur_f64(int with_p, double beta, double rro, double *pw, double *rrn, bool zr)
 {
     const long pt = g.pitch;
     const long ring = 2L * (g.nx + 2L * g.h) * g.h + 2L * g.ny * g.h;
@@ -2306,24 +2320,36 @@ bool cg_begin_ur_f64(int with_p, double beta, double rro, double *pw, double *rr
         g.p_halo_dirty[0] = false;
     }
     const double *src = g.prec != PREC_NONE ? (const double *)F(X_Z) : (const double *)F(F_R);
-    if (!begin_march_launch<double, false>("cg_begin", with_p, beta, nullptr, (const double *)F(F_P), F(X_P2), src,
-                                           nullptr, nullptr, nullptr) &&
-        !launch("cg_begin", k_ppcg_begin<double>, interior(), pt, with_p, beta, (const double *)nullptr,
-                (const double *)F(F_P), F(X_P2), src, (double *)nullptr, (const double *)F(X_KX),
-                (const double *)F(X_KY), (const double *)nullptr, (double *)nullptr, red_ctx()))
-        return false;
+    bool begun = zr ? begin_march_launch<double, false, true>("cg_begin", with_p, beta, nullptr, (const double *)F(F_P),
+                                                              F(X_P2), (const double *)F(F_R), nullptr, nullptr,
+                                                              nullptr)
+                    : begin_march_launch<double, false>("cg_begin", with_p, beta, nullptr, (const double *)F(F_P),
+                                                        F(X_P2), src, nullptr, nullptr, nullptr);
+    if (!begun) {
+        if (zr) {   // the tile form reads z
+            g.pending_zr = 64;
+            settle_z_r();
+        }
+        if (!launch("cg_begin", k_ppcg_begin<double>, interior(), pt, with_p, beta, (const double *)nullptr,
+                    (const double *)F(F_P), F(X_P2), src, (double *)nullptr, (const double *)F(X_KX),
+                    (const double *)F(X_KY), (const double *)nullptr, (double *)nullptr, red_ctx()))
+            return false;
+    }
     std::swap(g.f[F_P], g.f[X_P2]);
     std::swap(g.fs[F_P], g.fs[X_P2]);
     if (!finish(0, 1, g.d_scal)) return false;
+    const bool owe_z = cg_z_from_r();
     auto go = [&](auto kernel) {
         return launch("cg_ur", kernel, interior(), pt, rro, (const double *)g.d_scal, F(F_U), (const double *)F(F_P),
-                      F(F_R), F(X_Z), (const double *)F(X_MI), (const double *)F(X_KX), (const double *)F(X_KY),
-                      red_ctx());
+                      F(F_R), owe_z ? (double *)nullptr : F(X_Z), (const double *)F(X_MI), (const double *)F(X_KX),
+                      (const double *)F(X_KY), red_ctx());
     };
     const bool ok = g.prec == PREC_BLOCK ? go(k_cg_ur_wp<double, PREC_BLOCK>)
                     : g.prec == PREC_DIAG ? go(k_cg_ur_wp<double, PREC_DIAG>)
                                           : go(k_cg_ur_wp<double, PREC_NONE>);
-    if (!ok || !finish(2, 1, nullptr)) return false;
+    if (!ok) return false;
+    g.pending_zr = owe_z ? 64 : 0;
+    if (!finish(2, 1, nullptr)) return false;
     g.pending_w = 64;
     if (!collect(pw, 0, 1)) return false;
     *rrn = g.h_out[2];
@@ -2343,8 +2369,11 @@ EXPORT void tea_engine_cg_begin_ur(int with_p, double beta, double rro, double *
         return;
     }
     g.pending_w = 0;   // rewritten below
+    // z = Mi r owed by the previous fused iteration: this begin forms it from r
+    const bool zr = g.pending_zr == 64 && g.prec == PREC_DIAG && cg_z_from_r();
+    if (zr) g.pending_zr = 0;
     settle_pending();
-    cg_begin_ur_f64(with_p, beta, rro, pw, rrn);
+    cg_begin_ur_f64(with_p, beta, rro, pw, rrn, zr);
 }
 
 namespace {

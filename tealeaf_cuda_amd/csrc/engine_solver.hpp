@@ -124,6 +124,22 @@ __global__ __launch_bounds__(BLOCK) void k_minv_r(Ext e, long pitch, const T *__
     STRIP_END
 }
 
+// z = M^-1 r with jac_diag, Mi = 1/diag formed from Kx and Ky with the
+// expression Mi was set with (tea_leaf_common.cuknl:141-145): the z a fused
+// CG iteration (k_cg_ur_wp) leaves unstored, written out for a reader (no
+// reference counterpart as a kernel; the same values).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_minv_r_diag(Ext e, long pitch, const T *__restrict__ r, T *__restrict__ z,
+                                                       const T *__restrict__ Kx, const T *__restrict__ Ky)
+{
+    STRIP_BEGIN
+    for (int k = 0; k < s.len; k++) {
+        const long i = i0 + k * pitch;
+        z[i] = r[i] * (T(1.0) / diag_A(Kx, Ky, i, pitch));
+    }
+    STRIP_END
+}
+
 // ---- CG ---------------------------------------------------------------------------
 
 // tea_leaf_cg.cuknl:5-39 (+ block_solve for jac_block): z = M^-1 r, p = z or r, sum r*p
@@ -1886,8 +1902,10 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_ppcg_inner_ringl(
 // and walks down it keeping p' of the rows above and below in registers, so
 // every field row is read once per band (k_ppcg_begin forms p' of a strip's
 // rows above and below from memory again).  Columns come from lane shifts.
+// Z_FROM_R (jac_diag): `z` holds r, and z = r * (1/diag) is formed on the fly
+// with the expressions k_cg_ur_wp forms it with.
 constexpr int BEGIN_OUT_X = WAVE - 2;
-template <typename T, bool STORE_R>
+template <typename T, bool STORE_R, bool Z_FROM_R = false>
 __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long pitch, int rows, int with_p,
                                                                    double beta_in, const double *__restrict__ beta_num,
                                                                    const T *__restrict__ p, T *__restrict__ pn,
@@ -1896,6 +1914,7 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long p
                                                                    const T *__restrict__ r, T *__restrict__ rstore,
                                                                    Red red)
 {
+    static_assert(!(Z_FROM_R && STORE_R), "r.z with z formed from r is not needed");
     const T beta = (T)(beta_num ? beta_num[0] / beta_in : beta_in);
     const int lane = threadIdx.x & (WAVE - 1);
     const int nbands = (e.x1 - e.x0 + BEGIN_OUT_X) / BEGIN_OUT_X;
@@ -1917,7 +1936,9 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long p
             if (!col_mem || row < e.y0 - 1 || row > e.y1 + 1) return T(0.0);
             const long i = (long)c + (long)row * pitch;
             const bool in = col_in && row >= e.y0 && row <= e.y1;
-            return (with_p && in) ? z[i] + beta * p[i] : p[i];
+            if (!(with_p && in)) return p[i];
+            const T zv = Z_FROM_R ? z[i] * (T(1.0) / diag_A(Kx, Ky, i, pitch)) : z[i];
+            return zv + beta * p[i];
         };
         auto kyv = [&](int row) -> T {   // Ky at (c, row) (rows y0 .. y1+1 are faces of the extent)
             return (col_in && row >= e.y0 && row <= e.y1 + 1) ? Ky[(long)c + (long)row * pitch] : T(0.0);
@@ -2497,7 +2518,8 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_ur_sd_wp(Ext e, long pitch, doub
 // p.w: u += alpha p; r -= alpha w; z = M^-1 r; sum r*z (r*r).  The same
 // expressions as cg_calc_w and cg_calc_ur, so the same values.  jac_diag:
 // Mi = 1/diag is formed from the stencil's own diagonal (the expression of
-// tea_leaf_common.cuknl:141-145, which is how Mi was set) instead of read.
+// tea_leaf_common.cuknl:141-145, which is how Mi was set) instead of read,
+// and z is not stored when z is null (the next begin forms it from r).
 template <typename T, int PREC>
 __global__ __launch_bounds__(BLOCK) void k_cg_ur_wp(Ext e, long pitch, double rro, const double *__restrict__ pw,
                                                     T *__restrict__ u, const T *__restrict__ p, T *__restrict__ r,
@@ -2546,7 +2568,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur_wp(Ext e, long pitch, double rr
     for (int k = 0; k < STRIP; k++) {
         if (k < s.len) {
             if (PREC != PREC_NONE) {
-                z[i0 + k * pitch] = zv[k];
+                if (z) z[i0 + k * pitch] = zv[k];
                 dd_add_prod(acc[0], (double)rv[k], (double)zv[k]);
             } else {
                 dd_add_prod(acc[0], (double)rv[k], (double)rv[k]);

@@ -109,7 +109,7 @@ def test_deck_bitwise(name, oracle_so, driver_built, oracle_env, tmp_path):
 @pytest.mark.parametrize("switch", ["TEA_FUSED_OUTER", "TEA_INNER_PASSES", "TEA_INNER_MARCH", "TEA_SPECULATE",
                                     "TEA_W_FROM_P", "TEA_FUSED_CG", "TEA_BEGIN_MARCH", "TEA_FUSED_CHEBY",
                                     "TEA_PPCG_ITERATION", "TEA_FUSED_END", "TEA_CHEBY_PASSES", "TEA_SD_FROM_R",
-                                    "TEA_END_STORE_R"])
+                                    "TEA_END_STORE_R", "TEA_CG_Z_FROM_R"])
 @pytest.mark.parametrize("name", ["ppcg_block", "ppcg_diag_h2", "mixed_ppcg_block", "random_layout_ppcg", "cg_diag",
                                   "cg_block_ragged", "cheby_none", "mixed_cheby_diag"])
 def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracle_env, monkeypatch, tmp_path):
@@ -129,7 +129,9 @@ def test_deck_bitwise_unfused_outer(name, switch, oracle_so, driver_built, oracl
     (TEA_CHEBY_PASSES=0), and a fused ur that stores z and sd itself instead of
     leaving sd to the first inner pass (TEA_SD_FROM_R=0), and the fused begin
     storing r and forming r.z itself instead of the end before it
-    (TEA_END_STORE_R=0), on the GPU give the same results as the oracle."""
+    (TEA_END_STORE_R=0), and the fused CG iteration with jac_diag storing z
+    instead of leaving it to the next begin (TEA_CG_Z_FROM_R=0), on the GPU
+    give the same results as the oracle."""
     oracle_env("dot2")
     monkeypatch.setenv(switch, "0")
     print(f"[gpu parity] unfused deck {name}", flush=True)
@@ -205,6 +207,13 @@ CALLS = {
     "fused_cg_begin_ur_then_norm_then_w": lambda e: (e.cg_begin_ur(True, 0.61, 0.8), e.calc_2norm(1),
                                                      e.cg_calc_w()),
     "fused_cg_begin_ur_twice": lambda e: (e.cg_begin_ur(False, 0.0, 0.8), e.cg_begin_ur(True, 0.3, 1.1)),
+    # with jac_diag z = Mi r is owed as well: the next fused begin forms it from r, a separate call
+    # (calc_p reads z; the norm of r.z) has it written out first
+    "fused_cg_begin_ur_three": lambda e: (e.cg_begin_ur(True, 0.61, 0.8), e.cg_begin_ur(True, 0.3, 1.1),
+                                          e.cg_begin_ur(True, 0.45, 0.9)),
+    "fused_cg_begin_ur_then_calc_p": lambda e: (e.cg_begin_ur(True, 0.61, 0.8), e.cg_calc_p(0.4)),
+    "fused_cg_begin_ur_then_norm_rz": lambda e: (e.cg_begin_ur(True, 0.61, 0.8), e.calc_2norm(2),
+                                                 e.cg_begin_ur(True, 0.3, 1.1)),
 }
 def _tile_kernels(fn):
     """Runs fn with the engine's inner-step passes on the tile kernels

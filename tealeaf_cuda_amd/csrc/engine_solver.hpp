@@ -2017,11 +2017,13 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_cheby_march(Ext e, long p
         const T kx0 = col_mem ? Kx[i] : T(0.0);
         const T kx1 = lane_from_above(kx0);
         const T left = lane_from_below(uc), right = lane_from_above(uc);
-        const T wv = (T(1.0) + (ky1 + ky0) + (kx1 + kx0)) * uc - (ky1 * up + ky0 * um) - (kx1 * right + kx0 * left);
+        const T diag = T(1.0) + (ky1 + ky0) + (kx1 + kx0);
+        const T wv = diag * uc - (ky1 * up + ky0 * um) - (kx1 * right + kx0 * left);
         if (out) {
             const T rv = u0[i] - wv;
             T pv;
-            if (PREC == PREC_DIAG) pv = alpha * p[i] + beta * Mi[i] * rv;
+            // Mi = 1/diag, the expression it was set with (tea_leaf_common.cuknl:141-145), not read
+            if (PREC == PREC_DIAG) pv = alpha * p[i] + beta * (T(1.0) / diag) * rv;
             else pv = alpha * p[i] + beta * rv;
             p[i] = pv;
             u_out[i] = uc + pv;

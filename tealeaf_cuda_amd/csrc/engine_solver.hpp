@@ -1945,27 +1945,67 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long p
         };
         T pm = pval(ya - 1), pc = pval(ya);
         T ky0 = kyv(ya);
-        for (int row = ya; row <= yb; row++) {
-            const long i = (long)c + (long)row * pitch;
-            const T pp = pval(row + 1);
-            const T ky1 = kyv(row + 1);
-            const T kx0 = col_mem ? Kx[i] : T(0.0);
-            const T kx1 = lane_from_above(kx0);   // Kx on the right face: the next column's
-            const T left = lane_from_below(pc), right = lane_from_above(pc);
-            const T wval = (T(1.0) + (ky1 + ky0) + (kx1 + kx0)) * pc - (ky1 * pp + ky0 * pm) - (kx1 * right + kx0 * left);
-            if (out) {
-                pn[i] = pc;
-                if (w) w[i] = wval;
-                dd_add_prod(acc[0], (double)pc, (double)wval);
-                if (STORE_R) {
-                    const T rv = r[i];
-                    rstore[i] = rv;
-                    dd_add_prod(acc[1], (double)rv, (double)z[i]);
+        if constexpr (Z_FROM_R) {
+            // The diagonal of the row ahead from registers: its Kx and the Ky
+            // of the row after it are loaded one row early and then serve
+            // that row's own stencil (no re-read).  The last lane reads the
+            // Kx of the column to its right itself.
+            T ky1 = kyv(ya + 1);
+            T kx0 = col_mem ? Kx[(long)c + (long)ya * pitch] : T(0.0);
+            for (int row = ya; row <= yb; row++) {
+                const long i = (long)c + (long)row * pitch, i1 = i + pitch;
+                const T ky2 = kyv(row + 2);
+                const T kxn = col_mem ? Kx[i1] : T(0.0);   // row + 1 <= y1 + 1: a halo row at most
+                const T kxn_r = lane_from_above(kxn);
+                const T kx1 = lane_from_above(kx0);
+                T pp = T(0.0);
+                if (col_mem && row + 1 <= e.y1 + 1) {
+                    if (with_p && col_in && row + 1 <= e.y1) {
+                        const T kr = lane == WAVE - 1 ? Kx[i1 + 1] : kxn_r;
+                        const T diag = T(1.0) + (ky2 + ky1) + (kr + kxn);
+                        pp = z[i1] * (T(1.0) / diag) + beta * p[i1];
+                    } else {
+                        pp = p[i1];
+                    }
                 }
+                const T left = lane_from_below(pc), right = lane_from_above(pc);
+                const T wval =
+                    (T(1.0) + (ky1 + ky0) + (kx1 + kx0)) * pc - (ky1 * pp + ky0 * pm) - (kx1 * right + kx0 * left);
+                if (out) {
+                    pn[i] = pc;
+                    if (w) w[i] = wval;
+                    dd_add_prod(acc[0], (double)pc, (double)wval);
+                }
+                pm = pc;
+                pc = pp;
+                ky0 = ky1;
+                ky1 = ky2;
+                kx0 = kxn;
             }
-            pm = pc;
-            pc = pp;
-            ky0 = ky1;
+        } else {
+            for (int row = ya; row <= yb; row++) {
+                const long i = (long)c + (long)row * pitch;
+                const T pp = pval(row + 1);
+                const T ky1 = kyv(row + 1);
+                const T kx0 = col_mem ? Kx[i] : T(0.0);
+                const T kx1 = lane_from_above(kx0);   // Kx on the right face: the next column's
+                const T left = lane_from_below(pc), right = lane_from_above(pc);
+                const T wval =
+                    (T(1.0) + (ky1 + ky0) + (kx1 + kx0)) * pc - (ky1 * pp + ky0 * pm) - (kx1 * right + kx0 * left);
+                if (out) {
+                    pn[i] = pc;
+                    if (w) w[i] = wval;
+                    dd_add_prod(acc[0], (double)pc, (double)wval);
+                    if (STORE_R) {
+                        const T rv = r[i];
+                        rstore[i] = rv;
+                        dd_add_prod(acc[1], (double)rv, (double)z[i]);
+                    }
+                }
+                pm = pc;
+                pc = pp;
+                ky0 = ky1;
+            }
         }
     }
     block_reduce<2>(acc, red);

@@ -1983,9 +1983,21 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long p
                 kx0 = kxn;
             }
         } else {
+            // STORE_R: z of the row ahead is loaded once, for p' there and for
+            // that row's r.z one row later (zc), not read again
+            T zc = (STORE_R && col_in) ? z[(long)c + (long)ya * pitch] : T(0.0);
             for (int row = ya; row <= yb; row++) {
                 const long i = (long)c + (long)row * pitch;
-                const T pp = pval(row + 1);
+                T pp, zn = T(0.0);
+                if constexpr (STORE_R) {
+                    const long i1 = i + pitch;
+                    const bool in1 = col_in && row + 1 <= e.y1;
+                    if (in1) zn = z[i1];
+                    pp = T(0.0);
+                    if (col_mem && row + 1 <= e.y1 + 1) pp = (with_p && in1) ? zn + beta * p[i1] : p[i1];
+                } else {
+                    pp = pval(row + 1);
+                }
                 const T ky1 = kyv(row + 1);
                 const T kx0 = col_mem ? Kx[i] : T(0.0);
                 const T kx1 = lane_from_above(kx0);   // Kx on the right face: the next column's
@@ -1999,12 +2011,13 @@ __global__ __launch_bounds__(MARCH_WAVES *WAVE) void k_begin_march(Ext e, long p
                     if (STORE_R) {
                         const T rv = r[i];
                         rstore[i] = rv;
-                        dd_add_prod(acc[1], (double)rv, (double)z[i]);
+                        dd_add_prod(acc[1], (double)rv, (double)zc);
                     }
                 }
                 pm = pc;
                 pc = pp;
                 ky0 = ky1;
+                zc = zn;
             }
         }
     }
